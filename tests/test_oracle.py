@@ -1,0 +1,337 @@
+"""Pins for the CPU f64 oracle (oracle/rt_oracle.cpp) before it is trusted as the
+parity judge of the HIP path.
+
+* End-to-end: byte equality with every current render the reference commits
+  (doc/images, regenerated by herebyfile.mjs:208-412 with the CLI at -s 500).
+* RNG: ChaCha block function against RFC 8439 vectors (run at 20 rounds), the
+  PCG32 XSH-RR output function against the pcg32 demo sequence, and the
+  reference's own disk-sampling invariant (src/lighting.rs:538-582).
+* Transforms, hex colours, outline and k-d consistency against the reference's
+  unit tests (src/scene.rs:274-308, src/lighting.rs:513-519, src/outline.rs:319-429,
+  tools/test_kdtree_consistency.rs).
+"""
+import json
+import math
+import os
+import tempfile
+
+import numpy as np
+import pytest
+from PIL import Image
+
+from conftest import REF_IMAGES, REF_SCENES
+from oracle import pyoracle as O
+
+# (golden image, scene file, CLI arguments) — herebyfile.mjs:208-412
+DOC_CASES = [
+    ("camera-basic", "camera-basic.json", {}),
+    ("camera-perspective", "camera-perspective.json", {}),
+    ("object-sphere", "object-sphere.json", {}),
+    ("object-plane-grid", "object-plane-grid.json", {}),
+    ("object-cube", "object-cube.json", {}),
+    ("object-mesh", "object-mesh.json", {}),
+    ("material-properties", "material-properties.json", {}),
+    ("material-reflectivity", "material-reflectivity.json", {}),
+    ("texture-grid-variations", "texture-grid-variations.json", {}),
+    ("checkerboard-basic", "checkerboard-basic.json", {}),
+    ("checkerboard-advanced", "checkerboard-advanced.json", {}),
+    ("lighting-multiple", "lighting-multiple.json", {}),
+    ("sampling-antialiasing", "sampling-antialiasing.json", {"samples": 4}),
+    ("sampling-antialiasing-nosamples", "sampling-antialiasing.json", {"anti_aliasing": "no-jitter"}),
+    ("example-complete", "example-complete.json", {}),
+    ("outline_complex", "outline_complex.json", {}),
+    ("outline-demo-basic", "outline_demo.json", {"anti_aliasing": "no-jitter"}),
+    ("outline-demo-complex", "outline_complex.json", {"anti_aliasing": "no-jitter"}),
+    ("outline-demo-no-outline", "outline_demo.json#no-outline", {"anti_aliasing": "no-jitter"}),
+    ("scene-backgrounds-1", "scene-backgrounds-1.json", {}),
+    ("scene-backgrounds-2", "scene-backgrounds-2.json", {}),
+    ("scene-fog-light", "scene-fog-light.json", {}),
+    ("scene-fog-heavy", "scene-fog-heavy.json", {}),
+    ("quincunx-demo", "quincunx-demo.json", {}),
+]
+
+
+def scene_path(spec, tmpdir):
+    """Resolve a case's scene; '#no-outline' deletes scene_settings.outline as the
+    reference's render:doc:outline-demo-no-outline task does."""
+    if spec.endswith("#no-outline"):
+        with open(os.path.join(REF_SCENES, spec.split("#")[0])) as f:
+            sc = json.load(f)
+        sc["scene_settings"].pop("outline", None)
+        p = os.path.join(tmpdir, "no_outline.json")
+        with open(p, "w") as f:
+            json.dump(sc, f)
+        return p
+    return os.path.join(REF_SCENES, spec)
+
+
+def load_png(name):
+    return np.asarray(Image.open(os.path.join(REF_IMAGES, name + ".png")).convert("RGB"))
+
+
+@pytest.mark.parametrize("name,scene,kw", DOC_CASES, ids=[c[0] for c in DOC_CASES])
+def test_doc_images_byte_exact(name, scene, kw, tmp_path):
+    path = scene_path(scene, str(tmp_path))
+    opts = O.cli_options(path, size=500, **kw)
+    gold = load_png(name)
+    assert (opts.height, opts.width) == gold.shape[:2]
+    rgb, _, _ = O.render(path, opts)
+    assert np.array_equal(rgb, gold), f"{np.count_nonzero((rgb != gold).any(-1))} pixels differ"
+
+
+def test_sampling_comparison_no_jitter_800x600():
+    path = os.path.join(REF_SCENES, "sampling-comparison.json")
+    opts = O.cli_options(path, size=1000, anti_aliasing="no-jitter")
+    assert (opts.width, opts.height) == (800, 600)
+    rgb, _, _ = O.render(path, opts)
+    assert np.array_equal(rgb, load_png("sampling-comparison-no-jitter"))
+
+
+def test_fog_demonstration_renderer_new_800x600():
+    rgb, _, _ = O.render(os.path.join(REF_SCENES, "fog_demonstration.json"), O.make_opts(800, 600))
+    assert np.array_equal(rgb, load_png("fog_demonstration_800x600"))
+
+
+def test_plus_front_kdtree_equals_brute_force():
+    """The reference's plus_front_{kdtree,brute_force}_800x600.png are byte-identical
+    twins; the oracle's k-d and brute-force renders must be identical too."""
+    path = os.path.join(REF_SCENES, "plus_front.json")
+    a, _, _ = O.render(path, O.make_opts(200, 150, use_kdtree=True))
+    b, _, _ = O.render(path, O.make_opts(200, 150, use_kdtree=False))
+    assert np.array_equal(a, b)
+    assert np.array_equal(load_png("plus_front_kdtree_800x600"), load_png("plus_front_brute_force_800x600"))
+
+
+# ---------------------------------------------------------------- RNG pins
+CONST = [0x61707865, 0x3320646E, 0x79622D32, 0x6B206574]
+
+
+def test_chacha20_rfc8439_vector_zero_key():
+    """RFC 8439 A.1 test vector #1 (all-zero key, nonce and counter) at 20 rounds."""
+    out = O.chacha_block(CONST + [0] * 12, 10)
+    expect = bytes.fromhex(
+        "76b8e0ada0f13d90405d6ae55386bd28bdd219b8a08ded1aa836efcc8b770dc7"
+        "da41597c5157488d7724e03fb8d84a376a43b8f41518a11cc387b669b2ee6586")
+    assert out.astype("<u4").tobytes() == expect
+
+
+def test_chacha20_rfc8439_section_2_3_2():
+    """RFC 8439 §2.3.2: key 00..1f, counter 1, nonce 000000090000004a00000000."""
+    key = list(np.frombuffer(bytes(range(32)), dtype="<u4"))
+    nonce = list(np.frombuffer(bytes.fromhex("000000090000004a00000000"), dtype="<u4"))
+    out = O.chacha_block(CONST + key + [1] + nonce, 10)
+    expect = bytes.fromhex(
+        "10f1e7e4d13b5915500fdd1fa32071c4c7d1f4c733c068030422aa9ac3d46c4e"
+        "d2826446079faa0914c2d705d98b02a2b5129cd1de164eb9cbd083e8a2503c4e")
+    assert out.astype("<u4").tobytes() == expect
+
+
+def test_chacha_round_count_matters():
+    """ChaCha12 (6 double rounds) differs from ChaCha20 on the same input."""
+    assert not np.array_equal(O.chacha_block(CONST + [0] * 12, 6), O.chacha_block(CONST + [0] * 12, 10))
+
+
+def test_pcg32_output_function_against_demo_sequence():
+    """rand_core 0.6 seed_from_u64 advances an LCG (MUL 6364136223846793005) and emits
+    PCG32 XSH-RR of the new state. The pcg32 demo (pcg32_srandom_r(42, 54)) emits
+    xsh_rr(S_k) for S_1 = 151*MUL + 109, S_{k+1} = S_k*MUL + 109; choosing the seed with
+    seed*MUL + INC == S_k makes the first key word equal the k-th demo output."""
+    MUL, INC, M64 = 6364136223846793005, 11634580027462260723, (1 << 64) - 1
+    inv = pow(MUL, -1, 1 << 64)
+    demo = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+    s = (151 * MUL + 109) & M64
+    for expect in demo:
+        seed = ((s - INC) * inv) & M64
+        assert int(O.seed_key(seed)[0]) == expect
+        s = (s * MUL + 109) & M64
+
+
+def test_gen_f64_recipe_and_range():
+    """gen::<f64>() = (next_u64 >> 11) * 2^-53 (rand 0.8 Standard)."""
+    u = O.stdrng_u64(12345, 64)
+    f = O.stdrng_f64(12345, 64)
+    assert np.array_equal(f, (u >> np.uint64(11)).astype(np.float64) * 2.0 ** -53)
+    assert (f >= 0).all() and (f < 1).all()
+
+
+def test_stdrng_streams_differ_by_seed_and_are_deterministic():
+    a = O.stdrng_u64(0, 40)
+    assert np.array_equal(a, O.stdrng_u64(0, 40))
+    assert not np.array_equal(a, O.stdrng_u64(1, 40))
+    # 40 u64 = 80 words crosses a 16-word block boundary several times
+    assert len(set(a.tolist())) == 40
+
+
+def test_sample_disk_point_inside_disk():
+    """src/lighting.rs:538-555: 100 samples from seed 42 lie inside a radius-2 disk."""
+    xy = O.sample_disk(42, 2.0, 100)
+    assert (np.hypot(xy[:, 0], xy[:, 1]) <= 2.0).all()
+    assert (np.abs(xy) < 2.0).all()
+
+
+# ------------------------------------------------------- scene / transforms
+def test_transform_order_scenario():
+    """src/scene.rs:274-308: rotate(0,0,180), translate(15,0,0), scale(8,8,8) maps (1,0,0) to (112,0,0)."""
+    m = O.parse_transforms(["rotate(0, 0, 180)", "translate(15, 0, 0)", "scale(8, 8, 8)"])
+    p = m @ np.array([1.0, 0.0, 0.0, 1.0])
+    assert abs(p[0] - 112.0) < 1e-10 and abs(p[1]) < 1e-10 and abs(p[2]) < 1e-10
+
+
+def test_transform_identity_and_translation():
+    assert np.array_equal(O.parse_transforms([]), np.eye(4))
+    t = np.eye(4)
+    t[:3, 3] = [1, 2, 3]
+    assert np.array_equal(O.parse_transforms(["translate(1, 2, 3)"]), t)
+
+
+@pytest.mark.parametrize("bad", ["invalid(1, 2, 3)", "rotate(1, 2, 3", "rotate(a, b, c)", "rotate(1, 2)"])
+def test_transform_errors(bad):
+    assert O.parse_transforms([bad]) is None
+
+
+def test_hex_to_color():
+    assert O.hex_to_color("#FF0000") == (1.0, 0.0, 0.0)
+    assert O.hex_to_color("#12345") is None
+    assert O.hex_to_color("#GG0000") is None
+
+
+# -------------------------------------------------------------- outline
+def _outline_opts(**kw):
+    d = dict(depth_weight=1.0, normal_weight=1.0, threshold=0.1, color=(0.0, 0.0, 0.0),
+             thickness=1.0, use_8_neighbors=False)
+    d.update(kw)
+    return O.make_opts(3, 3, aa="no-jitter", outline=d)
+
+
+def test_outline_edge_detection_simple():
+    """src/outline.rs:363-383: a depth discontinuity in the middle column edges the centre."""
+    depth = np.where(np.arange(3)[None, :] == 1, 10.0, 1.0) * np.ones((3, 1))
+    normal = np.zeros((3, 3, 3))
+    normal[..., 2] = 1
+    col = np.full((3, 3, 3), 0.5)
+    out = O.outline(depth, normal, col, _outline_opts(color=(1.0, 0.0, 0.0)))
+    assert out[1, 1, 0] > 0.5
+
+
+def test_outline_detection_integration():
+    """src/outline.rs:385-429."""
+    depth = np.where(np.arange(3)[None, :] == 1, 5.0, 1.0) * np.ones((3, 1))
+    normal = np.zeros((3, 3, 3))
+    normal[:, :, 2] = 1
+    normal[:, 1, :] = [1, 0, 0]
+    col = np.full((3, 3, 3), 0.5)
+    out = O.outline(depth, normal, col, _outline_opts(threshold=0.05, color=(1.0, 0.0, 0.0)))
+    assert out[1, 1, 0] > 0.5
+
+
+def test_outline_border_asymmetry():
+    """Right/bottom neighbours outside the image read as background (src/outline.rs:185-202,
+    get_depth bounds check :68-74), so a uniform foreground image edges only its
+    right column and bottom row."""
+    depth = np.ones((4, 5))
+    normal = np.zeros((4, 5, 3))
+    normal[..., 2] = 1
+    out = O.outline(depth, normal, np.ones((4, 5, 3)), _outline_opts())
+    edged = out[..., 0] < 1.0
+    expect = np.zeros((4, 5), bool)
+    expect[:, -1] = True
+    expect[-1, :] = True
+    assert np.array_equal(edged, expect)
+
+
+# ------------------------------------------------------------ k-d tree
+def _plus_rays(bmin, bmax):
+    """tools/test_kdtree_consistency.rs:102-187 (214 rays)."""
+    center = (bmin + bmax) / 2.0
+    size = bmax - bmin
+    me = max(size[0], size[1], size[2])
+    cx, cy, cz = center
+    P = [(bmin[0] - me, cy, cz), (bmax[0] + me, cy, cz), (cx, bmin[1] - me, cz), (cx, bmax[1] + me, cz),
+         (cx, cy, bmin[2] - me), (cx, cy, bmax[2] + me),
+         (bmin[0] - me, cy + 0.1, cz + 0.1), (bmax[0] + me, cy - 0.1, cz - 0.1),
+         (cx + 0.1, bmin[1] - me, cz + 0.1), (cx - 0.1, bmax[1] + me, cz - 0.1),
+         (cx + 0.1, cy + 0.1, bmin[2] - me), (cx - 0.1, cy - 0.1, bmax[2] + me)]
+    D = [(1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+         (1, 1e-9, 1e-9), (-1, 1e-9, -1e-9), (1e-9, 1, 1e-9), (-1e-9, -1, 1e-9), (1e-9, 1e-9, 1), (-1e-9, -1e-9, -1)]
+    rays = [(p, d) for p in P for d in D]
+    for i in range(20):
+        th = 2 * math.pi * i / 20
+        ph = math.pi * 0.5
+        d = np.array([math.sin(th) * math.cos(ph), math.cos(th) * math.cos(ph), math.sin(ph)])
+        rays.append((center - d * me * 2.0, d))
+    for i in range(50):
+        a1 = 2 * math.pi * (i / 50.0)
+        a2 = math.pi * ((i * 3) % 50) / 50.0
+        o = np.array([cx + math.sin(a1) * me * 1.5, cy + math.cos(a2) * me * 1.5, cz + math.sin(a1 + a2) * me * 1.5])
+        rays.append((o, center - o))
+    return rays
+
+
+def test_kdtree_consistency_plus_stl():
+    path = os.path.join(REF_SCENES, "plus.stl")
+    raw = open(path, "rb").read()
+    n = int.from_bytes(raw[80:84], "little")
+    rec = np.frombuffer(raw[84:84 + 50 * n], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+    v = rec["v"].astype(np.float64).reshape(-1, 3)
+    rays = _plus_rays(v.min(0), v.max(0))
+    assert len(rays) == 214
+    hits = 0
+    for o, d in rays:
+        a = O.mesh_hit(path, True, o, d)
+        b = O.mesh_hit(path, False, o, d)
+        # the tool compares t / point / normal (1e-6), not which of two coplanar
+        # triangles sharing the hit point won the tie
+        assert (a[0] >= 0) == (b[0] >= 0)
+        if a[0] >= 0:
+            assert abs(a[1] - b[1]) <= 1e-6
+        hits += a[0] >= 0
+    assert hits > 50
+
+
+def test_kdtree_build_stats():
+    """K-d build (src/mesh.rs:75-101): leaves hold <= 10 triangles unless depth 16 is reached."""
+    s = O.kd_stats(os.path.join(REF_SCENES, "icosahedron.stl"))
+    assert s["tris"] == 20 and s["leaves"] >= 2
+    s = O.kd_stats(os.path.join(REF_SCENES, "plus.stl"))
+    assert s["tris"] == 1568 and s["nodes"] == 2 * s["leaves"] - 1
+
+
+# ------------------------------------------------------ renderer semantics
+def _tiny_scene(tmpdir, diameter=0.5):
+    sc = {
+        "camera": {"kind": "ortho", "position": [0, -5, 2], "target": [0, 0, 0], "up": [0, 0, 1],
+                   "width": 10, "height": 10},
+        "objects": [{"kind": "sphere", "center": [0, 0, 0], "radius": 1.0,
+                     "material": {"color": "#FFFFFF", "ambient": 0.1, "diffuse": 0.7, "specular": 0.3,
+                                  "shininess": 32}}],
+        "lights": [{"position": [2, 2, 2], "color": "#FFFFFF", "intensity": 1.0, "diameter": diameter}],
+        "scene_settings": {"ambient_illumination": {"color": "#FFFFFF", "intensity": 0.1},
+                           "background_color": "#000000"},
+    }
+    p = os.path.join(tmpdir, "tiny.json")
+    with open(p, "w") as f:
+        json.dump(sc, f)
+    return p
+
+
+def test_zero_samples_error(tmp_path):
+    """src/renderer.rs:1220-1240."""
+    p = _tiny_scene(str(tmp_path))
+    with pytest.raises(RuntimeError, match="Samples must be greater than 0"):
+        O.render(p, O.make_opts(10, 10, samples=0))
+
+
+def test_thread_count_independence(tmp_path):
+    """src/renderer.rs:1119-1171: stochastic 4 spp, area light, seed 42, 1 vs 4 threads."""
+    p = _tiny_scene(str(tmp_path))
+    a, _, _ = O.render(p, O.make_opts(50, 50, aa="stochastic", samples=4, seed=42, threads=1))
+    b, _, _ = O.render(p, O.make_opts(50, 50, aa="stochastic", samples=4, seed=42, threads=4))
+    assert np.array_equal(a, b)
+    assert a.any()
+
+
+def test_quincunx_ray_count(tmp_path):
+    """Quincunx shares corners: (W+1)(H+1) + W*H primary rays (src/renderer.rs:768-886)."""
+    p = _tiny_scene(str(tmp_path))
+    _, st, _ = O.render(p, O.make_opts(40, 30))
+    assert st["rays_primary"] == 41 * 31 + 40 * 30

@@ -1,0 +1,166 @@
+"""Benchmark: Mrays/s of the per-pixel trace path at 1080p on one MI355X.
+
+Workload (default `config2_1080p`): the procedural mesh scene of configuration 2
+(81,920-triangle noisy icosphere with a host-built k-d tree, 3 transformed cubes, a
+reflective grid plane, a disk area light with 16 shadow samples per hit, fog 0.05,
+reflections to depth 4) rendered at 1920x1080 with quincunx anti-aliasing. A step is one
+full frame (camera rays, closest-hit, shading, shadow rays, reflections, resolve,
+quantise) rendered into a device buffer; the scene is resident in HBM before timing.
+
+Rays = primary + reflection + shadow rays, counted exactly by the pipeline.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per unit of traversal work (layout in rtrace_amd/csrc/common/layout.hpp)
+B_NODE, B_TRI, B_RETEST = 16, 48, 72
+B_SPHERE, B_PLANE, B_CUBE = 32, 48, 304
+B_RAY_CLOSEST = 64 + 16  # RayRec in, HitRec out
+B_RAY_SHADOW = 48 + 24 + 1  # hit point + normal, light sample point in, visibility byte out
+
+
+def kernel_bytes(work, rays, per_ray):
+    return (work["nodes"] * B_NODE + work["tris"] * B_TRI + work["retests"] * B_RETEST +
+            work["spheres"] * B_SPHERE + work["planes"] * B_PLANE + work["cubes"] * B_CUBE + rays * per_ray)
+
+
+def make_workload(name, out_dir):
+    from rtrace_amd import scenes
+    if name.startswith("config2"):
+        path = scenes.config2(out_dir, subdiv=6)
+    elif name.startswith("config3"):
+        path = scenes.config3(out_dir, 1000)
+    elif name.startswith("config1"):
+        path = scenes.config1(out_dir)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return path, scenes.RENDER[name]
+
+
+def configure(R, renderer, cfg, scene):
+    renderer.max_depth = cfg["max_depth"]
+    renderer.samples = cfg["samples"]
+    renderer.anti_aliasing_mode = {"quincunx": R.AntiAliasingMode.Quincunx, "stochastic": R.AntiAliasingMode.Stochastic,
+                                   "no-jitter": R.AntiAliasingMode.NoJitter}[cfg["aa"]]
+    if cfg["aa"] != "quincunx":
+        renderer.outline_config = scene.info()["outline"]
+
+
+def cpu_baseline(path, cfg, budget_s=20.0):
+    """The CPU f64 oracle (a restatement of the reference renderer) on the box's host
+    cores, on a reduced-resolution render of the same scene and settings."""
+    from oracle import pyoracle as O
+    threads = min(16, os.cpu_count() or 1)
+    w, h = 64, 36
+    with open(path) as f:
+        scene = json.load(f)
+    outline = O.outline_from_scene(scene) if cfg["aa"] != "quincunx" else None
+    spent = 0.0
+    best = None
+    while True:
+        opts = O.make_opts(w, h, max_depth=cfg["max_depth"], samples=cfg["samples"], aa=cfg["aa"], threads=threads,
+                           outline=outline)
+        _, st, _ = O.render(path, opts)
+        rays = st["rays_primary"] + st["rays_reflect"] + st["rays_shadow"]
+        best = (rays / st["render_seconds"] / 1e6, w, h, st["render_seconds"], rays)
+        spent += st["render_seconds"]
+        if st["render_seconds"] * 4 > budget_s - spent or w >= cfg["width"]:
+            break
+        w, h = w * 2, h * 2
+    return {"value": best[0], "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle render of the same scene at {best[1]}x{best[2]} ({best[4]} rays, {best[3]:.1f} s, "
+                      f"{threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="config2_1080p")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the trace path has no multi-GPU mode in this build (asked for {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import rtrace_amd as R
+
+    out_dir = os.path.join(tempfile.gettempdir(), "rtrace_amd_bench")
+    path, cfg = make_workload(args.workload, out_dir)
+    scene = R.Scene.from_json_file(path)
+    W, H = cfg["width"], cfg["height"]
+    r = R.Renderer(W, H)
+    configure(R, r, cfg, scene)
+    out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
+
+    # untimed passes: exact work counters, then per-stage hipEvent timing
+    r.count_work = True
+    st_work = r.render_device(scene, out)
+    r.count_work = False
+    r.profile = True
+    st_prof = r.render_device(scene, out)
+    r.profile = False
+
+    for _ in range(args.warmup):
+        r.render_device(scene, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = []
+    for _ in range(args.steps):
+        st = r.render_device(scene, out)
+        dev_ms.append(st.ms_render)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    rays = st.rays_total
+    value = rays / (ms_per_step * 1e-3) / 1e6
+
+    # roofline of the dominant traversal kernel (bytes from the counters, time from events)
+    prof = st_prof.as_dict()
+    wk = st_work.as_dict()["work"]
+    kernels = {
+        "k_shadow": (kernel_bytes(wk["shadow"], st_work.rays_shadow, B_RAY_SHADOW), prof["ms_stage"]["shadow"],
+                     prof["launches_stage"]["shadow"]),
+        "k_closest": (kernel_bytes(wk["closest"], st_work.rays_primary + st_work.rays_reflect, B_RAY_CLOSEST),
+                      prof["ms_stage"]["closest"], prof["launches_stage"]["closest"]),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    byt, ms, launches = kernels[dom]
+    achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    line = {
+        "metric": "Mrays/s at 1080p", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+f64",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": args.workload, "width": W, "height": H, "aa": cfg["aa"], "samples": cfg["samples"],
+                   "max_depth": cfg["max_depth"], "triangles": 81920 if args.workload.startswith("config2") else 0,
+                   "rays_per_frame": rays, "primary": st.rays_primary, "reflect": st.rays_reflect,
+                   "shadow": st.rays_shadow},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_per_launch": int(byt / max(launches, 1)),
+                     "ms_per_launch": round(ms / max(launches, 1), 4)},
+        "stages_ms": {k: round(v, 3) for k, v in prof["ms_stage"].items()},
+        "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
+        "work": wk,
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(path, cfg)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,222 @@
+/*
+ * rtrace_hip.h — C ABI of the MI355X (gfx950) per-pixel trace path.
+ *
+ * Drop-in boundary for rtrace's `Renderer::render(&self, &Scene) -> Result<RgbImage, _>`
+ * (src/renderer.rs:120-382) and `render_to_file` (src/renderer.rs:914-923). Plain C types
+ * only; no exception crosses this boundary; every call returns RT_OK (0) or a negative
+ * RT_E_* code and leaves a message in rt_last_error().
+ *
+ * Threading: a context is used by one host thread at a time; calls block until results
+ * are in caller memory (the reference's render call also blocks). Output is independent
+ * of launch shape, as the reference's is independent of thread count
+ * (src/renderer.rs:1119-1171).
+ */
+#ifndef RTRACE_HIP_H
+#define RTRACE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirroring the reference's error cases) ---- */
+enum {
+    RT_OK = 0,
+    RT_E_INVALID_ARG = -1, /* e.g. "Samples must be greater than 0" (src/renderer.rs:122-124) */
+    RT_E_SCENE = -2,       /* scene JSON / serde errors (src/scene.rs:462-480) */
+    RT_E_IO = -3,          /* file and STL errors (src/mesh.rs:751-1003) */
+    RT_E_CAMERA = -4,      /* bad FOV / camera kind / hex colour (src/camera.rs:62,115-117; src/scene.rs:15-30) */
+    RT_E_DEVICE = -5,      /* no usable gfx950 device or a HIP runtime failure */
+    RT_E_OOM = -6,         /* device allocation failed */
+    RT_E_INTERNAL = -99
+};
+
+/* Renderer.anti_aliasing_mode (src/renderer.rs:17-24) */
+enum { RT_AA_NO_JITTER = 0, RT_AA_QUINCUNX = 1, RT_AA_STOCHASTIC = 2 };
+
+/* Object kinds (src/scene.rs:314-344) */
+enum { RT_OBJ_SPHERE = 0, RT_OBJ_PLANE = 1, RT_OBJ_CUBE = 2, RT_OBJ_MESH = 3 };
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_scene rt_scene;
+
+/* ---- flattened scene (what a host that already parsed the Scene hands over) ---- */
+
+/* Camera config (src/scene.rs:34-46); the basis is built per render from the aspect
+ * ratio as Camera::from_config does (src/camera.rs:23-152). */
+typedef struct {
+    int32_t kind; /* 0 = "ortho", 1 = "perspective" */
+    double position[3], target[3], up[3];
+    double width, height;
+    int32_t has_fov;
+    double fov;
+    int32_t has_grid; /* grid_pitch, grid_color and grid_thickness all present */
+    double grid_pitch, grid_thickness, grid_color[3];
+} rt_camera_desc;
+
+/* Material (src/scene.rs:67-105) with colours already parsed. */
+typedef struct {
+    double color[3];
+    double ambient, diffuse, specular, shininess;
+    int32_t has_reflectivity;
+    double reflectivity;
+    int32_t texture; /* 0 none, 1 grid, 2 checkerboard */
+    /* grid: the line colour after the reference's "#{:02X}" round trip (src/lighting.rs:28-36) */
+    double grid_line_color[3];
+    double line_width, cell_size;
+    int32_t material_b; /* checkerboard: index of the alternate material in the table */
+} rt_material_desc;
+
+/* One World object in scene order (object index == material index, src/renderer.rs:141),
+ * transforms already baked as World assembly does (src/renderer.rs:137-310). */
+typedef struct {
+    int32_t kind;
+    int32_t material;
+    double center[3];     /* sphere, cube */
+    double radius;        /* sphere */
+    double point[3];      /* plane */
+    double normal[3];     /* plane, unit */
+    double half_size[3];  /* cube */
+    double world_to_local[16]; /* cube: M^-1, column-major (identity when untransformed) */
+    double local_to_world[16]; /* cube: M, column-major */
+    int32_t mesh;         /* mesh: index into rt_scene_desc.meshes */
+} rt_object_desc;
+
+/* Light (src/scene.rs:348-353); is_area when `diameter` is Some. */
+typedef struct {
+    double position[3];
+    double color[3];
+    double intensity;
+    int32_t is_area;
+    double diameter;
+} rt_light_desc;
+
+/* Triangle mesh, vertices in f64 after the object's transform (src/renderer.rs:270-296).
+ * The k-d tree is built on the host from these with the reference algorithm
+ * (src/mesh.rs:75-197) and flattened to device arrays. */
+typedef struct {
+    int32_t n_tris;
+    const double* vertices; /* n_tris * 9 */
+} rt_mesh_desc;
+
+typedef struct {
+    rt_camera_desc camera;
+    int32_t n_objects;
+    const rt_object_desc* objects;
+    int32_t n_materials;
+    const rt_material_desc* materials;
+    int32_t n_lights;
+    const rt_light_desc* lights;
+    int32_t n_meshes;
+    const rt_mesh_desc* meshes;
+    double ambient_color[3];
+    double ambient_intensity;
+    int32_t has_fog;
+    double fog_color[3], fog_density, fog_start, fog_end;
+    double background[3];
+} rt_scene_desc;
+
+/* Mirrors the Renderer fields (src/renderer.rs:37-47) plus backend knobs. */
+typedef struct {
+    int32_t width, height;
+    int32_t max_depth;   /* default 10 */
+    int32_t use_kdtree;  /* default 1; 0 = brute-force mesh intersection */
+    int32_t samples;     /* default 1; 0 is rejected like the reference */
+    int32_t aa_mode;     /* RT_AA_* (default quincunx) */
+    int32_t has_seed;    /* Renderer.seed = Some(seed) */
+    uint64_t seed;
+    /* Renderer.outline_config (src/outline.rs:5-18) */
+    int32_t outline_enabled;
+    int32_t outline_use_8_neighbors;
+    double outline_depth_weight, outline_normal_weight, outline_threshold;
+    double outline_color[3];
+    double outline_thickness;
+    /* backend knobs */
+    int32_t count_work;  /* 1 = also collect traversal counters into rt_render_stats */
+    int32_t profile;     /* 1 = time each pipeline stage with hipEvents (rt_render_stats.ms_stage) */
+} rt_render_opts;
+
+/* Pipeline stages timed when rt_render_opts.profile is set. */
+enum {
+    RT_STAGE_RAYGEN = 0,   /* counters init + camera rays */
+    RT_STAGE_CLOSEST = 1,  /* closest-hit traversal, all levels */
+    RT_STAGE_SHADE = 2,    /* exact hit records, light samples, reflection queue */
+    RT_STAGE_SHADOW = 3,   /* any-hit shadow traversal, all levels */
+    RT_STAGE_FINISH = 4,   /* Phong + fog per hit */
+    RT_STAGE_POST = 5,     /* path combine, AA resolve, outline, quantise */
+    RT_STAGE_COUNT = 6
+};
+
+/* Work counters of the two traversal kernels (count_work). */
+enum { RT_KERNEL_CLOSEST = 0, RT_KERNEL_SHADOW = 1 };
+
+typedef struct {
+    uint64_t nodes;    /* k-d nodes fetched (16 B each) */
+    uint64_t tris;     /* FP32 triangle tests (48 B record each) */
+    uint64_t retests;  /* FP64 exact triangle re-tests (72 B of f64 vertices each) */
+    uint64_t spheres;  /* sphere tests */
+    uint64_t planes;   /* plane tests */
+    uint64_t cubes;    /* cube tests */
+} rt_work;
+
+typedef struct {
+    uint64_t rays_primary;  /* camera rays traced (quincunx: centres + unique corners) */
+    uint64_t rays_reflect;  /* reflection rays traced */
+    uint64_t rays_shadow;   /* shadow rays traced */
+    rt_work work[2];        /* per traversal kernel (RT_KERNEL_*), when count_work */
+    double ms_render;       /* device time of the render, hipEvents */
+    double ms_upload;       /* host->device copies of the last scene upload */
+    double ms_stage[RT_STAGE_COUNT]; /* per-stage device time, when profile */
+    int32_t launches_stage[RT_STAGE_COUNT];
+    int32_t levels;         /* trace levels launched (max_depth) */
+} rt_render_stats;
+
+/* ---- context ---- */
+int rt_ctx_create(int device, rt_ctx** out);
+void rt_ctx_destroy(rt_ctx* ctx);
+/* Message of the last failed call on ctx (thread-local when ctx is NULL). */
+const char* rt_last_error(rt_ctx* ctx);
+
+/* ---- scenes ---- */
+/* Scene::from_json_file (src/scene.rs:462-470) + World assembly; meshes resolve
+ * relative to the scene file. */
+int rt_scene_load_json_file(rt_ctx* ctx, const char* path, rt_scene** out);
+/* Scene::from_json_str (src/scene.rs:473-480); meshes resolve relative to base_dir
+ * (NULL = current directory, as the reference). */
+int rt_scene_load_json_str(rt_ctx* ctx, const char* json, const char* base_dir, rt_scene** out);
+/* Upload a scene a host already flattened (the Rust `Renderer` would build this from
+ * its `World`). The library copies what it needs; desc may be freed afterwards. */
+int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out);
+void rt_scene_free(rt_scene* scene);
+/* Camera aspect width/height and the scene_settings.outline block, for CLI glue
+ * (cli/src/main.rs:79-124). outline_* are filled only when enabled. */
+int rt_scene_info(rt_scene* scene, double* cam_width, double* cam_height, int32_t* outline_enabled,
+                  rt_render_opts* outline_out);
+
+/* Defaults of Renderer::new(width, height) (src/renderer.rs:50-62). */
+void rt_render_opts_default(rt_render_opts* opts, int32_t width, int32_t height);
+
+/* ---- render ---- */
+/* Renderer::render: writes width*height*3 u8 (row-major RGB) to rgb_out (host memory). */
+int rt_render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, uint8_t* rgb_out,
+              rt_render_stats* stats);
+/* Same, into a device buffer (no host copy); the call still blocks until done. */
+int rt_render_device(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
+                     rt_render_stats* stats);
+/* Primary-ray hit records of the last render on ctx, in canonical order
+ * (quincunx: [W*H centres][(W+1)*(H+1) corners]; else pixel-major, sample-minor):
+ * object index (-1 miss), triangle index (-1 non-mesh), t. n = capacity. */
+int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, size_t n);
+
+/* ---- ray queries (the `World::hit` seam, src/ray.rs:699-711) ---- */
+/* Closest hit of n rays (origins/dirs: n*3 f64; dirs are normalized as Ray::new does). */
+int rt_intersect(rt_ctx* ctx, rt_scene* scene, const double* origins, const double* dirs, size_t n,
+                 double t_min, double t_max, int use_kdtree, int32_t* obj_out, int32_t* tri_out,
+                 double* t_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTRACE_HIP_H */

@@ -1,0 +1,291 @@
+"""rtrace_amd — MI355X (gfx950) backend for rtrace's per-pixel trace path.
+
+Python mirror of the reference's render API over the C ABI in include/rtrace_hip.h:
+
+    scene = Scene.from_json_file("scene.json")          # src/scene.rs:462-470
+    r = Renderer(800, 600)                               # src/renderer.rs:50-62
+    r.anti_aliasing_mode = AntiAliasingMode.Stochastic
+    image = r.render(scene)                              # (H, W, 3) uint8, src/renderer.rs:120
+    r.render_to_file(scene, "out.png")                   # src/renderer.rs:914-923
+
+There is no CPU fallback: every render runs the HIP kernels in librtrace_hip.so and
+fails loudly when the library or a gfx950 device is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["AntiAliasingMode", "OutlineConfig", "Renderer", "Scene", "RenderStats", "RtraceError", "lib",
+           "intersect", "last_primary_hits", "cli_size"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtrace_hip.so")
+
+
+class RtraceError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+class AntiAliasingMode:
+    """src/renderer.rs:17-24"""
+    NoJitter = 0
+    Quincunx = 1
+    Stochastic = 2
+
+
+class _Opts(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_int32), ("height", ctypes.c_int32), ("max_depth", ctypes.c_int32),
+        ("use_kdtree", ctypes.c_int32), ("samples", ctypes.c_int32), ("aa_mode", ctypes.c_int32),
+        ("has_seed", ctypes.c_int32), ("seed", ctypes.c_uint64),
+        ("outline_enabled", ctypes.c_int32), ("outline_use_8_neighbors", ctypes.c_int32),
+        ("outline_depth_weight", ctypes.c_double), ("outline_normal_weight", ctypes.c_double),
+        ("outline_threshold", ctypes.c_double), ("outline_color", ctypes.c_double * 3),
+        ("outline_thickness", ctypes.c_double), ("count_work", ctypes.c_int32), ("profile", ctypes.c_int32),
+    ]
+
+
+STAGES = ["raygen", "closest", "shade", "shadow", "finish", "post"]
+KERNEL_CLOSEST, KERNEL_SHADOW = 0, 1
+
+
+class Work(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("nodes", "tris", "retests", "spheres", "planes", "cubes")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RenderStats(ctypes.Structure):
+    _fields_ = [
+        ("rays_primary", ctypes.c_uint64), ("rays_reflect", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64),
+        ("work", Work * 2), ("ms_render", ctypes.c_double), ("ms_upload", ctypes.c_double),
+        ("ms_stage", ctypes.c_double * 6), ("launches_stage", ctypes.c_int32 * 6), ("levels", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k in ("rays_primary", "rays_reflect", "rays_shadow", "ms_render", "ms_upload",
+                                            "levels")}
+        d["work"] = {"closest": self.work[0].as_dict(), "shadow": self.work[1].as_dict()}
+        d["ms_stage"] = dict(zip(STAGES, list(self.ms_stage)))
+        d["launches_stage"] = dict(zip(STAGES, list(self.launches_stage)))
+        return d
+
+    @property
+    def rays_total(self):
+        return self.rays_primary + self.rays_reflect + self.rays_shadow
+
+
+_lib = None
+_ctx = None
+
+
+def lib():
+    """Load librtrace_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RtraceError(-5, f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        L.rt_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(P)]
+        L.rt_ctx_destroy.argtypes = [P]
+        L.rt_last_error.argtypes = [P]
+        L.rt_last_error.restype = ctypes.c_char_p
+        L.rt_scene_load_json_file.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(P)]
+        L.rt_scene_load_json_str.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(P)]
+        L.rt_scene_upload.argtypes = [P, P, ctypes.POINTER(P)]
+        L.rt_scene_free.argtypes = [P]
+        L.rt_scene_info.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(_Opts)]
+        L.rt_render_opts_default.argtypes = [ctypes.POINTER(_Opts), ctypes.c_int32, ctypes.c_int32]
+        L.rt_render.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
+        L.rt_render_device.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
+        L.rt_last_primary_hits.argtypes = [P, P, P, P, ctypes.c_size_t]
+        L.rt_intersect.argtypes = [P, P, P, P, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                   P, P, P]
+        _lib = L
+    return _lib
+
+
+def _context():
+    global _ctx
+    if _ctx is None:
+        h = ctypes.c_void_p()
+        rc = lib().rt_ctx_create(0, ctypes.byref(h))
+        if rc != 0:
+            raise RtraceError(rc, lib().rt_last_error(None).decode())
+        _ctx = h
+    return _ctx
+
+
+def _check(rc):
+    if rc != 0:
+        raise RtraceError(rc, lib().rt_last_error(_context()).decode())
+
+
+class Scene:
+    """A scene uploaded to the device (Scene + World assembly, src/scene.rs / src/renderer.rs:137-317)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @classmethod
+    def from_json_file(cls, path):
+        h = ctypes.c_void_p()
+        _check(lib().rt_scene_load_json_file(_context(), os.fspath(path).encode(), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_json_str(cls, text, base_dir=None):
+        h = ctypes.c_void_p()
+        _check(lib().rt_scene_load_json_str(_context(), text.encode(), base_dir.encode() if base_dir else None,
+                                            ctypes.byref(h)))
+        return cls(h)
+
+    def info(self):
+        w, h, en = ctypes.c_double(), ctypes.c_double(), ctypes.c_int32()
+        o = _Opts()
+        _check(lib().rt_scene_info(self._h, ctypes.byref(w), ctypes.byref(h), ctypes.byref(en), ctypes.byref(o)))
+        outline = None
+        if en.value:
+            outline = OutlineConfig(o.outline_depth_weight, o.outline_normal_weight, o.outline_threshold,
+                                    tuple(o.outline_color), bool(o.outline_use_8_neighbors), o.outline_thickness)
+        return {"camera_width": w.value, "camera_height": h.value, "outline": outline}
+
+    def close(self):
+        if self._h:
+            lib().rt_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class OutlineConfig:
+    """src/outline.rs:5-31"""
+
+    def __init__(self, depth_weight=1.0, normal_weight=1.0, threshold=0.1, edge_color=(0.0, 0.0, 0.0),
+                 use_8_neighbors=False, line_thickness=1.0):
+        self.depth_weight = depth_weight
+        self.normal_weight = normal_weight
+        self.threshold = threshold
+        self.edge_color = tuple(edge_color)
+        self.use_8_neighbors = use_8_neighbors
+        self.line_thickness = line_thickness
+
+
+class Renderer:
+    """Fields and defaults of `Renderer` (src/renderer.rs:37-118). thread_count is
+    accepted for API parity; the GPU output is independent of it, as the reference's
+    is of its thread count."""
+
+    def __init__(self, width, height, use_kdtree=True, thread_count=None):
+        self.width = width
+        self.height = height
+        self.max_depth = 10
+        self.use_kdtree = use_kdtree
+        self.thread_count = thread_count
+        self.samples = 1
+        self.anti_aliasing_mode = AntiAliasingMode.Quincunx
+        self.seed = 0
+        self.outline_config = None
+        self.count_work = False
+        self.profile = False
+        self.last_stats = None
+
+    new = classmethod(lambda cls, w, h: cls(w, h))
+    new_brute_force = classmethod(lambda cls, w, h: cls(w, h, use_kdtree=False))
+    new_with_threads = classmethod(lambda cls, w, h, n: cls(w, h, thread_count=n))
+    new_with_options = classmethod(lambda cls, w, h, use_kdtree, thread_count: cls(w, h, use_kdtree, thread_count))
+
+    def with_outline_detection(self, config):
+        self.outline_config = config
+        return self
+
+    def _opts(self):
+        o = _Opts()
+        lib().rt_render_opts_default(ctypes.byref(o), self.width, self.height)
+        o.max_depth = self.max_depth
+        o.use_kdtree = int(bool(self.use_kdtree))
+        o.samples = self.samples
+        o.aa_mode = self.anti_aliasing_mode
+        o.has_seed = 0 if self.seed is None else 1
+        o.seed = 0 if self.seed is None else self.seed
+        oc = self.outline_config
+        if oc is not None:
+            o.outline_enabled = 1
+            o.outline_use_8_neighbors = int(bool(oc.use_8_neighbors))
+            o.outline_depth_weight = oc.depth_weight
+            o.outline_normal_weight = oc.normal_weight
+            o.outline_threshold = oc.threshold
+            for i in range(3):
+                o.outline_color[i] = oc.edge_color[i]
+            o.outline_thickness = oc.line_thickness
+        o.count_work = int(bool(self.count_work))
+        o.profile = int(bool(self.profile))
+        return o
+
+    def render(self, scene):
+        """-> (height, width, 3) uint8, row-major RGB (create_image_from_data, src/renderer.rs:899-912)."""
+        o = self._opts()
+        img = np.zeros((self.height, self.width, 3), dtype=np.uint8)
+        st = RenderStats()
+        _check(lib().rt_render(_context(), scene._h, ctypes.byref(o), img.ctypes.data, ctypes.byref(st)))
+        self.last_stats = st
+        return img
+
+    def render_device(self, scene, out_tensor):
+        """Render into a device buffer (e.g. a torch.uint8 CUDA tensor of H*W*3); blocks until done."""
+        o = self._opts()
+        st = RenderStats()
+        _check(lib().rt_render_device(_context(), scene._h, ctypes.byref(o), ctypes.c_void_p(out_tensor.data_ptr()),
+                                      ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def render_to_file(self, scene, path):
+        from PIL import Image
+        Image.fromarray(self.render(scene)).save(path)
+
+
+def last_primary_hits(n):
+    """Primary-ray (object, triangle, t) of the last render, canonical order."""
+    obj = np.zeros(n, np.int32)
+    tri = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float64)
+    _check(lib().rt_last_primary_hits(_context(), obj.ctypes.data, tri.ctypes.data, t.ctypes.data, n))
+    return obj, tri, t
+
+
+def intersect(scene, origins, dirs, t_min=0.001, t_max=float("inf"), use_kdtree=True):
+    """World::hit for a batch of rays (src/ray.rs:699-711) -> (obj, tri, t)."""
+    o = np.ascontiguousarray(origins, np.float64).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, np.float64).reshape(-1, 3)
+    n = len(o)
+    obj = np.zeros(n, np.int32)
+    tri = np.zeros(n, np.int32)
+    t = np.zeros(n, np.float64)
+    _check(lib().rt_intersect(_context(), scene._h, o.ctypes.data, d.ctypes.data, n, t_min, t_max, int(use_kdtree),
+                              obj.ctypes.data, tri.ctypes.data, t.ctypes.data))
+    return obj, tri, t
+
+
+def cli_size(camera_width, camera_height, diagonal):
+    """cli/src/main.rs:79-89: H = round(D / sqrt(R^2 + 1)), W = round(R * H), round half away from zero."""
+    import math
+    r = camera_width / camera_height
+    h = diagonal / math.sqrt(r * r + 1.0)
+    w = r * h
+
+    def rnd(x):
+        f = math.floor(abs(x))
+        return int(math.copysign(f + (1.0 if abs(x) - f >= 0.5 else 0.0), x))
+
+    return rnd(w), rnd(h)

@@ -1,0 +1,96 @@
+// layout.hpp — device-resident scene layout (HBM) shared by the host flattener and
+// the kernels.
+//
+// Per mesh:
+//   nodes   : KdNode[16 B], depth-first pre-order (left child = self + 1). Inner
+//             nodes keep the reference's f64 split so that every traversal decision
+//             (origin side, far-child visit, src/mesh.rs:422-462) is taken in f64
+//             exactly as the reference takes it.
+//   recs    : TriRec[48 B] = 3 x float4, one per (leaf, triangle) pair in leaf order
+//             (triangles are copied into their leaves, no index indirection):
+//             v0 - centre, e1 = v1 - v0, e2 = v2 - v0 (computed in f64, rounded
+//             to f32), and the triangle index. The brute-force order (all triangles
+//             in file order, src/ray.rs:650-671) is one extra leaf at the end.
+//   tri64   : 9 x f64 per triangle (the reference's vertices) for the exact re-test.
+#pragma once
+
+#include <cstdint>
+
+namespace rt {
+
+enum : uint32_t { kAxisLeaf = 3u, kFlagMisplaced = 4u };
+
+struct alignas(16) KdNode {
+    double split;  // inner: split position (f64, as built by the reference)
+    uint32_t a;    // inner: index of the right child; leaf: first TriRec
+    uint32_t b;    // bits 0-1 axis (3 = leaf); inner bit 2 = a child holds a triangle
+                   // moved there by the empty-side rule (src/mesh.rs:174-178);
+                   // leaf: count << 2 | 3
+};
+static_assert(sizeof(KdNode) == 16, "KdNode is 16 bytes");
+
+constexpr int kMaxKdDepth = 16;  // KdTree::new(.., 16, 10) (src/mesh.rs:1036-1039)
+constexpr int kStackSize = kMaxKdDepth + 2;
+
+// Device view of one mesh.
+struct MeshDev {
+    const KdNode* nodes;
+    const float* recs;  // 12 floats per TriRec
+    const double* tri64;
+    uint32_t root_kd;   // 0
+    uint32_t root_bf;   // brute-force leaf
+    uint32_t n_tris;
+    float radius;              // max |v - centre|_1 (scale of the f32 rounding error)
+    double cx, cy, cz;         // centre subtracted from the f32 copies
+    float bmin[3], bmax[3];    // bounds relative to the centre, rounded outward
+    double bmin64[3], bmax64[3];
+};
+
+enum : int32_t { kSphere = 0, kPlane = 1, kCube = 2, kMesh = 3 };
+
+struct ObjDev {
+    int32_t kind;
+    int32_t mat;
+    int32_t mesh;
+    int32_t t_identity;  // cube: world_to_local == identity (src/ray.rs:474)
+    double c[3];         // sphere/cube centre, plane point
+    double r;            // sphere radius
+    double n[3];         // plane unit normal
+    double half[3];      // cube half size
+    double w2l[16];      // cube world_to_local (M^-1), column-major
+    double l2w[16];      // cube local_to_world (M)
+    double ntf[9];       // cube: (inv(w2l 3x3))^T row-major, applied to normals
+    float cf[3];         // sphere centre (f32) for the prefilter
+    float rf;            // sphere radius (f32)
+};
+
+struct MatDev {
+    double color[3];
+    double ambient, diffuse, specular, shininess;
+    int32_t has_refl;
+    int32_t texture;  // 0 none, 1 grid, 2 checkerboard
+    double refl;
+    double line_color[3];
+    double line_width, cell_size;
+    int32_t mat_b;
+    int32_t pad;
+};
+
+struct LightDev {
+    double pos[3];
+    double color[3];
+    double intensity;
+    double diameter;
+    int32_t is_area;
+    int32_t pad;
+};
+
+struct CameraDev {
+    double origin[3], horizontal[3], vertical[3], llc[3], view[3];
+    int32_t persp;
+    int32_t has_grid;
+    double pitch, thick;
+    double grid_color[3];
+};
+
+}  // namespace rt

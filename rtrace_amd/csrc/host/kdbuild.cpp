@@ -1,0 +1,158 @@
+// kdbuild.cpp — host k-d build with the reference algorithm, emitted directly in the
+// flat device layout (common/layout.hpp).
+//
+// KdTree::new / build_recursive (src/mesh.rs:75-197): leaf when depth >= 16 or <= 10
+// triangles; axis = depth % 3; stable sort by centroid[axis] (centroid = ((v0+v1)+v2)/3);
+// split = centroid of element n/2; a triangle goes left when min[axis] <= split and
+// right when max[axis] > split (in sorted order); an empty side takes the other side's
+// last triangle; children recurse left then right. Nodes are written in that same
+// depth-first pre-order, so the left child of node i is i + 1.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+#include "../common/layout.hpp"
+#include "../common/vec.hpp"
+#include "kdbuild.hpp"
+
+namespace rt {
+
+namespace {
+
+struct Builder {
+    const double* v;  // 9 per triangle
+    double cx, cy, cz;
+    FlatKd* out;
+
+    double centroid(int t, int axis) const {
+        const double* p = v + (size_t)t * 9;
+        double s = p[axis] + p[3 + axis];
+        s = s + p[6 + axis];
+        return s / 3.0;
+    }
+    double tmin(int t, int axis) const {
+        const double* p = v + (size_t)t * 9;
+        return fmin_r(fmin_r(p[axis], p[3 + axis]), p[6 + axis]);
+    }
+    double tmax(int t, int axis) const {
+        const double* p = v + (size_t)t * 9;
+        return fmax_r(fmax_r(p[axis], p[3 + axis]), p[6 + axis]);
+    }
+    void emit_leaf(const std::vector<int>& tris) {
+        KdNode n{};
+        n.split = 0.0;
+        n.a = (uint32_t)(out->recs.size() / 12);
+        n.b = ((uint32_t)tris.size() << 2) | kAxisLeaf;
+        out->nodes.push_back(n);
+        for (int t : tris) push_rec(t);
+        out->max_leaf = std::max<int>(out->max_leaf, (int)tris.size());
+        out->leaves++;
+    }
+    void push_rec(int t) {
+        const double* p = v + (size_t)t * 9;
+        float r[12];
+        r[0] = (float)(p[0] - cx);
+        r[1] = (float)(p[1] - cy);
+        r[2] = (float)(p[2] - cz);
+        r[3] = (float)(p[3] - p[0]);
+        r[4] = (float)(p[4] - p[1]);
+        r[5] = (float)(p[5] - p[2]);
+        r[6] = (float)(p[6] - p[0]);
+        r[7] = (float)(p[7] - p[1]);
+        r[8] = (float)(p[8] - p[2]);
+        uint32_t idx = (uint32_t)t;
+        memcpy(&r[9], &idx, 4);
+        // E = max(|e1|_1, |e2|_1): scale of the FP32 rounding error of the edge terms
+        double n1 = std::fabs(p[3] - p[0]) + std::fabs(p[4] - p[1]) + std::fabs(p[5] - p[2]);
+        double n2 = std::fabs(p[6] - p[0]) + std::fabs(p[7] - p[1]) + std::fabs(p[8] - p[2]);
+        r[10] = (float)(std::max(n1, n2) * (1.0 + 1e-6));
+        r[11] = 0.0f;
+        out->recs.insert(out->recs.end(), r, r + 12);
+    }
+    void build(std::vector<int> tris, int depth) {
+        if (depth >= kMaxKdDepth || tris.size() <= 10) {
+            emit_leaf(tris);
+            return;
+        }
+        int axis = depth % 3;
+        std::vector<std::pair<double, int>> keyed(tris.size());
+        for (size_t i = 0; i < tris.size(); ++i) {
+            double c = centroid(tris[i], axis);
+            if (std::isnan(c)) throw std::runtime_error("k-d build: NaN centroid (the reference panics here)");
+            keyed[i] = {c, tris[i]};
+        }
+        std::stable_sort(keyed.begin(), keyed.end(),
+                         [](const std::pair<double, int>& a, const std::pair<double, int>& b) { return a.first < b.first; });
+        double split = keyed[keyed.size() / 2].first;
+        std::vector<int> L, R;
+        L.reserve(keyed.size());
+        R.reserve(keyed.size());
+        for (auto& k : keyed) {
+            if (tmin(k.second, axis) <= split) L.push_back(k.second);
+            if (tmax(k.second, axis) > split) R.push_back(k.second);
+        }
+        uint32_t flags = 0;
+        if (L.empty()) {
+            L.push_back(R.back());
+            R.pop_back();
+            flags = kFlagMisplaced;
+        } else if (R.empty()) {
+            R.push_back(L.back());
+            L.pop_back();
+            flags = kFlagMisplaced;
+        }
+        size_t me = out->nodes.size();
+        KdNode n{};
+        n.split = split;
+        n.b = (uint32_t)axis | flags;
+        out->nodes.push_back(n);
+        build(std::move(L), depth + 1);
+        out->nodes[me].a = (uint32_t)out->nodes.size();
+        build(std::move(R), depth + 1);
+    }
+};
+
+}  // namespace
+
+void build_flat_kd(const std::vector<double>& verts, FlatKd& out) {
+    size_t n = verts.size() / 9;
+    out = FlatKd{};
+    // bounds (Mesh::compute_bounds, src/mesh.rs:1017-1033) and the centre used for
+    // the f32 copies
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = 0; i < n * 3; ++i)
+        for (int k = 0; k < 3; ++k) {
+            mn[k] = fmin_r(mn[k], verts[i * 3 + k]);
+            mx[k] = fmax_r(mx[k], verts[i * 3 + k]);
+        }
+    if (n == 0) {
+        for (int k = 0; k < 3; ++k) mn[k] = mx[k] = 0.0;
+    }
+    for (int k = 0; k < 3; ++k) {
+        out.bmin[k] = mn[k];
+        out.bmax[k] = mx[k];
+        out.center[k] = 0.5 * (mn[k] + mx[k]);
+    }
+    Builder b{verts.data(), out.center[0], out.center[1], out.center[2], &out};
+    if (n > 0) {
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        b.build(std::move(all), 0);
+    }
+    // brute-force order: one leaf over every triangle in file order (src/ray.rs:650-671)
+    out.root_bf = (uint32_t)out.nodes.size();
+    KdNode bf{};
+    bf.a = (uint32_t)(out.recs.size() / 12);
+    bf.b = ((uint32_t)n << 2) | kAxisLeaf;
+    out.nodes.push_back(bf);
+    for (size_t t = 0; t < n; ++t) b.push_rec((int)t);
+    if (n == 0) {
+        // an empty mesh has no k-d root (src/mesh.rs:82): point it at the empty leaf
+        out.root_kd = out.root_bf;
+    }
+}
+
+}  // namespace rt

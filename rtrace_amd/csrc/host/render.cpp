@@ -1,0 +1,650 @@
+// render.cpp — device scene upload, frame buffers and the Renderer::render driver
+// (src/renderer.rs:120-382) on the HIP pipeline, plus the C ABI (include/rtrace_hip.h).
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../../include/rtrace_hip.h"
+#include "../common/layout.hpp"
+#include "../common/vec.hpp"
+#include "../kernels/launch.hpp"
+#include "kdbuild.hpp"
+#include "scene_host.hpp"
+
+namespace rt {
+namespace {
+
+thread_local std::string g_tls_error;
+
+#define HIP_TRY(expr)                                                                             \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            throw Error(e_ == hipErrorOutOfMemory ? RT_E_OOM : RT_E_DEVICE,                       \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                       \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void reserve(size_t n) {
+        if (n <= bytes && p) return;
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+        }
+        size_t want = n ? n : 16;
+        HIP_TRY(hipMalloc(&p, want));
+        bytes = want;
+    }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+template <class T>
+void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+    b.reserve(v.size() * sizeof(T));
+    if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+float round_down(double x) {
+    float f = (float)x;
+    if ((double)f > x) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+float round_up(double x) {
+    float f = (float)x;
+    if ((double)f < x) f = std::nextafter(f, INFINITY);
+    return f;
+}
+
+// (inv(A))^T for the upper-left 3x3 of a column-major 4x4, with nalgebra's do_inverse3
+// formulas (applied to cube normals, src/ray.rs:474-481); row-major out.
+void normal_matrix(const double* w2l, double out[9]) {
+    auto a = [&](int r, int c) { return w2l[c * 4 + r]; };
+    double m12_23 = a(1, 1) * a(2, 2) - a(2, 1) * a(1, 2);
+    double m11_23 = a(1, 0) * a(2, 2) - a(2, 0) * a(1, 2);
+    double m11_22 = a(1, 0) * a(2, 1) - a(2, 0) * a(1, 1);
+    double det = a(0, 0) * m12_23 - a(0, 1) * m11_23 + a(0, 2) * m11_22;
+    double inv[3][3];
+    if (det == 0.0) {
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) inv[r][c] = r == c ? 1.0 : 0.0;
+    } else {
+        inv[0][0] = m12_23 / det;
+        inv[0][1] = (a(0, 2) * a(2, 1) - a(2, 2) * a(0, 1)) / det;
+        inv[0][2] = (a(0, 1) * a(1, 2) - a(0, 2) * a(1, 1)) / det;
+        inv[1][0] = -m11_23 / det;
+        inv[1][1] = (a(0, 0) * a(2, 2) - a(0, 2) * a(2, 0)) / det;
+        inv[1][2] = (a(0, 2) * a(1, 0) - a(0, 0) * a(1, 2)) / det;
+        inv[2][0] = m11_22 / det;
+        inv[2][1] = (a(0, 1) * a(2, 0) - a(0, 0) * a(2, 1)) / det;
+        inv[2][2] = (a(0, 0) * a(1, 1) - a(0, 1) * a(1, 0)) / det;
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[r * 3 + c] = inv[c][r];
+}
+
+bool is_identity(const double* m) {
+    for (int i = 0; i < 16; ++i)
+        if (m[i] != ((i % 5 == 0) ? 1.0 : 0.0)) return false;
+    return true;
+}
+
+}  // namespace
+
+struct SceneDevice {
+    HostScene host;
+    int device = 0;
+    DevBuf objs, meshes, mats, lights;
+    std::vector<std::unique_ptr<DevBuf>> mesh_bufs;
+    int n_obj = 0, n_lights = 0, shadow_per_hit = 0;
+    double ms_upload = 0;
+    std::vector<FlatKd> kd;  // host copies (stats)
+};
+
+struct FrameBuffers {
+    DevBuf path_seed, lvl, prim_obj, prim_tri, prim_t, ol_dn, q0, q1, counts, hits, hitq, hit_count, shade, spt, vis,
+        work, pix, rgb, px_dn, edge, edge2;
+};
+
+}  // namespace rt
+
+struct rt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    rt::FrameBuffers fb;
+    rt::StageTimer timer;
+    int last_paths = 0;
+};
+
+struct rt_scene {
+    std::unique_ptr<rt::SceneDevice> dev;
+    rt_ctx* ctx = nullptr;
+};
+
+namespace rt {
+namespace {
+
+void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
+    const HostScene& H = D.host;
+    auto t0 = std::chrono::steady_clock::now();
+    hipStream_t st = ctx->stream;
+    std::vector<ObjDev> objs;
+    std::vector<MeshDev> meshes;
+    D.kd.resize(H.meshes.size());
+    D.mesh_bufs.clear();
+    for (size_t m = 0; m < H.meshes.size(); ++m) {
+        const std::vector<double>& v = H.meshes[m];
+        FlatKd& kd = D.kd[m];
+        build_flat_kd(v, kd);
+        auto nodes = std::make_unique<DevBuf>(), recs = std::make_unique<DevBuf>(), tri64 = std::make_unique<DevBuf>();
+        upload(*nodes, kd.nodes, st);
+        upload(*recs, kd.recs, st);
+        upload(*tri64, v, st);
+        MeshDev md{};
+        md.nodes = nodes->as<KdNode>();
+        md.recs = recs->as<float>();
+        md.tri64 = tri64->as<double>();
+        md.root_kd = kd.root_kd;
+        md.root_bf = kd.root_bf;
+        md.n_tris = (uint32_t)(v.size() / 9);
+        md.cx = kd.center[0];
+        md.cy = kd.center[1];
+        md.cz = kd.center[2];
+        double R = 0;
+        for (size_t i = 0; i < v.size(); i += 3)
+            R = std::max(R, std::fabs(v[i] - md.cx) + std::fabs(v[i + 1] - md.cy) + std::fabs(v[i + 2] - md.cz));
+        md.radius = round_up(R * (1.0 + 1e-6));
+        for (int k = 0; k < 3; ++k) {
+            double c = k == 0 ? md.cx : (k == 1 ? md.cy : md.cz);
+            md.bmin[k] = round_down(kd.bmin[k] - c);
+            md.bmax[k] = round_up(kd.bmax[k] - c);
+            md.bmin64[k] = kd.bmin[k];
+            md.bmax64[k] = kd.bmax[k];
+        }
+        meshes.push_back(md);
+        D.mesh_bufs.push_back(std::move(nodes));
+        D.mesh_bufs.push_back(std::move(recs));
+        D.mesh_bufs.push_back(std::move(tri64));
+    }
+    for (const rt_object_desc& o : H.objects) {
+        ObjDev d{};
+        d.kind = o.kind;
+        d.mat = o.material;
+        d.mesh = o.mesh;
+        for (int k = 0; k < 3; ++k) {
+            d.c[k] = o.kind == RT_OBJ_PLANE ? o.point[k] : o.center[k];
+            d.n[k] = o.normal[k];
+            d.half[k] = o.half_size[k];
+            d.cf[k] = (float)o.center[k];
+        }
+        d.r = o.radius;
+        d.rf = round_up(o.radius);
+        memcpy(d.w2l, o.world_to_local, sizeof(d.w2l));
+        memcpy(d.l2w, o.local_to_world, sizeof(d.l2w));
+        d.t_identity = is_identity(o.world_to_local);
+        normal_matrix(o.world_to_local, d.ntf);
+        objs.push_back(d);
+    }
+    std::vector<MatDev> mats;
+    for (const rt_material_desc& m : H.materials) {
+        MatDev d{};
+        memcpy(d.color, m.color, sizeof(d.color));
+        d.ambient = m.ambient;
+        d.diffuse = m.diffuse;
+        d.specular = m.specular;
+        d.shininess = m.shininess;
+        d.has_refl = m.has_reflectivity;
+        d.refl = m.reflectivity;
+        d.texture = m.texture;
+        memcpy(d.line_color, m.grid_line_color, sizeof(d.line_color));
+        d.line_width = m.line_width;
+        d.cell_size = m.cell_size;
+        d.mat_b = m.material_b;
+        mats.push_back(d);
+    }
+    std::vector<LightDev> lights;
+    D.shadow_per_hit = 0;
+    for (const rt_light_desc& l : H.lights) {
+        LightDev d{};
+        memcpy(d.pos, l.position, sizeof(d.pos));
+        memcpy(d.color, l.color, sizeof(d.color));
+        d.intensity = l.intensity;
+        d.is_area = l.is_area;
+        d.diameter = l.diameter;
+        if (l.is_area && !(l.diameter / 2.0 > -l.diameter / 2.0))
+            throw Error(RT_E_INVALID_ARG, "cannot sample empty range");  // gen_range(-r..r) with r <= 0
+        D.shadow_per_hit += l.is_area ? 16 : 1;
+        lights.push_back(d);
+    }
+    upload(D.objs, objs, st);
+    upload(D.meshes, meshes, st);
+    upload(D.mats, mats, st);
+    upload(D.lights, lights, st);
+    HIP_TRY(hipStreamSynchronize(st));
+    D.n_obj = (int)objs.size();
+    D.n_lights = (int)lights.size();
+    D.ms_upload = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// Camera::from_config (src/camera.rs:23-152) for this image's aspect ratio.
+CameraDev make_camera(const rt_camera_desc& c, double aspect) {
+    d3 origin = mk(c.position[0], c.position[1], c.position[2]);
+    d3 target = mk(c.target[0], c.target[1], c.target[2]);
+    d3 up = mk(c.up[0], c.up[1], c.up[2]);
+    d3 w = unit(sub(origin, target));
+    d3 u = unit(cross(up, w));
+    d3 v = cross(w, u);
+    d3 view = unit(neg(w));
+    CameraDev cam{};
+    d3 hor, ver, llc;
+    if (c.kind == 0) {
+        double vh = c.height;
+        double vw = fmax_r(c.width, vh * aspect);
+        hor = scale(vw, u);
+        ver = scale(vh, v);
+        llc = sub(sub(origin, divs(hor, 2.0)), divs(ver, 2.0));
+        cam.persp = 0;
+    } else {
+        double fov = c.has_fov ? c.fov : 45.0;
+        if (fov <= 0.0 || fov >= 180.0) throw Error(RT_E_CAMERA, "Field of view must be between 0 and 180 degrees");
+        double focal = 1.0;
+        double theta = fov * (M_PI / 180.0);
+        double hh = std::tan(theta / 2.0);
+        double hw = aspect * hh;
+        double vph = (2.0 * hh) * focal;
+        double vpw = (2.0 * hw) * focal;
+        hor = scale(vpw, u);
+        ver = scale(vph, v);
+        d3 center = add(origin, scale(focal, view));
+        llc = sub(sub(center, divs(hor, 2.0)), divs(ver, 2.0));
+        cam.persp = 1;
+    }
+    auto put = [](double* dst, d3 x) {
+        dst[0] = x.x;
+        dst[1] = x.y;
+        dst[2] = x.z;
+    };
+    put(cam.origin, origin);
+    put(cam.horizontal, hor);
+    put(cam.vertical, ver);
+    put(cam.llc, llc);
+    put(cam.view, view);
+    cam.has_grid = c.has_grid;
+    cam.pitch = c.grid_pitch;
+    cam.thick = c.grid_thickness;
+    memcpy(cam.grid_color, c.grid_color, sizeof(cam.grid_color));
+    return cam;
+}
+
+SceneView make_view(const SceneDevice& D, const CameraDev& cam) {
+    const HostScene& H = D.host;
+    SceneView S{};
+    S.objs = D.objs.as<ObjDev>();
+    S.meshes = D.meshes.as<MeshDev>();
+    S.mats = D.mats.as<MatDev>();
+    S.lights = D.lights.as<LightDev>();
+    S.n_obj = D.n_obj;
+    S.n_lights = D.n_lights;
+    S.shadow_per_hit = D.shadow_per_hit > 0 ? D.shadow_per_hit : 1;
+    S.has_fog = H.has_fog;
+    S.cam = cam;
+    memcpy(S.cam_pos, H.camera.position, sizeof(S.cam_pos));
+    memcpy(S.amb, H.ambient_color, sizeof(S.amb));
+    S.amb_i = H.ambient_intensity;
+    memcpy(S.fog_color, H.fog_color, sizeof(S.fog_color));
+    S.fog_density = H.fog_density;
+    S.fog_start = H.fog_start;
+    S.fog_end = H.fog_end;
+    memcpy(S.bg, H.background, sizeof(S.bg));
+    return S;
+}
+
+void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host_rgb, void* dev_rgb,
+            rt_render_stats* stats) {
+    if (!scene || !scene->dev || !o) throw Error(RT_E_INVALID_ARG, "null scene or options");
+    if (o->samples == 0) throw Error(RT_E_INVALID_ARG, "Samples must be greater than 0");
+    if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
+    SceneDevice& D = *scene->dev;
+    const int W = o->width, H = o->height;
+    const bool quin = o->aa_mode == RT_AA_QUINCUNX;
+    if (o->aa_mode < 0 || o->aa_mode > 2) throw Error(RT_E_INVALID_ARG, "unknown anti-aliasing mode");
+    CameraDev cam = make_camera(D.host.camera, (double)W / (double)H);
+    SceneView S = make_view(D, cam);
+    FrameBuffers& B = ctx->fb;
+    hipStream_t st = ctx->stream;
+
+    FrameView F{};
+    F.W = W;
+    F.H = H;
+    F.N = quin ? 1 : o->samples;
+    F.aa = o->aa_mode;
+    F.max_depth = o->max_depth;
+    F.use_kd = o->use_kdtree;
+    F.outline = (!quin && o->outline_enabled) ? 1 : 0;
+    F.grid_miss = quin ? 0 : 1;  // quincunx calls ray_color without the camera
+    F.count_work = o->count_work;
+    F.seed = o->has_seed ? o->seed : 0;
+    F.tiles_x = (W + kTile - 1) / kTile;
+    long long tiled = (long long)F.tiles_x * ((H + kTile - 1) / kTile) * kTile * kTile;
+    long long n_paths, n_q0;
+    if (quin) {
+        long long tiled_c = (long long)((W + 1 + kTile - 1) / kTile) * ((H + 1 + kTile - 1) / kTile) * kTile * kTile;
+        n_paths = (long long)W * H + (long long)(W + 1) * (H + 1);
+        n_q0 = tiled + tiled_c;
+    } else {
+        n_paths = (long long)W * H * F.N;
+        n_q0 = tiled * F.N;
+    }
+    if (n_q0 > (1ll << 30)) throw Error(RT_E_INVALID_ARG, "image too large for one launch");
+    F.n_paths = (int)n_paths;
+    F.n_q0 = (int)n_q0;
+    const int depth = std::max(o->max_depth, 0);
+    const size_t cap = (size_t)n_q0;
+    const size_t K = (size_t)S.shadow_per_hit;
+
+    B.path_seed.reserve(n_paths * sizeof(uint64_t));
+    B.lvl.reserve((size_t)std::max(depth, 1) * n_paths * 4 * sizeof(double));
+    B.prim_obj.reserve(n_paths * sizeof(int32_t));
+    B.prim_tri.reserve(n_paths * sizeof(int32_t));
+    B.prim_t.reserve(n_paths * sizeof(double));
+    B.ol_dn.reserve(F.outline ? n_paths * 4 * sizeof(double) : 16);
+    B.q0.reserve(cap * sizeof(RayRec));
+    B.q1.reserve(cap * sizeof(RayRec));
+    B.counts.reserve((depth + 2) * sizeof(int32_t));
+    B.hits.reserve(cap * sizeof(HitRec));
+    B.hitq.reserve(cap * sizeof(int32_t));
+    B.hit_count.reserve((depth + 1) * sizeof(int32_t));
+    B.shade.reserve(cap * sizeof(ShadeRec));
+    B.spt.reserve(cap * K * 3 * sizeof(double));
+    B.vis.reserve(cap * K);
+    B.work.reserve(2 * kWorkCount * sizeof(unsigned long long));
+    B.pix.reserve((size_t)W * H * 3 * sizeof(double));
+    B.rgb.reserve((size_t)W * H * 3);
+    B.px_dn.reserve(F.outline ? (size_t)W * H * 4 * sizeof(double) : 16);
+    B.edge.reserve(F.outline ? (size_t)W * H * sizeof(double) : 16);
+    B.edge2.reserve(F.outline ? (size_t)W * H * sizeof(double) : 16);
+
+    F.path_seed = B.path_seed.as<uint64_t>();
+    F.lvl = B.lvl.as<double>();
+    F.prim_obj = B.prim_obj.as<int32_t>();
+    F.prim_tri = B.prim_tri.as<int32_t>();
+    F.prim_t = B.prim_t.as<double>();
+    F.ol_dn = B.ol_dn.as<double>();
+    F.counts = B.counts.as<int32_t>();
+    F.hits = B.hits.as<HitRec>();
+    F.hitq = B.hitq.as<int32_t>();
+    F.hit_count = B.hit_count.as<int32_t>();
+    F.shade = B.shade.as<ShadeRec>();
+    F.spt = B.spt.as<double>();
+    F.vis = B.vis.as<uint8_t>();
+    F.work = B.work.as<unsigned long long>();
+    F.pix = B.pix.as<double>();
+    F.rgb = dev_rgb ? static_cast<uint8_t*>(dev_rgb) : B.rgb.as<uint8_t>();
+    F.px_dn = B.px_dn.as<double>();
+    F.edge = B.edge.as<double>();
+    F.edge2 = B.edge2.as<double>();
+    F.ol_wd = o->outline_depth_weight;
+    F.ol_wn = o->outline_normal_weight;
+    F.ol_thr = o->outline_threshold;
+    F.ol_thick = o->outline_thickness;
+    memcpy(F.ol_color, o->outline_color, sizeof(F.ol_color));
+    F.ol_8 = o->outline_use_8_neighbors;
+    F.max_depth = depth;
+
+    if (depth == 0) {
+        // ray_color returns black at max_depth <= 0 before any World::hit; mark the
+        // primary records as misses
+        HIP_TRY(hipMemsetAsync(F.prim_obj, 0xff, n_paths * sizeof(int32_t), st));
+        HIP_TRY(hipMemsetAsync(F.prim_tri, 0xff, n_paths * sizeof(int32_t), st));
+    }
+    ctx->timer.reset();
+    HIP_TRY(hipEventRecord(ctx->ev0, st));
+    HIP_TRY(launch_frame(S, F, B.q0.as<RayRec>(), B.q1.as<RayRec>(), st, o->profile ? &ctx->timer : nullptr));
+    HIP_TRY(hipEventRecord(ctx->ev1, st));
+    if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> counts(depth + 2, 0), hit_count(depth + 1, 0);
+    unsigned long long work[2 * kWorkCount] = {};
+    HIP_TRY(hipMemcpyAsync(counts.data(), F.counts, (depth + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    if (depth > 0)
+        HIP_TRY(hipMemcpyAsync(hit_count.data(), F.hit_count, depth * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(work, F.work, sizeof(work), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    ctx->last_paths = (int)n_paths;
+    if (stats) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        memset(stats, 0, sizeof(*stats));
+        stats->rays_primary = depth > 0 ? (uint64_t)n_paths : 0;
+        for (int l = 1; l < depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
+        for (int l = 0; l < depth; ++l) stats->rays_shadow += (uint64_t)hit_count[l] * (uint64_t)D.shadow_per_hit;
+        for (int k = 0; k < 2; ++k) {
+            const unsigned long long* wk = work + k * kWorkCount;
+            stats->work[k].nodes = wk[kWorkNodes];
+            stats->work[k].tris = wk[kWorkTris];
+            stats->work[k].retests = wk[kWorkRetests];
+            stats->work[k].spheres = wk[kWorkSphere];
+            stats->work[k].planes = wk[kWorkPlane];
+            stats->work[k].cubes = wk[kWorkCube];
+        }
+        stats->ms_render = ms;
+        stats->ms_upload = D.ms_upload;
+        stats->levels = depth;
+        if (o->profile) {
+            for (size_t i = 0; i < ctx->timer.used / 2; ++i) {
+                float e = 0;
+                HIP_TRY(hipEventElapsedTime(&e, ctx->timer.ev[2 * i], ctx->timer.ev[2 * i + 1]));
+                int sidx = ctx->timer.stage[i];
+                stats->ms_stage[sidx] += e;
+                stats->launches_stage[sidx] += 1;
+            }
+        }
+    }
+}
+
+int fail(rt_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    g_tls_error = msg;
+    return code;
+}
+
+template <class F>
+int guarded(rt_ctx* ctx, F fn) {
+    try {
+        fn();
+        return RT_OK;
+    } catch (const Error& e) {
+        return fail(ctx, e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(ctx, RT_E_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(ctx, RT_E_INTERNAL, e.what());
+    }
+}
+
+void check_device(int device) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw Error(RT_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) throw Error(RT_E_DEVICE, "device index out of range");
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+        throw Error(RT_E_DEVICE, std::string("librtrace_hip is built for gfx950, device is ") + prop.gcnArchName);
+}
+
+rt_scene* make_scene(rt_ctx* ctx, HostScene&& hs) {
+    auto s = std::make_unique<rt_scene>();
+    s->ctx = ctx;
+    s->dev = std::make_unique<SceneDevice>();
+    s->dev->host = std::move(hs);
+    s->dev->device = ctx->device;
+    HIP_TRY(hipSetDevice(ctx->device));
+    build_device_scene(ctx, *s->dev);
+    return s.release();
+}
+
+}  // namespace
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" {
+
+int rt_ctx_create(int device, rt_ctx** out) {
+    if (!out) return fail(nullptr, RT_E_INVALID_ARG, "out is null");
+    *out = nullptr;
+    auto c = std::make_unique<rt_ctx>();
+    int rc = guarded(nullptr, [&] {
+        check_device(device);
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&c->ev0));
+        HIP_TRY(hipEventCreate(&c->ev1));
+        c->device = device;
+    });
+    if (rc == RT_OK) *out = c.release();
+    return rc;
+}
+
+void rt_ctx_destroy(rt_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* rt_last_error(rt_ctx* ctx) { return ctx ? ctx->err.c_str() : g_tls_error.c_str(); }
+
+int rt_scene_load_json_file(rt_ctx* ctx, const char* path, rt_scene** out) {
+    if (!ctx || !path || !out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    return guarded(ctx, [&] { *out = make_scene(ctx, load_scene_file(path)); });
+}
+
+int rt_scene_load_json_str(rt_ctx* ctx, const char* json, const char* base_dir, rt_scene** out) {
+    if (!ctx || !json || !out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    return guarded(ctx, [&] { *out = make_scene(ctx, load_scene_json(json, base_dir ? base_dir : ".")); });
+}
+
+int rt_scene_upload(rt_ctx* ctx, const rt_scene_desc* desc, rt_scene** out) {
+    if (!ctx || !desc || !out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    return guarded(ctx, [&] { *out = make_scene(ctx, from_desc(*desc)); });
+}
+
+void rt_scene_free(rt_scene* scene) {
+    if (!scene) return;
+    if (scene->ctx && scene->ctx->stream) (void)hipStreamSynchronize(scene->ctx->stream);
+    delete scene;
+}
+
+int rt_scene_info(rt_scene* scene, double* cam_width, double* cam_height, int32_t* outline_enabled,
+                  rt_render_opts* outline_out) {
+    if (!scene || !scene->dev) return fail(nullptr, RT_E_INVALID_ARG, "null scene");
+    const HostScene& H = scene->dev->host;
+    if (cam_width) *cam_width = H.camera.width;
+    if (cam_height) *cam_height = H.camera.height;
+    if (outline_enabled) *outline_enabled = H.outline_enabled;
+    if (H.outline_enabled && !H.outline_color_ok)
+        return fail(scene->ctx, RT_E_CAMERA, "Invalid hex color format");
+    if (outline_out && H.outline_enabled) {
+        outline_out->outline_enabled = 1;
+        outline_out->outline_use_8_neighbors = H.outline.outline_use_8_neighbors;
+        outline_out->outline_depth_weight = H.outline.outline_depth_weight;
+        outline_out->outline_normal_weight = H.outline.outline_normal_weight;
+        outline_out->outline_threshold = H.outline.outline_threshold;
+        memcpy(outline_out->outline_color, H.outline.outline_color, sizeof(outline_out->outline_color));
+        outline_out->outline_thickness = H.outline.outline_thickness;
+    }
+    return RT_OK;
+}
+
+void rt_render_opts_default(rt_render_opts* o, int32_t width, int32_t height) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->width = width;
+    o->height = height;
+    o->max_depth = 10;
+    o->use_kdtree = 1;
+    o->samples = 1;
+    o->aa_mode = RT_AA_QUINCUNX;
+    o->has_seed = 1;
+    o->seed = 0;
+    o->outline_depth_weight = 1.0;
+    o->outline_normal_weight = 1.0;
+    o->outline_threshold = 0.1;
+    o->outline_thickness = 1.0;
+}
+
+int rt_render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, uint8_t* rgb_out, rt_render_stats* stats) {
+    if (!ctx || !rgb_out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] { render(ctx, scene, opts, rgb_out, nullptr, stats); });
+}
+
+int rt_render_device(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
+                     rt_render_stats* stats) {
+    if (!ctx || !d_rgb_out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] { render(ctx, scene, opts, nullptr, d_rgb_out, stats); });
+}
+
+int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, size_t n) {
+    if (!ctx) return fail(nullptr, RT_E_INVALID_ARG, "null context");
+    return guarded(ctx, [&] {
+        size_t m = std::min(n, (size_t)ctx->last_paths);
+        if (obj) HIP_TRY(hipMemcpy(obj, ctx->fb.prim_obj.p, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (tri) HIP_TRY(hipMemcpy(tri, ctx->fb.prim_tri.p, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (t) HIP_TRY(hipMemcpy(t, ctx->fb.prim_t.p, m * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int rt_intersect(rt_ctx* ctx, rt_scene* scene, const double* origins, const double* dirs, size_t n, double t_min,
+                 double t_max, int use_kdtree, int32_t* obj_out, int32_t* tri_out, double* t_out) {
+    if (!ctx || !scene || !scene->dev || (n && (!origins || !dirs || !obj_out || !tri_out || !t_out)))
+        return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    if (n == 0) return RT_OK;
+    if (n > (size_t)INT32_MAX) return fail(ctx, RT_E_INVALID_ARG, "too many rays");
+    return guarded(ctx, [&] {
+        SceneDevice& D = *scene->dev;
+        SceneView S = make_view(D, CameraDev{});  // World::hit needs no camera
+        DevBuf o, d, ob, tr, tt;
+        o.reserve(n * 3 * sizeof(double));
+        d.reserve(n * 3 * sizeof(double));
+        ob.reserve(n * sizeof(int32_t));
+        tr.reserve(n * sizeof(int32_t));
+        tt.reserve(n * sizeof(double));
+        hipStream_t st = ctx->stream;
+        HIP_TRY(hipMemcpyAsync(o.p, origins, n * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(d.p, dirs, n * 3 * sizeof(double), hipMemcpyHostToDevice, st));
+        HIP_TRY(launch_intersect(S, o.as<double>(), d.as<double>(), (int)n, t_min, t_max, use_kdtree, ob.as<int32_t>(),
+                                 tr.as<int32_t>(), tt.as<double>(), st));
+        HIP_TRY(hipMemcpyAsync(obj_out, ob.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(tri_out, tr.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(t_out, tt.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    });
+}
+
+}  // extern "C"

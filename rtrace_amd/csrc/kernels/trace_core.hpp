@@ -1,0 +1,933 @@
+// trace_core.hpp — per-item bodies of the wavefront trace pipeline.
+//
+// Every stage is a function of (scene view, frame view, item index) so the __global__
+// kernels in trace.hip are thin grid-stride wrappers. The semantics are the
+// reference's (SeaRyanC/rtrace src/lighting.rs, src/ray.rs, src/renderer.rs); the
+// structure is the GPU's: FP32 k-d traversal with an LDS stack, FP64 re-test of
+// survivors, compacted ray / hit queues, per-level colour records combined at the end.
+#pragma once
+
+#include "device_math.hpp"
+
+namespace rt {
+
+// ------------------------------------------------------------------ views
+struct SceneView {
+    const ObjDev* objs;
+    const MeshDev* meshes;
+    const MatDev* mats;
+    const LightDev* lights;
+    int32_t n_obj;
+    int32_t n_lights;
+    int32_t shadow_per_hit;  // sum over lights of (area ? 16 : 1)
+    int32_t has_fog;
+    CameraDev cam;
+    double cam_pos[3];
+    double amb[3];
+    double amb_i;
+    double fog_color[3];
+    double fog_density, fog_start, fog_end;
+    double bg[3];
+};
+
+struct RayRec {  // 64 B queue entry
+    double o[3];
+    double d[3];
+    int32_t path;   // -1 = hole (tiled raygen)
+    int32_t level;
+};
+
+struct HitRec {  // per queue slot
+    double t;
+    int32_t obj;  // -1 miss
+    int32_t tri;
+};
+
+struct ShadeRec {  // per hit-queue entry
+    double p[3];
+    double n[3];
+    double mc[3];   // effective material colour (texture applied)
+    int32_t mat;    // material whose coefficients apply (checkerboard alternate or base)
+    int32_t path;
+    int32_t level;
+    int32_t base;   // object material (reflectivity source)
+    int32_t spawned;  // a reflection ray continues this path
+    int32_t pad;
+};
+
+// work counters (rt_work field order)
+enum : int32_t { kWorkNodes = 0, kWorkTris = 1, kWorkRetests = 2, kWorkSphere = 3, kWorkPlane = 4, kWorkCube = 5,
+                 kWorkCount = 6 };
+
+struct FrameView {
+    int32_t W, H, N;  // image size, samples per pixel (1 for quincunx)
+    int32_t aa;       // 0 no-jitter, 1 quincunx, 2 stochastic
+    int32_t max_depth;
+    int32_t use_kd;
+    int32_t outline;      // record first-hit depth/normal
+    int32_t grid_miss;    // misses may show the ortho grid (camera passed to ray_color)
+    int32_t count_work;
+    int32_t n_paths;
+    int32_t tiles_x;      // raygen tiling
+    int32_t n_q0;         // level-0 queue length including holes
+    uint64_t seed;        // Renderer.seed.unwrap_or(0)
+    uint64_t* path_seed;  // [n_paths]
+    double* lvl;          // [max_depth][n_paths][4] (r, g, b, rho or -1 = terminal)
+    int32_t* prim_obj;    // [n_paths]
+    int32_t* prim_tri;
+    double* prim_t;
+    double* ol_dn;        // [n_paths][4] depth, nx, ny, nz (NaN depth = no hit)
+    RayRec* rays;         // current level queue
+    RayRec* rays_next;    // next level queue
+    int32_t* counts;      // [max_depth + 1] queue lengths; counts[0] = n_q0
+    HitRec* hits;         // [queue capacity]
+    int32_t* hitq;        // [queue capacity] compacted slots that hit
+    int32_t* hit_count;   // [max_depth]
+    ShadeRec* shade;      // [queue capacity]
+    double* spt;          // [queue capacity * shadow_per_hit * 3] sample points
+    uint8_t* vis;         // [queue capacity * shadow_per_hit]
+    unsigned long long* work;  // [2][kWorkCount]: closest, shadow
+    double* pix;          // [W*H*3] final linear colour
+    uint8_t* rgb;         // [W*H*3]
+    // outline buffers
+    double* px_dn;        // [W*H*4] per-pixel depth/normal (NaN = background)
+    double* edge;         // [W*H]
+    double* edge2;        // [W*H]
+    double ol_wd, ol_wn, ol_thr, ol_thick;
+    double ol_color[3];
+    int32_t ol_8;
+};
+
+// --------------------------------------------------------------- helpers
+struct Work {
+    unsigned long long v[kWorkCount] = {0, 0, 0, 0, 0, 0};
+};
+
+// Per-lane traversal stack: entry i of this lane lives at base[i * stride].
+struct StackRef {
+    uint32_t* node;
+    float* t0;
+    float* t1;
+    int stride;
+};
+
+RT_HD d3 ld3(const double* p) { return mk(p[0], p[1], p[2]); }
+RT_HD void st3(double* p, d3 v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+
+// HitRecord::new face orientation (src/ray.rs:39-63)
+RT_HD d3 face(const d3& dir, const d3& outward) {
+    bool front = dot(dir, outward) < 0.0;
+    return front ? unit(outward) : unit(neg(outward));
+}
+
+// reflect (src/lighting.rs:291-294)
+RT_HD d3 reflect_dir(d3 inc, d3 n) {
+    double k = 2.0 * dot(inc, n);
+    return unit(sub(inc, scale(k, n)));
+}
+
+constexpr float kEps32 = 5.9604645e-08f;  // 2^-24
+constexpr float kTolC = 64.0f * kEps32;   // slack factor for FP32 error bounds
+
+// ------------------------------------------------------------ mesh traversal
+// kAny = shadow (any hit with tmin < t < tmax); otherwise closest hit improving `best`
+// strictly (t < best, first visited wins ties, src/ray.rs:620-649).
+//
+// Visit order and child selection are the reference's (src/mesh.rs:399-465), decided in
+// f64: origin side first (origin <= split -> left), far side only when t_split >= 0
+// (sign test, no division), only the origin side when |d_axis| < 1e-9. On top of that,
+// FP32 [t0, t1] intervals with conservative slack skip subtrees that cannot hold a hit
+// before `best`; a node whose child received a triangle by the empty-side rule
+// (kFlagMisplaced) passes its whole interval to both children.
+template <bool kAny>
+RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
+                      StackRef stk, Work& w, bool count) {
+    const float ox = (float)(r.o.x - M.cx), oy = (float)(r.o.y - M.cy), oz = (float)(r.o.z - M.cz);
+    const float dx = (float)r.d.x, dy = (float)r.d.y, dz = (float)r.d.z;
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const float R = M.radius;  // max |v - centre|_1 over the mesh
+
+    // mesh AABB (src/ray.rs:586-610), conservative
+    float c0 = (float)tmin, c1 = (float)best;
+    {
+        const float o3[3] = {ox, oy, oz}, d3_[3] = {dx, dy, dz}, i3[3] = {ix, iy, iz};
+        for (int a = 0; a < 3; ++a) {
+            float lo = M.bmin[a], hi = M.bmax[a];
+            if (d3_[a] == 0.0f) {
+                float tol = kTolC * (fabsf(o3[a]) + R);
+                if (o3[a] < lo - tol || o3[a] > hi + tol) return false;
+                continue;
+            }
+            float ta = (lo - o3[a]) * i3[a], tb = (hi - o3[a]) * i3[a];
+            if (ta > tb) {
+                float tmp = ta;
+                ta = tb;
+                tb = tmp;
+            }
+            float tol = kTolC * (fabsf(ta) + fabsf(tb) + (fabsf(o3[a]) + R) * fabsf(i3[a]));
+            c0 = fmaxf(c0, ta - tol);
+            c1 = fminf(c1, tb + tol);
+        }
+        c0 = fmaxf(c0, (float)tmin * 0.5f);
+        if (!(c0 <= c1)) return false;
+    }
+    const bool par0 = fabs(r.d.x) < 1e-9, par1 = fabs(r.d.y) < 1e-9, par2 = fabs(r.d.z) < 1e-9;
+    const float4* recs = reinterpret_cast<const float4*>(M.recs);
+    uint32_t node = use_kd ? M.root_kd : M.root_bf;
+    int sp = 0;
+    bool found = false;
+    float best_hi = (float)best * (1.0f + kTolC) + kTolC;  // upper bound of best in f32
+    const float tmin_lo = (float)tmin * (1.0f - kTolC);
+    for (;;) {
+        bool descend = false;
+        if (c0 <= best_hi) {
+            KdNode nd = M.nodes[node];
+            if (count) w.v[kWorkNodes]++;
+            uint32_t ax = nd.b & 3u;
+            if (ax != kAxisLeaf) {
+                double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
+                double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
+                float ia = ax == 0 ? ix : (ax == 1 ? iy : iz);
+                bool par = ax == 0 ? par0 : (ax == 1 ? par1 : par2);
+                bool left_near = oa <= nd.split;
+                uint32_t nearc = left_near ? node + 1 : nd.a;
+                uint32_t farc = left_near ? nd.a : node + 1;
+                bool far_ok = !par && ((oa == nd.split) || ((oa < nd.split) == (da > 0.0)));
+                descend = true;
+                if (!far_ok) {
+                    node = nearc;
+                } else if (nd.b & kFlagMisplaced) {
+                    stk.node[sp * stk.stride] = farc;
+                    stk.t0[sp * stk.stride] = c0;
+                    stk.t1[sp * stk.stride] = c1;
+                    ++sp;
+                    node = nearc;
+                } else {
+                    float ts = (float)(nd.split - oa) * ia;
+                    float tol = kTolC * fabsf(ts) + 1e-30f;
+                    bool dn = c0 <= ts + tol;
+                    bool dfar = ts - tol <= c1;
+                    if (dn && dfar) {
+                        stk.node[sp * stk.stride] = farc;
+                        stk.t0[sp * stk.stride] = fmaxf(c0, ts - tol);
+                        stk.t1[sp * stk.stride] = c1;
+                        ++sp;
+                        node = nearc;
+                        c1 = fminf(c1, ts + tol);
+                    } else if (dn) {
+                        node = nearc;
+                        c1 = fminf(c1, ts + tol);
+                    } else if (dfar) {
+                        node = farc;
+                        c0 = fmaxf(c0, ts - tol);
+                    } else {
+                        descend = false;
+                    }
+                }
+            } else {
+                // leaf: FP32 Moller-Trumbore with error bounds; survivors re-tested in f64
+                uint32_t first = nd.a, cnt = nd.b >> 2;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    float4 r0 = recs[(size_t)(first + k) * 3 + 0];
+                    float4 r1 = recs[(size_t)(first + k) * 3 + 1];
+                    float4 r2 = recs[(size_t)(first + k) * 3 + 2];
+                    if (count) w.v[kWorkTris]++;
+                    float e1x = r0.w, e1y = r1.x, e1z = r1.y;
+                    float e2x = r1.z, e2y = r1.w, e2z = r2.x;
+                    float E = r2.z;  // max(|e1|_1, |e2|_1)
+                    float hx = dy * e2z - dz * e2y, hy = dz * e2x - dx * e2z, hz = dx * e2y - dy * e2x;
+                    float a = e1x * hx + e1y * hy + e1z * hz;
+                    float sx = ox - r0.x, sy = oy - r0.y, sz = oz - r0.z;
+                    float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
+                    float fa = fabsf(a);
+                    bool sure_in = false;
+                    bool maybe = true;
+                    if (fa > 4e-8f + kTolC * E * E) {
+                        float f = 1.0f / a;
+                        float g = E * fabsf(f);
+                        float du = kTolC * (S + E) * g + 1e-7f;
+                        float u = f * (sx * hx + sy * hy + sz * hz);
+                        if (u < -du || u > 1.0f + du) {
+                            maybe = false;
+                        } else {
+                            float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+                            float v = f * (dx * qx + dy * qy + dz * qz);
+                            if (v < -du || u + v > 1.0f + 2.0f * du) {
+                                maybe = false;
+                            } else {
+                                float t = f * (e2x * qx + e2y * qy + e2z * qz);
+                                float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
+                                if (t + dt <= tmin_lo || t - dt >= best_hi) {
+                                    maybe = false;
+                                } else if (kAny) {
+                                    sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
+                                              t - dt > (float)tmin * (1.0f + kTolC) &&
+                                              t + dt < (float)best * (1.0f - kTolC);
+                                }
+                            }
+                        }
+                    }
+                    if (!maybe) continue;
+                    if (kAny && sure_in) return true;
+                    uint32_t ti = f32_to_bits(r2.y);
+                    double tt, uu, vv;
+                    d3 nn;
+                    if (count) w.v[kWorkRetests]++;
+                    if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) {
+                        if (kAny) return true;
+                        best = tt;
+                        best_tri = (int32_t)ti;
+                        best_hi = (float)best * (1.0f + kTolC) + kTolC;
+                        found = true;
+                    }
+                }
+            }
+        }
+        if (descend) continue;
+        // pop the next pending subtree that can still beat `best`
+        bool popped = false;
+        while (sp > 0) {
+            --sp;
+            node = stk.node[sp * stk.stride];
+            c0 = stk.t0[sp * stk.stride];
+            c1 = stk.t1[sp * stk.stride];
+            if (c0 <= best_hi) {
+                popped = true;
+                break;
+            }
+        }
+        if (!popped) break;
+        if (!kAny) c1 = fminf(c1, best_hi);
+    }
+    return found;
+}
+
+// ------------------------------------------------------------ World::hit
+// Closest hit over all objects in scene order (src/ray.rs:699-711): analytic objects
+// accept t == closest (and so replace an equal earlier hit), meshes need t < closest.
+RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk,
+                         Work& w, bool count, double& t_out, int32_t& obj_out, int32_t& tri_out) {
+    double closest = tmax;
+    int32_t bo = -1, bt = -1;
+    RayF f = make_rayf(r);
+    for (int i = 0; i < S.n_obj; ++i) {
+        const ObjDev& ob = S.objs[i];
+        double t;
+        if (ob.kind == kSphere) {
+            if (count) w.v[kWorkSphere]++;
+            if (sphere_may_hit(ob, r, f, tmin, closest) && sphere_exact(ob, r, tmin, closest, t)) {
+                closest = t;
+                bo = i;
+                bt = -1;
+            }
+        } else if (ob.kind == kPlane) {
+            if (count) w.v[kWorkPlane]++;
+            if (plane_exact(ob, r, tmin, closest, t)) {
+                closest = t;
+                bo = i;
+                bt = -1;
+            }
+        } else if (ob.kind == kCube) {
+            if (count) w.v[kWorkCube]++;
+            d3 p, n;
+            if (cube_exact(ob, r, tmin, closest, t, p, n)) {
+                closest = t;
+                bo = i;
+                bt = -1;
+            }
+        } else {
+            int32_t tri = -1;
+            double best = closest;
+            if (mesh_trace<false>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w, count)) {
+                closest = best;
+                bo = i;
+                bt = tri;
+            }
+        }
+    }
+    t_out = closest;
+    obj_out = bo;
+    tri_out = bt;
+    return bo >= 0;
+}
+
+// Shadow query: World::hit(shadow_ray, 0.001, dist).is_some() (src/lighting.rs:120,185).
+RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w,
+                     bool count) {
+    RayF f = make_rayf(r);
+    for (int i = 0; i < S.n_obj; ++i) {
+        const ObjDev& ob = S.objs[i];
+        double t;
+        if (ob.kind == kSphere) {
+            if (count) w.v[kWorkSphere]++;
+            if (sphere_may_hit(ob, r, f, tmin, tmax) && sphere_exact(ob, r, tmin, tmax, t)) return true;
+        } else if (ob.kind == kPlane) {
+            if (count) w.v[kWorkPlane]++;
+            if (plane_exact(ob, r, tmin, tmax, t)) return true;
+        } else if (ob.kind == kCube) {
+            if (count) w.v[kWorkCube]++;
+            d3 p, n;
+            if (cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
+        } else {
+            int32_t tri = -1;
+            double best = tmax;
+            if (mesh_trace<true>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w, count)) return true;
+        }
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ camera
+// Camera::get_ray (src/camera.rs:155-167)
+RT_HD RayD camera_ray(const CameraDev& c, double u, double v) {
+    d3 vp = add(add(ld3(c.llc), scale(u, ld3(c.horizontal))), scale(v, ld3(c.vertical)));
+    RayD r;
+    if (c.persp) {
+        r.o = ld3(c.origin);
+        r.d = unit(unit(sub(vp, r.o)));
+    } else {
+        r.o = vp;
+        r.d = unit(ld3(c.view));
+    }
+    return r;
+}
+
+// Camera::get_grid_color (src/camera.rs:171-253)
+RT_HD bool grid_color(const CameraDev& c, const RayD& r, d3& out) {
+    if (c.persp || !c.has_grid) return false;
+    double ht = c.thick / 2.0, p = c.pitch;
+    for (int plane = 0; plane < 3; ++plane) {
+        double dcomp = plane == 0 ? r.d.z : (plane == 1 ? r.d.y : r.d.x);
+        double ocomp = plane == 0 ? r.o.z : (plane == 1 ? r.o.y : r.o.x);
+        if (fabs(dcomp) > 1e-10) {
+            double t = -ocomp / dcomp;
+            if (t > 0.0) {
+                d3 ip = add(r.o, scale(t, r.d));
+                double a = plane == 2 ? ip.y : ip.x;
+                double b = plane == 0 ? ip.y : ip.z;
+                double am = fabs(fract_r(a / p)), bm = fabs(fract_r(b / p));
+                double ad = fmin_r(am * p, (1.0 - am) * p);
+                double bd = fmin_r(bm * p, (1.0 - bm) * p);
+                if (ad <= ht || bd <= ht) {
+                    out = ld3(c.grid_color);
+                    return true;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ raygen
+constexpr int kTile = 8;  // 8x8 pixel tile = one 64-lane wavefront
+
+// Level-0 queue slot q -> (path, ray). Slots enumerate 8x8 tiles (holes outside the
+// image get path = -1); quincunx puts centres first, then the (W+1)x(H+1) corners.
+RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
+    RayRec rec;
+    rec.level = 0;
+    rec.path = -1;
+    const CameraDev& cam = S.cam;
+    if (F.aa == 1) {
+        // render_quincunx (src/renderer.rs:752-897)
+        int tiles_c = F.tiles_x * ((F.H + kTile - 1) / kTile) * kTile * kTile;
+        bool corner = q >= tiles_c;
+        int qq = corner ? q - tiles_c : q;
+        int gw = corner ? F.W + 1 : F.W, gh = corner ? F.H + 1 : F.H;
+        int tx = (gw + kTile - 1) / kTile;
+        int tile = qq / (kTile * kTile), in = qq % (kTile * kTile);
+        int x = (tile % tx) * kTile + in % kTile, y = (tile / tx) * kTile + in / kTile;
+        if (x < gw && y < gh) {
+            double pw = 1.0 / (double)F.W, ph = 1.0 / (double)F.H;
+            double u, v;
+            uint64_t s;
+            int path;
+            if (corner) {
+                u = clamp_r((double)x * pw, 0.0, 1.0);
+                v = clamp_r(1.0 - (double)y * ph, 0.0, 1.0);
+                s = F.seed * 0x9E3779B97F4A7C15ull + (uint64_t)x + (uint64_t)y * 0x85EBCA6Bull;
+                path = F.W * F.H + y * (F.W + 1) + x;
+            } else {
+                u = ((double)x + 0.5) * pw;
+                v = 1.0 - ((double)y + 0.5) * ph;
+                s = F.seed * 0x9E3779B97F4A7C15ull + (uint64_t)x * 0x85EBCA6Bull + (uint64_t)y * 0xC2B2AE35ull +
+                    0x12345678ull;
+                path = y * F.W + x;
+            }
+            RayD r = camera_ray(cam, u, v);
+            st3(rec.o, r.o);
+            st3(rec.d, r.d);
+            rec.path = path;
+            F.path_seed[path] = s;
+        }
+    } else {
+        // render_standard / render_standard_with_outline (src/renderer.rs:445-749);
+        // slot = sample * (tiled pixels) + tiled pixel
+        int tiled = F.tiles_x * ((F.H + kTile - 1) / kTile) * kTile * kTile;
+        int smp = q / tiled, qq = q % tiled;
+        int tile = qq / (kTile * kTile), in = qq % (kTile * kTile);
+        int x = (tile % F.tiles_x) * kTile + in % kTile, y = (tile / F.tiles_x) * kTile + in / kTile;
+        if (x < F.W && y < F.H && smp < F.N) {
+            double pu = (double)x / (double)(F.W - 1);
+            double pv = (double)(F.H - 1 - y) / (double)(F.H - 1);
+            double pw = 1.0 / (double)(F.W - 1), ph = 1.0 / (double)(F.H - 1);
+            uint64_t ps = F.seed * 0x9E3779B97F4A7C15ull + (uint64_t)x * 0x85EBCA6Bull + (uint64_t)y * 0xC2B2AE35ull;
+            double su = pu, sv = pv;
+            if (F.aa == 2) {
+                StdRngDev rng;
+                if (F.N == 1) {
+                    rng.init(ps, 0);
+                    double ju = rng.gen_f64() - 0.5;
+                    double jv = rng.gen_f64() - 0.5;
+                    su = pu + ju * pw;
+                    sv = pv + jv * ph;
+                } else {
+                    // sample s draws u64 #2s and #2s+1 of the pixel's stream
+                    rng.init(ps, (uint64_t)smp * 4);
+                    double angle = 2.0 * 3.141592653589793 * (double)smp / (double)F.N;
+                    double phase = rng.gen_f64() * 2.0 * 3.141592653589793;
+                    double rot = angle + phase;
+                    double rad = 0.5 * rng.gen_f64();
+                    double ju = rad * cos(rot);
+                    double jv = rad * sin(rot);
+                    su = pu + ju * pw;
+                    sv = pv + jv * ph;
+                }
+            }
+            RayD r = camera_ray(cam, su, sv);
+            st3(rec.o, r.o);
+            st3(rec.d, r.d);
+            int path = (y * F.W + x) * F.N + smp;
+            rec.path = path;
+            F.path_seed[path] = ps + (uint64_t)smp * 0x1F845FEDull;
+        }
+    }
+    F.rays[q] = rec;
+}
+
+// ------------------------------------------------------------ closest hit stage
+// Returns true when the ray hit (the caller appends the slot to the hit queue).
+RT_HD bool closest_item(const SceneView& S, const FrameView& F, int slot, int level, StackRef stk, Work& w) {
+    const RayRec& rr = F.rays[slot];
+    HitRec h;
+    h.t = 0.0;
+    h.obj = -1;
+    h.tri = -1;
+    if (rr.path < 0) {
+        F.hits[slot] = h;
+        return false;
+    }
+    RayD r;
+    r.o = ld3(rr.o);
+    r.d = ld3(rr.d);
+    double t;
+    int32_t obj, tri;
+    bool hit = world_closest(S, r, 0.001, INFINITY, F.use_kd != 0, stk, w, F.count_work != 0, t, obj, tri);
+    if (hit) {
+        h.t = t;
+        h.obj = obj;
+        h.tri = tri;
+    }
+    F.hits[slot] = h;
+    if (level == 0) {
+        F.prim_obj[rr.path] = hit ? obj : -1;
+        F.prim_tri[rr.path] = hit ? tri : -1;
+        F.prim_t[rr.path] = hit ? t : 0.0;
+    }
+    if (!hit) {
+        // miss: background, or the ortho grid when the camera is passed (src/lighting.rs:497-505)
+        d3 c = ld3(S.bg);
+        if (F.grid_miss) {
+            d3 g;
+            if (grid_color(S.cam, r, g)) c = g;
+        }
+        double* L = F.lvl + ((size_t)level * F.n_paths + rr.path) * 4;
+        L[0] = c.x;
+        L[1] = c.y;
+        L[2] = c.z;
+        L[3] = -1.0;
+        if (level == 0 && F.outline) {
+            F.ol_dn[(size_t)rr.path * 4] = NAN;
+        }
+    }
+    return hit;
+}
+
+// ------------------------------------------------------------ shade, part 1
+// Exact hit record, effective material, shadow sample points, reflection ray.
+// Returns true when a reflection ray was written to *refl.
+RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, int level, RayRec* refl) {
+    int slot = F.hitq[hq];
+    const RayRec& rr = F.rays[slot];
+    const HitRec& h = F.hits[slot];
+    RayD r;
+    r.o = ld3(rr.o);
+    r.d = ld3(rr.d);
+    const ObjDev& ob = S.objs[h.obj];
+    d3 p, n;
+    bool has_uv = false;
+    double u = 0, v = 0;
+    if (ob.kind == kSphere) {
+        p = add(r.o, scale(h.t, r.d));
+        n = face(r.d, divs(sub(p, ld3(ob.c)), ob.r));
+    } else if (ob.kind == kPlane) {
+        // Plane::hit uv (src/ray.rs:150-173)
+        p = add(r.o, scale(h.t, r.d));
+        d3 pn = ld3(ob.n);
+        n = face(r.d, pn);
+        d3 ua = fabs(pn.x) > 0.9 ? mk(0, 1, 0) : mk(1, 0, 0);
+        ua = unit(cross(ua, pn));
+        d3 va = unit(cross(pn, ua));
+        d3 rel = sub(p, ld3(ob.c));
+        u = dot(rel, ua);
+        v = dot(rel, va);
+        has_uv = true;
+    } else if (ob.kind == kCube) {
+        double t;
+        d3 wn;
+        cube_exact(ob, r, 0.001, h.t, t, p, wn);  // same t, point and normal as the hit
+        n = face(r.d, wn);
+    } else {
+        const MeshDev& M = S.meshes[ob.mesh];
+        double t;
+        d3 tn;
+        tri_exact(M.tri64 + (size_t)h.tri * 9, r, 0.001, INFINITY, t, tn, u, v);
+        p = add(r.o, scale(h.t, r.d));
+        n = face(r.d, tn);
+        has_uv = true;
+    }
+    // effective material (src/lighting.rs:229-242, apply_texture :9-54)
+    const MatDev& base = S.mats[ob.mat];
+    int eff = ob.mat;
+    d3 mc = ld3(base.color);
+    if (base.texture != 0 && has_uv) {
+        if (base.texture == 1) {
+            double hw = base.line_width / 2.0;
+            double um = fabs(fract_r(u / base.cell_size));
+            double vm = fabs(fract_r(v / base.cell_size));
+            bool on_u = um <= hw || um >= (1.0 - hw);
+            bool on_v = vm <= hw || vm >= (1.0 - hw);
+            if (on_u || on_v) mc = ld3(base.line_color);
+        } else {
+            int32_t cu = to_i32_sat(floor(u)), cv = to_i32_sat(floor(v));
+            int32_t sum = (int32_t)((uint32_t)cu + (uint32_t)cv);
+            if (sum % 2 != 0) {
+                eff = base.mat_b;
+                mc = ld3(S.mats[eff].color);
+            }
+        }
+    }
+    ShadeRec sr;
+    st3(sr.p, p);
+    st3(sr.n, n);
+    st3(sr.mc, mc);
+    sr.mat = eff;
+    sr.path = rr.path;
+    sr.level = level;
+    sr.base = ob.mat;
+    sr.spawned = base.has_refl && base.refl > 0.0 && (F.max_depth - level) > 1;
+    sr.pad = 0;
+    F.shade[hq] = sr;
+    if (level == 0 && F.outline) {
+        double* o = F.ol_dn + (size_t)rr.path * 4;
+        o[0] = length(sub(p, ld3(S.cam_pos)));
+        o[1] = n.x;
+        o[2] = n.y;
+        o[3] = n.z;
+    }
+    // shadow sample points, lights in scene order (src/lighting.rs:251-285)
+    uint64_t seed = F.path_seed[rr.path];
+    double* spt = F.spt + (size_t)hq * S.shadow_per_hit * 3;
+    int j = 0;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const LightDev& L = S.lights[li];
+        d3 lp = ld3(L.pos);
+        if (!L.is_area) {
+            st3(spt + 3 * j, lp);
+            ++j;
+            continue;
+        }
+        // calculate_diffuse_light_contribution seeding (src/lighting.rs:161-166)
+        uint64_t ls = seed * 0x9E3779B97F4A7C15ull;
+        ls = ls + to_u64_sat(p.x * 1000.0) * 0x85EBCA6Bull;
+        ls = ls + to_u64_sat(p.y * 1000.0) * 0xC2B2AE35ull;
+        ls = ls + to_u64_sat(p.z * 1000.0) * 0x6C8E9CF5ull;
+        StdRngDev rng;
+        rng.init(ls, 0);
+        double radius = L.diameter / 2.0;
+        d3 ldir = unit(sub(lp, p));
+        d3 up = fabs(ldir.x) < 0.9 ? mk(1, 0, 0) : mk(0, 1, 0);
+        d3 ub = unit(cross(up, ldir));
+        d3 vb = unit(cross(ldir, ub));
+        for (int s = 0; s < 16; ++s) {
+            double x, y;
+            for (;;) {  // sample_disk_point (src/lighting.rs:57-66)
+                x = rng.gen_range(-radius, radius);
+                y = rng.gen_range(-radius, radius);
+                if (x * x + y * y <= radius * radius) break;
+            }
+            d3 sp = add(add(lp, scale(x, ub)), scale(y, vb));
+            st3(spt + 3 * j, sp);
+            ++j;
+        }
+    }
+    // reflection (src/lighting.rs:469-494): mirror of the view direction from the camera
+    if (sr.spawned) {
+        d3 vd = unit(sub(ld3(S.cam_pos), p));
+        d3 rd = reflect_dir(neg(vd), n);
+        RayRec o;
+        st3(o.o, add(p, scale(0.001, n)));
+        st3(o.d, unit(rd));
+        o.path = rr.path;
+        o.level = level + 1;
+        *refl = o;
+        return true;
+    }
+    return false;
+}
+
+// ------------------------------------------------------------ shadow stage
+RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRef stk, Work& w) {
+    int hq = task / S.shadow_per_hit;
+    const ShadeRec& sr = F.shade[hq];
+    d3 p = ld3(sr.p), n = ld3(sr.n);
+    d3 sp = ld3(F.spt + (size_t)task * 3);
+    d3 ld = unit(sub(sp, p));
+    double dist = length(sub(sp, p));
+    RayD r;
+    r.o = add(p, scale(0.001, n));
+    r.d = unit(ld);
+    bool blocked = world_any(S, r, 0.001, dist, F.use_kd != 0, stk, w, F.count_work != 0);
+    F.vis[task] = blocked ? 0 : 1;
+}
+
+// ------------------------------------------------------------ shade, part 2
+// phong_lighting (src/lighting.rs:220-288) with visibility from the shadow stage, then
+// apply_fog (:297-319). Writes the level record (colour, reflectivity or -1).
+RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
+    const ShadeRec& sr = F.shade[hq];
+    d3 p = ld3(sr.p), n = ld3(sr.n), mc = ld3(sr.mc);
+    const MatDev& m = S.mats[sr.mat];
+    d3 color = scale(m.ambient * S.amb_i, hadamard(ld3(S.amb), mc));
+    d3 cam = ld3(S.cam_pos);
+    const double* spt = F.spt + (size_t)hq * S.shadow_per_hit * 3;
+    const uint8_t* vis = F.vis + (size_t)hq * S.shadow_per_hit;
+    int j = 0;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const LightDev& L = S.lights[li];
+        d3 lc = ld3(L.color);
+        if (!L.is_area) {
+            // calculate_point_light_contribution (src/lighting.rs:100-142)
+            if (vis[j]) {
+                d3 ld = unit(sub(ld3(L.pos), p));
+                double ds = fmax_r(dot(n, ld), 0.0);
+                d3 diffuse = scale((m.diffuse * ds) * L.intensity, hadamard(lc, mc));
+                d3 spec = mk(0, 0, 0);
+                if (ds > 0.0) {
+                    d3 vd = unit(sub(cam, p));
+                    d3 rd = reflect_dir(neg(ld), n);
+                    double ss = pow(fmax_r(dot(vd, rd), 0.0), m.shininess);
+                    spec = scale((m.specular * ss) * L.intensity, lc);
+                }
+                color = add(color, add(diffuse, spec));
+            } else {
+                color = add(color, mk(0, 0, 0));
+            }
+            ++j;
+            continue;
+        }
+        // calculate_diffuse_light_contribution (src/lighting.rs:145-217)
+        d3 total = mk(0, 0, 0);
+        int visible = 0;
+        for (int s = 0; s < 16; ++s, ++j) {
+            if (!vis[j]) continue;
+            ++visible;
+            d3 ld = unit(sub(ld3(spt + 3 * j), p));
+            double ds = fmax_r(dot(n, ld), 0.0);
+            d3 diffuse = scale((m.diffuse * ds) * L.intensity, hadamard(lc, mc));
+            d3 spec = mk(0, 0, 0);
+            if (ds > 0.0) {
+                d3 vd = unit(sub(cam, p));
+                d3 rd = reflect_dir(neg(ld), n);
+                double ss = pow(fmax_r(dot(vd, rd), 0.0), m.shininess);
+                spec = scale((m.specular * ss) * L.intensity, lc);
+            }
+            total = add(total, add(diffuse, spec));
+        }
+        // (total / 16) * (visible / 16)
+        color = add(color, scale((double)visible / 16.0, divs(total, 16.0)));
+    }
+    if (S.has_fog) {
+        double dist = length(sub(p, cam));
+        double f;
+        if (dist <= S.fog_start) f = 0.0;
+        else if (dist >= S.fog_end) f = 1.0;
+        else f = (dist - S.fog_start) / (S.fog_end - S.fog_start);
+        f = 1.0 - exp(-S.fog_density * f);
+        f = clamp_r(f, 0.0, 1.0);
+        d3 fc = ld3(S.fog_color);
+        color = add(scale(1.0 - f, color), scale(f, fc));  // color * (1 - f) + fog * f
+    }
+    double* Lr = F.lvl + ((size_t)sr.level * F.n_paths + sr.path) * 4;
+    Lr[0] = color.x;
+    Lr[1] = color.y;
+    Lr[2] = color.z;
+    Lr[3] = sr.spawned ? S.mats[sr.base].refl : -1.0;
+}
+
+}  // namespace rt
+
+namespace rt {
+
+// ------------------------------------------------------------ path colour
+// ray_color recursion unrolled: c_L = c_L * (1 - rho_L) + c_{L+1} * rho_L from the
+// deepest level up (src/lighting.rs:469-494), in the reference's evaluation order.
+RT_HD d3 path_color(const FrameView& F, int path) {
+    int last = 0;
+    while (last + 1 < F.max_depth) {
+        const double* L = F.lvl + ((size_t)last * F.n_paths + path) * 4;
+        if (!(L[3] >= 0.0)) break;
+        ++last;
+    }
+    const double* Ld = F.lvl + ((size_t)last * F.n_paths + path) * 4;
+    d3 c = mk(Ld[0], Ld[1], Ld[2]);
+    for (int l = last - 1; l >= 0; --l) {
+        const double* L = F.lvl + ((size_t)l * F.n_paths + path) * 4;
+        double rho = L[3];
+        c = add(scale(1.0 - rho, mk(L[0], L[1], L[2])), scale(rho, c));
+    }
+    return c;
+}
+
+// Per pixel: quincunx (centre + 4 shared corners) / 5, or the mean over samples; with
+// the outline on, the nearest first-hit depth/normal over the samples
+// (src/renderer.rs:552, :692-698, :881-886).
+RT_HD void pixel_item(const FrameView& F, int pix) {
+    int x = pix % F.W, y = pix / F.W;
+    d3 c;
+    if (F.max_depth <= 0) {
+        c = mk(0, 0, 0);
+    } else if (F.aa == 1) {
+        int base = F.W * F.H;
+        c = path_color(F, pix);
+        c = add(c, path_color(F, base + y * (F.W + 1) + x));
+        c = add(c, path_color(F, base + y * (F.W + 1) + x + 1));
+        c = add(c, path_color(F, base + (y + 1) * (F.W + 1) + x));
+        c = add(c, path_color(F, base + (y + 1) * (F.W + 1) + x + 1));
+        c = divs(c, 5.0);
+    } else {
+        d3 tot = mk(0, 0, 0);
+        for (int s = 0; s < F.N; ++s) tot = add(tot, path_color(F, pix * F.N + s));
+        c = divs(tot, (double)F.N);
+    }
+    st3(F.pix + (size_t)pix * 3, c);
+    if (F.outline) {
+        double* o = F.px_dn + (size_t)pix * 4;
+        o[0] = NAN;
+        o[1] = o[2] = o[3] = 0.0;
+        if (F.max_depth > 0) {
+            bool have = false;
+            for (int s = 0; s < F.N; ++s) {
+                const double* q = F.ol_dn + ((size_t)pix * F.N + s) * 4;
+                double d = q[0];
+                if (d == d && (!have || d < o[0])) {
+                    have = true;
+                    o[0] = d;
+                    o[1] = q[1];
+                    o[2] = q[2];
+                    o[3] = q[3];
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------ outline
+// compute_edge_strength (src/outline.rs:132-182) on per-pixel depth/normal. `fetch`
+// reads (depth, normal) of a pixel; false = background or outside the image (the
+// reference reads x+1 / y+1 past the last column/row as background, :68-82).
+template <class Fetch>
+RT_HD double edge_strength(const FrameView& F, int x, int y, Fetch fetch) {
+    double cd, cn[3];
+    bool cf = fetch(x, y, cd, cn);
+    double mx = 0.0;
+    int nbr[8][2];
+    int k = 0;
+    if (F.ol_8) {
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (!dx && !dy) continue;
+                if (x + dx >= 0 && y + dy >= 0) {
+                    nbr[k][0] = x + dx;
+                    nbr[k][1] = y + dy;
+                    ++k;
+                }
+            }
+    } else {
+        if (x > 0) { nbr[k][0] = x - 1; nbr[k][1] = y; ++k; }
+        nbr[k][0] = x + 1; nbr[k][1] = y; ++k;
+        if (y > 0) { nbr[k][0] = x; nbr[k][1] = y - 1; ++k; }
+        nbr[k][0] = x; nbr[k][1] = y + 1; ++k;
+    }
+    for (int i = 0; i < k; ++i) {
+        double nd, nn[3];
+        bool nf = fetch(nbr[i][0], nbr[i][1], nd, nn);
+        double e = 0.0;
+        if (cf && nf) {
+            double dd = fabs(cd - nd);
+            double ndd = dd / fmax_r(cd * 0.1, 0.1);
+            double dp = clamp_r(cn[0] * nn[0] + cn[1] * nn[1] + cn[2] * nn[2], -1.0, 1.0);
+            e = F.ol_wd * ndd + F.ol_wn * (1.0 - dp);
+        } else if (cf != nf) {
+            e = F.ol_wd * 2.0 + F.ol_wn * 2.0;
+        }
+        mx = fmax_r(mx, e);
+    }
+    return mx;
+}
+
+// detect_edges threshold/normalise (src/outline.rs:113-129)
+RT_HD double edge_mask_value(const FrameView& F, double e) {
+    if (e > F.ol_thr) return (e - F.ol_thr) / fmax_r(1.0 - F.ol_thr, 0.001);
+    return 0.0;
+}
+
+// dilate_edges (src/outline.rs:227-268) for one pixel, reading the undilated mask
+RT_HD double dilate_item(const FrameView& F, const double* mask, int x, int y) {
+    int radius = (int)ceil(F.ol_thick - 1.0);
+    double m = mask[(size_t)y * F.W + x];
+    for (int dy = -radius; dy <= radius; ++dy)
+        for (int dx = -radius; dx <= radius; ++dx) {
+            int nx = x + dx, ny = y + dy;
+            if (nx >= 0 && ny >= 0 && nx < F.W && ny < F.H) {
+                double dist = sqrt((double)(dx * dx + dy * dy));
+                if (dist <= F.ol_thick) {
+                    double fall = 1.0 - (dist / F.ol_thick);
+                    m = fmax_r(m, mask[(size_t)ny * F.W + nx] * fall);
+                }
+            }
+        }
+    return m;
+}
+
+// blend_colors (src/outline.rs:271-278) then create_image_from_data quantisation
+// (src/renderer.rs:899-912)
+RT_HD void finalize_item(const FrameView& F, const double* mask, int pix) {
+    double* c = F.pix + (size_t)pix * 3;
+    double r = c[0], g = c[1], b = c[2];
+    if (mask && mask[pix] > 0.0) {
+        double w = clamp_r(fmin_r(mask[pix], 1.0), 0.0, 1.0);
+        r = r * (1.0 - w) + F.ol_color[0] * w;
+        g = g * (1.0 - w) + F.ol_color[1] * w;
+        b = b * (1.0 - w) + F.ol_color[2] * w;
+    }
+    F.rgb[(size_t)pix * 3 + 0] = to_u8_sat(clamp_r(r, 0.0, 1.0) * 255.0);
+    F.rgb[(size_t)pix * 3 + 1] = to_u8_sat(clamp_r(g, 0.0, 1.0) * 255.0);
+    F.rgb[(size_t)pix * 3 + 2] = to_u8_sat(clamp_r(b, 0.0, 1.0) * 255.0);
+}
+
+}  // namespace rt

@@ -1,0 +1,202 @@
+"""Seeded procedural scenes for the benchmark configurations.
+
+Every generator is deterministic (SplitMix64 streams) and writes a Scene JSON in
+the reference's schema (src/scene.rs:34-446) plus binary STL meshes (f32, as the
+reference's loader reads them, src/mesh.rs:919-1003), so the oracle and the HIP
+path consume identical vertices.
+
+  config1   800x600 ortho sphere over a reflective grid plane, point light, quincunx.
+  config2   ~82k-triangle noisy icosphere + 3 transformed cubes + grid plane,
+            perspective 50 deg, disk area light (16 samples), fog 0.05, depth 4.
+  config3   1000 reflective spheres on a ring over a grid plane, 4 point lights,
+            stochastic 4 spp, depth 10, outline on.
+"""
+import json
+import math
+import os
+import struct
+
+import numpy as np
+
+MASK64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014): the seeded stream of every generator here."""
+
+    def __init__(self, seed):
+        self.state = seed & MASK64
+
+    def next_u64(self):
+        self.state = (self.state + 0x9E3779B97F4A7C15) & MASK64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        return z ^ (z >> 31)
+
+    def uniform(self, lo, hi):
+        return lo + (hi - lo) * ((self.next_u64() >> 11) * (1.0 / 9007199254740992.0))
+
+
+def material(color, ambient=0.1, diffuse=0.7, specular=0.3, shininess=32, reflectivity=None, texture=None):
+    m = {"color": color, "ambient": ambient, "diffuse": diffuse, "specular": specular, "shininess": shininess}
+    if reflectivity is not None:
+        m["reflectivity"] = reflectivity
+    if texture is not None:
+        m["texture"] = texture
+    return m
+
+
+def hex_color(rgb):
+    return "#%02X%02X%02X" % tuple(int(max(0, min(255, round(c * 255)))) for c in rgb)
+
+
+def write_stl(path, tris):
+    """Binary STL of an (n, 3, 3) float array (normal field zero, never read by the path)."""
+    tris = np.asarray(tris, dtype=np.float32).reshape(-1, 3, 3)
+    rec = np.zeros(len(tris), dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+    rec["v"] = tris
+    with open(path, "wb") as f:
+        f.write(b"rtrace_amd procedural mesh".ljust(80, b" "))
+        f.write(struct.pack("<I", len(tris)))
+        f.write(rec.tobytes())
+
+
+def icosphere(subdiv, radius, noise, seed):
+    """Icosahedron subdivided `subdiv` times (20 * 4^subdiv faces), vertices pushed to
+    radius * (1 + U(-noise, noise)) per vertex (shared vertices get one draw)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+             (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+             (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    rng = SplitMix64(seed)
+    V = np.array(verts)
+    scale = np.array([radius * (1.0 + rng.uniform(-noise, noise)) for _ in range(len(V))])
+    V = V * scale[:, None]
+    F = np.array(faces, dtype=np.int64)
+    return V[F]
+
+
+def _camera_persp(pos, target, fov, aspect=(16, 9)):
+    return {"kind": "perspective", "position": list(pos), "target": list(target), "up": [0, 0, 1],
+            "width": aspect[0], "height": aspect[1], "fov": fov}
+
+
+def _settings(background="#101822", fog=None, outline=None):
+    s = {"ambient_illumination": {"color": "#FFFFFF", "intensity": 0.1}, "background_color": background}
+    if fog:
+        s["fog"] = fog
+    if outline:
+        s["outline"] = outline
+    return s
+
+
+def config1(out_dir):
+    """800x600 (Renderer::new(800, 600)); the 6x6 ortho view widens to 8x6 (src/camera.rs:79)."""
+    sc = {
+        "camera": {"kind": "ortho", "position": [0, -10, 5], "target": [0, 0, 0], "up": [0, 0, 1],
+                   "width": 6, "height": 6},
+        "objects": [
+            {"kind": "sphere", "center": [0, 0, 0], "radius": 1.5,
+             "material": material("#FF4444", 0.1, 0.8, 0.4, 32)},
+            {"kind": "plane", "point": [0, 0, -1.5], "normal": [0, 0, 1],
+             "material": material("#CCCCCC", 0.1, 0.7, 0.2, 16, reflectivity=0.3,
+                                  texture={"type": "grid", "line_color": "#333333", "line_width": 0.05,
+                                           "cell_size": 1.0})},
+        ],
+        "lights": [{"position": [3, -3, 5], "color": "#FFFFFF", "intensity": 1.0}],
+        "scene_settings": _settings("#001122"),
+    }
+    return _write(out_dir, "config1", sc)
+
+
+def config2(out_dir, subdiv=6, light="disk"):
+    """Procedural mesh scene. subdiv=6 gives 81,920 triangles (BASELINE: ~80k)."""
+    os.makedirs(out_dir, exist_ok=True)
+    rng = SplitMix64(2)
+    tris = icosphere(subdiv, 1.5, 0.05, seed=2)
+    mesh_name = f"config2_icosphere_s{subdiv}.stl"
+    write_stl(os.path.join(out_dir, mesh_name), tris)
+    objects = [{"kind": "mesh", "filename": mesh_name,
+                "material": material("#8FB8DE", 0.1, 0.7, 0.4, 48, reflectivity=0.25)}]
+    for i in range(3):
+        ang = 2 * math.pi * i / 3 + rng.uniform(-0.3, 0.3)
+        r = rng.uniform(2.6, 3.4)
+        tf = [f"rotate({rng.uniform(0, 90):.4f}, {rng.uniform(0, 90):.4f}, {rng.uniform(0, 90):.4f})",
+              f"scale({rng.uniform(0.8, 1.3):.4f}, {rng.uniform(0.8, 1.3):.4f}, {rng.uniform(0.8, 1.3):.4f})",
+              f"translate({r * math.cos(ang):.4f}, {r * math.sin(ang):.4f}, {rng.uniform(-0.8, 0.4):.4f})"]
+        col = hex_color((rng.uniform(0.3, 1), rng.uniform(0.3, 1), rng.uniform(0.3, 1)))
+        objects.append({"kind": "cube", "center": [0, 0, 0], "size": [1, 1, 1], "transform": tf,
+                        "material": material(col, 0.1, 0.7, 0.3, 32, reflectivity=0.4)})
+    objects.append({"kind": "plane", "point": [0, 0, -1.8], "normal": [0, 0, 1],
+                    "material": material("#D8D8D0", 0.1, 0.7, 0.1, 8, reflectivity=0.2,
+                                         texture={"type": "grid", "line_color": "#404040", "line_width": 0.04,
+                                                  "cell_size": 1.0})})
+    lights = [{"position": [3, -4, 6], "color": "#FFFFFF", "intensity": 1.0, "diameter": 1.0}]
+    sc = {"camera": _camera_persp([0, -7.5, 3.5], [0, 0, 0], 50), "objects": objects, "lights": lights,
+          "scene_settings": _settings("#202838", fog={"color": "#B0C4DE", "density": 0.05, "start": 2.0,
+                                                       "end": 20.0})}
+    return _write(out_dir, f"config2_s{subdiv}", sc)
+
+
+def config3(out_dir, n_spheres=1000):
+    """1000 reflective spheres on a ring (annulus 14..20 around the origin, resting on
+    the plane z = 0), 4 point lights; render with stochastic 4 spp, depth 10."""
+    rng = SplitMix64(3)
+    objects = []
+    for _ in range(n_spheres):
+        r = rng.uniform(0.1, 0.6)
+        a = rng.uniform(0, 2 * math.pi)
+        d = math.sqrt(rng.uniform(14.0 ** 2, 20.0 ** 2))
+        col = hex_color((rng.uniform(0.2, 1), rng.uniform(0.2, 1), rng.uniform(0.2, 1)))
+        objects.append({"kind": "sphere", "center": [round(d * math.cos(a), 5), round(d * math.sin(a), 5), round(r, 5)],
+                        "radius": round(r, 5),
+                        "material": material(col, 0.1, 0.7, 0.4, 32, reflectivity=round(rng.uniform(0.2, 0.9), 4))})
+    objects.append({"kind": "plane", "point": [0, 0, 0], "normal": [0, 0, 1],
+                    "material": material("#C8C8C8", 0.1, 0.7, 0.1, 8,
+                                         texture={"type": "grid", "line_color": "#505050", "line_width": 0.05,
+                                                  "cell_size": 2.0})})
+    lights = [{"position": [x, y, 12], "color": "#FFFFFF", "intensity": 0.45}
+              for x, y in [(15, 15), (-15, 15), (-15, -15), (15, -15)]]
+    outline = {"enabled": True, "depth_weight": 1.0, "normal_weight": 1.0, "threshold": 0.1, "color": "#000000",
+               "thickness": 1.0, "use_8_neighbors": False}
+    sc = {"camera": _camera_persp([0, -34, 16], [0, 0, 0], 50), "objects": objects, "lights": lights,
+          "scene_settings": _settings("#0A0A14", outline=outline)}
+    return _write(out_dir, f"config3_n{n_spheres}", sc)
+
+
+def _write(out_dir, name, scene):
+    os.makedirs(out_dir, exist_ok=True)
+    path = os.path.join(out_dir, name + ".json")
+    with open(path, "w") as f:
+        json.dump(scene, f, indent=1)
+    return path
+
+
+# Render settings of each configuration (Renderer fields); the 1080p variants are the
+# headline "Mrays/s at 1080p" measurement of the same scene.
+RENDER = {
+    "config1": dict(width=800, height=600, aa="quincunx", samples=1, max_depth=10),
+    "config2": dict(width=1280, height=720, aa="quincunx", samples=1, max_depth=4),
+    "config2_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
+    "config3": dict(width=1920, height=1080, aa="stochastic", samples=4, max_depth=10),
+}

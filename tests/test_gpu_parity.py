@@ -1,0 +1,226 @@
+"""HIP path vs the CPU f64 oracle, through the C ABI (ctypes), on an MI355X.
+
+Bars (stated per test):
+* primary-ray hit ids (object, triangle): bit-exact except an epsilon-band of at most
+  EPS_BAND_FRAC of the primary rays (rays grazing triangle edges, where the FP32
+  prefilter's error bound can exceed its margin); the mismatch count is printed;
+* 8-bit pixels: within +-1 LSB on >= 99.9 % of pixels, max abs diff printed.
+The committed reference renders (tests/golden/ref/images) equal the oracle byte for
+byte (tests/test_oracle.py), so matching the oracle is matching the reference.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REF_IMAGES, REF_SCENES
+from oracle import pyoracle as O
+from test_oracle import DOC_CASES, _plus_rays, load_png, scene_path
+
+pytestmark = pytest.mark.gpu
+
+EPS_BAND_FRAC = 1e-4   # hit-id mismatches allowed, as a fraction of primary rays
+PIX_LSB = 1            # +-1 LSB
+PIX_FRAC = 0.999       # on >= 99.9 % of pixels
+
+
+@pytest.fixture(scope="module")
+def R():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    import rtrace_amd
+
+    rtrace_amd.lib()
+    return rtrace_amd
+
+
+def gpu_render(R, path, opts, with_hits=True):
+    r = R.Renderer(opts.width, opts.height)
+    r.max_depth = opts.max_depth
+    r.use_kdtree = bool(opts.use_kdtree)
+    r.samples = opts.samples
+    r.anti_aliasing_mode = opts.aa_mode
+    r.seed = opts.seed if opts.has_seed else None
+    if opts.outline_enabled:
+        r.outline_config = R.OutlineConfig(opts.outline_depth_weight, opts.outline_normal_weight,
+                                           opts.outline_threshold, tuple(opts.outline_color),
+                                           bool(opts.outline_use_8_neighbors), opts.outline_thickness)
+    scene = R.Scene.from_json_file(path)
+    img = r.render(scene)
+    hits = None
+    if with_hits:
+        W, H = opts.width, opts.height
+        n = W * H + (W + 1) * (H + 1) if opts.aa_mode == O.AA_QUINCUNX else W * H * opts.samples
+        hits = R.last_primary_hits(n)
+    return img, r.last_stats, hits
+
+
+def compare(name, gpu, ref, ghits=None, rhits=None):
+    d = np.abs(gpu.astype(int) - ref.astype(int)).max(-1)
+    frac = float(np.mean(d <= PIX_LSB))
+    msg = f"{name}: pixels exact {np.mean(d == 0) * 100:.4f}% within 1 LSB {frac * 100:.4f}% max {d.max()}"
+    mism = 0
+    if ghits is not None:
+        mism = int(np.count_nonzero((ghits[0] != rhits[0]) | (ghits[1] != rhits[1])))
+        same = (ghits[0] == rhits[0]) & (ghits[1] == rhits[1])
+        tdiff = np.abs(ghits[2] - rhits[2])[same]
+        msg += f" | hit-id mismatches {mism}/{len(ghits[0])} max |dt| on equal ids {tdiff.max() if tdiff.size else 0:.3g}"
+        assert mism <= max(1, EPS_BAND_FRAC * len(ghits[0])), msg
+    print(msg)
+    assert frac >= PIX_FRAC, msg
+    return d, mism
+
+
+@pytest.mark.parametrize("name,scene,kw", DOC_CASES, ids=[c[0] for c in DOC_CASES])
+def test_doc_scene_matches_reference_render(R, name, scene, kw, tmp_path):
+    path = scene_path(scene, str(tmp_path))
+    opts = O.cli_options(path, size=500, **kw)
+    img, st, gh = gpu_render(R, path, opts)
+    ref, ost, rh = O.render(path, opts, want_hits=True)
+    assert np.array_equal(ref, load_png(name))  # the oracle is the reference render
+    compare(name, img, ref, gh, rh)
+    assert st.rays_primary == ost["rays_primary"]
+    assert st.rays_reflect == ost["rays_reflect"]
+    assert st.rays_shadow == ost["rays_shadow"]
+
+
+@pytest.mark.parametrize("scene,w,h,aa,spp", [
+    ("diffuse_light_demo.json", 200, 150, "quincunx", 1),
+    ("diffuse_light_demo.json", 160, 120, "stochastic", 4),
+    ("diffuse_light_demo.json", 160, 120, "stochastic", 1),
+    ("diffuse_light_demo.json", 120, 90, "no-jitter", 3),
+    ("plus_front.json", 200, 150, "quincunx", 1),
+    ("plus_side.json", 200, 150, "quincunx", 1),
+    ("plus_perspective.json", 200, 150, "quincunx", 1),
+    ("icosahedron_front.json", 200, 150, "quincunx", 1),
+])
+def test_area_light_and_mesh_scenes(R, scene, w, h, aa, spp):
+    path = os.path.join(REF_SCENES, scene)
+    opts = O.make_opts(w, h, aa=aa, samples=spp)
+    img, st, gh = gpu_render(R, path, opts)
+    ref, ost, rh = O.render(path, opts, want_hits=True)
+    compare(f"{scene} {aa} {spp}", img, ref, gh, rh)
+    assert st.rays_shadow == ost["rays_shadow"]
+
+
+def _config_cases():
+    return [("config1", 800, 600, "quincunx", 1, 10), ("config2_s4", 320, 180, "quincunx", 1, 4),
+            ("config2_s6", 160, 90, "quincunx", 1, 4), ("config3_n1000", 192, 108, "stochastic", 4, 10)]
+
+
+@pytest.fixture(scope="module")
+def config_dir(tmp_path_factory):
+    from rtrace_amd import scenes
+    d = str(tmp_path_factory.mktemp("configs"))
+    return {"config1": scenes.config1(d), "config2_s4": scenes.config2(d, subdiv=4),
+            "config2_s6": scenes.config2(d, subdiv=6), "config3_n1000": scenes.config3(d, 1000)}
+
+
+@pytest.mark.parametrize("name,w,h,aa,spp,depth", _config_cases(), ids=[c[0] for c in _config_cases()])
+def test_config_scenes_vs_oracle(R, config_dir, name, w, h, aa, spp, depth):
+    path = config_dir[name]
+    with open(path) as f:
+        outline = O.outline_from_scene(json.load(f))
+    opts = O.make_opts(w, h, aa=aa, samples=spp, max_depth=depth, outline=outline if aa != "quincunx" else None)
+    img, st, gh = gpu_render(R, path, opts)
+    ref, ost, rh = O.render(path, opts, want_hits=True)
+    compare(name, img, ref, gh, rh)
+    for k in ("rays_primary", "rays_reflect", "rays_shadow"):
+        assert getattr(st, k) == ost[k], k
+
+
+def test_kdtree_equals_brute_force_on_gpu(R):
+    path = os.path.join(REF_SCENES, "plus_front.json")
+    a, _, _ = gpu_render(R, path, O.make_opts(200, 150, use_kdtree=True), with_hits=False)
+    b, _, _ = gpu_render(R, path, O.make_opts(200, 150, use_kdtree=False), with_hits=False)
+    ref, _, _ = O.render(path, O.make_opts(200, 150, use_kdtree=False))
+    assert np.array_equal(a, b)
+    compare("plus_front brute force", b, ref)
+
+
+def test_intersect_kdtree_consistency_rays(R):
+    """tools/test_kdtree_consistency.rs rays through the rt_intersect (World::hit) seam."""
+    mesh = os.path.join(REF_SCENES, "plus.stl")
+    raw = open(mesh, "rb").read()
+    n = int.from_bytes(raw[80:84], "little")
+    rec = np.frombuffer(raw[84:84 + 50 * n], dtype=np.dtype([("n", "<f4", 3), ("v", "<f4", (3, 3)), ("a", "<u2")]))
+    v = rec["v"].astype(np.float64).reshape(-1, 3)
+    rays = _plus_rays(v.min(0), v.max(0))
+    scene = {"camera": {"kind": "ortho", "position": [0, -5, 2], "target": [0, 0, 0], "up": [0, 0, 1],
+                        "width": 10, "height": 10},
+             "objects": [{"kind": "mesh", "filename": "plus.stl",
+                          "material": {"color": "#FFFFFF", "ambient": 0.1, "diffuse": 0.7, "specular": 0.3,
+                                       "shininess": 32}}],
+             "lights": [], "scene_settings": {"ambient_illumination": {"color": "#FFFFFF", "intensity": 0.1}}}
+    sc = R.Scene.from_json_str(json.dumps(scene), REF_SCENES)
+    o = np.array([r[0] for r in rays], np.float64)
+    d = np.array([r[1] for r in rays], np.float64)
+    for use_kd in (True, False):
+        obj, tri, t = R.intersect(sc, o, d, use_kdtree=use_kd)
+        for i in range(len(rays)):
+            rt_, rtt = O.mesh_hit(mesh, use_kd, o[i], d[i])
+            assert (obj[i] >= 0) == (rt_ >= 0), i
+            if rt_ >= 0:
+                assert t[i] == rtt, (i, t[i], rtt)
+                assert tri[i] == rt_, (i, tri[i], rt_)
+
+
+def test_zero_samples_error(R, tmp_path):
+    path = os.path.join(REF_SCENES, "object-sphere.json")
+    r = R.Renderer(10, 10)
+    r.samples = 0
+    with pytest.raises(R.RtraceError, match="Samples must be greater than 0"):
+        r.render(R.Scene.from_json_file(path))
+
+
+def test_scene_errors_are_reported(R, tmp_path):
+    bad = tmp_path / "bad.json"
+    bad.write_text('{"camera": {"kind": "fisheye", "position": [0,0,1], "target": [0,0,0], "up": [0,1,0], '
+                   '"width": 1, "height": 1}, "objects": [], "lights": [], '
+                   '"scene_settings": {"ambient_illumination": {"color": "#FFFFFF", "intensity": 0.1}}}')
+    with pytest.raises(R.RtraceError, match="Unsupported camera type"):
+        R.Scene.from_json_file(str(bad))
+    with pytest.raises(R.RtraceError, match="missing field"):
+        R.Scene.from_json_str('{"camera": {}}')
+
+
+def test_max_depth_zero_and_one(R):
+    path = os.path.join(REF_SCENES, "material-reflectivity.json")
+    for depth in (0, 1, 2):
+        opts = O.make_opts(120, 90, max_depth=depth)
+        img, st, gh = gpu_render(R, path, opts)
+        ref, ost, rh = O.render(path, opts, want_hits=True)
+        compare(f"max_depth={depth}", img, ref, gh, rh)
+        assert st.rays_reflect == ost["rays_reflect"]
+
+
+def test_deterministic_repeat_and_seed(R):
+    """src/renderer.rs:1074-1218: repeated renders are byte-identical; seeds matter."""
+    path = os.path.join(REF_SCENES, "diffuse_light_demo.json")
+    a, _, _ = gpu_render(R, path, O.make_opts(96, 72, aa="stochastic", samples=4, seed=42), with_hits=False)
+    b, _, _ = gpu_render(R, path, O.make_opts(96, 72, aa="stochastic", samples=4, seed=42), with_hits=False)
+    c, _, _ = gpu_render(R, path, O.make_opts(96, 72, aa="stochastic", samples=4, seed=7), with_hits=False)
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+    ref, _, _ = O.render(path, O.make_opts(96, 72, aa="stochastic", samples=4, seed=7))
+    compare("seed 7", c, ref)
+
+
+def test_full_size_config2_invariants(R, config_dir):
+    """1920x1080 config2 (81,920 triangles): determinism and exact ray accounting
+    (quincunx primaries, 16 shadow rays per hit, reflections bounded by hits)."""
+    path = config_dir["config2_s6"]
+    opts = O.make_opts(1920, 1080, max_depth=4)
+    a, st, gh = gpu_render(R, path, opts)
+    b, st2, _ = gpu_render(R, path, opts, with_hits=False)
+    assert np.array_equal(a, b)
+    W, H = 1920, 1080
+    assert st.rays_primary == W * H + (W + 1) * (H + 1)
+    hits0 = int(np.count_nonzero(gh[0] >= 0))
+    assert st.rays_shadow >= 16 * hits0
+    assert st.rays_shadow % 16 == 0
+    assert 0 < st.rays_reflect <= st.rays_shadow // 16
+    assert st.rays_total == st2.rays_total

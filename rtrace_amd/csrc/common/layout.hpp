@@ -2,15 +2,17 @@
 // the kernels.
 //
 // Per mesh:
-//   nodes   : KdNode[16 B], depth-first pre-order (left child = self + 1). Inner
-//             nodes keep the reference's f64 split so that every traversal decision
-//             (origin side, far-child visit, src/mesh.rs:422-462) is taken in f64
-//             exactly as the reference takes it.
+//   nodes   : KdNode[48 B], depth-first pre-order (left child = self + 1). Each node
+//             carries the tight AABB of its triangles (the `bounds` every KdNode of
+//             the reference holds, src/mesh.rs:41-55) in f32 relative to the mesh
+//             centre, rounded outward, and inner nodes keep the reference's f64 split
+//             so that every traversal decision (origin side, far-child visit,
+//             src/mesh.rs:422-462) is taken in f64 exactly as the reference takes it.
 //   recs    : TriRec[48 B] = 3 x float4, one per (leaf, triangle) pair in leaf order
 //             (triangles are copied into their leaves, no index indirection):
 //             v0 - centre, e1 = v1 - v0, e2 = v2 - v0 (computed in f64, rounded
-//             to f32), and the triangle index. The brute-force order (all triangles
-//             in file order, src/ray.rs:650-671) is one extra leaf at the end.
+//             to f32), the triangle index and max(|e1|_1, |e2|_1). The brute-force
+//             order (all triangles in file order, src/ray.rs:650-671) is one extra leaf.
 //   tri64   : 9 x f64 per triangle (the reference's vertices) for the exact re-test.
 #pragma once
 
@@ -21,13 +23,16 @@ namespace rt {
 enum : uint32_t { kAxisLeaf = 3u, kFlagMisplaced = 4u };
 
 struct alignas(16) KdNode {
-    double split;  // inner: split position (f64, as built by the reference)
-    uint32_t a;    // inner: index of the right child; leaf: first TriRec
-    uint32_t b;    // bits 0-1 axis (3 = leaf); inner bit 2 = a child holds a triangle
-                   // moved there by the empty-side rule (src/mesh.rs:174-178);
-                   // leaf: count << 2 | 3
+    float bmin[3];  // tight bounds of the node's triangles, relative to the mesh centre
+    float bmax[3];
+    uint32_t a;     // inner: index of the right child; leaf: first TriRec
+    uint32_t b;     // bits 0-1 axis (3 = leaf); inner bit 2 = a child holds a triangle
+                    // moved there by the empty-side rule (src/mesh.rs:174-178);
+                    // leaf: count << 2 | 3
+    double split;   // inner: split position (f64, as built by the reference)
+    uint32_t pad[2];
 };
-static_assert(sizeof(KdNode) == 16, "KdNode is 16 bytes");
+static_assert(sizeof(KdNode) == 48, "KdNode is 48 bytes");
 
 constexpr int kMaxKdDepth = 16;  // KdTree::new(.., 16, 10) (src/mesh.rs:1036-1039)
 constexpr int kStackSize = kMaxKdDepth + 2;
@@ -37,6 +42,7 @@ struct MeshDev {
     const KdNode* nodes;
     const float* recs;  // 12 floats per TriRec
     const double* tri64;
+    const double* nb64;  // per node: f64 tight bounds (min xyz, max xyz) for the tie resolver
     uint32_t root_kd;   // 0
     uint32_t root_bf;   // brute-force leaf
     uint32_t n_tris;

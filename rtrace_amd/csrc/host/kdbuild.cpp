@@ -41,8 +41,32 @@ struct Builder {
         const double* p = v + (size_t)t * 9;
         return fmax_r(fmax_r(p[axis], p[3 + axis]), p[6 + axis]);
     }
+    // tight bounds of a triangle list (compute_bounds, src/mesh.rs:200-215), stored in
+    // f32 relative to the centre and rounded outward so the f32 box contains the f64 one
+    void set_bounds(KdNode& n, const std::vector<int>& tris) const {
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int t : tris)
+            for (int k = 0; k < 3; ++k) {
+                mn[k] = fmin_r(mn[k], tmin(t, k));
+                mx[k] = fmax_r(mx[k], tmax(t, k));
+            }
+        const double c[3] = {cx, cy, cz};
+        if (tris.empty())
+            for (int k = 0; k < 3; ++k) mn[k] = mx[k] = 0.0;  // compute_bounds of an empty list
+        out->nb64.insert(out->nb64.end(), mn, mn + 3);
+        out->nb64.insert(out->nb64.end(), mx, mx + 3);
+        for (int k = 0; k < 3; ++k) {
+            double lo = mn[k] - c[k], hi = mx[k] - c[k];
+            float flo = (float)lo, fhi = (float)hi;
+            if ((double)flo > lo) flo = std::nextafter(flo, -INFINITY);
+            if ((double)fhi < hi) fhi = std::nextafter(fhi, INFINITY);
+            n.bmin[k] = flo;
+            n.bmax[k] = fhi;
+        }
+    }
     void emit_leaf(const std::vector<int>& tris) {
         KdNode n{};
+        set_bounds(n, tris);
         n.split = 0.0;
         n.a = (uint32_t)(out->recs.size() / 12);
         n.b = ((uint32_t)tris.size() << 2) | kAxisLeaf;
@@ -106,6 +130,7 @@ struct Builder {
         }
         size_t me = out->nodes.size();
         KdNode n{};
+        set_bounds(n, tris);
         n.split = split;
         n.b = (uint32_t)axis | flags;
         out->nodes.push_back(n);
@@ -145,6 +170,11 @@ void build_flat_kd(const std::vector<double>& verts, FlatKd& out) {
     // brute-force order: one leaf over every triangle in file order (src/ray.rs:650-671)
     out.root_bf = (uint32_t)out.nodes.size();
     KdNode bf{};
+    {
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        b.set_bounds(bf, all);
+    }
     bf.a = (uint32_t)(out.recs.size() / 12);
     bf.b = ((uint32_t)n << 2) | kAxisLeaf;
     out.nodes.push_back(bf);

@@ -11,6 +11,7 @@ namespace rt {
 struct FlatKd {
     std::vector<KdNode> nodes;
     std::vector<float> recs;  // 12 floats per TriRec
+    std::vector<double> nb64; // 6 doubles per node: f64 tight bounds
     uint32_t root_kd = 0;
     uint32_t root_bf = 0;
     double center[3] = {0, 0, 0};

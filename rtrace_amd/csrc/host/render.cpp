@@ -155,14 +155,17 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         const std::vector<double>& v = H.meshes[m];
         FlatKd& kd = D.kd[m];
         build_flat_kd(v, kd);
-        auto nodes = std::make_unique<DevBuf>(), recs = std::make_unique<DevBuf>(), tri64 = std::make_unique<DevBuf>();
+        auto nodes = std::make_unique<DevBuf>(), recs = std::make_unique<DevBuf>(), tri64 = std::make_unique<DevBuf>(),
+             nb64 = std::make_unique<DevBuf>();
         upload(*nodes, kd.nodes, st);
         upload(*recs, kd.recs, st);
         upload(*tri64, v, st);
+        upload(*nb64, kd.nb64, st);
         MeshDev md{};
         md.nodes = nodes->as<KdNode>();
         md.recs = recs->as<float>();
         md.tri64 = tri64->as<double>();
+        md.nb64 = nb64->as<double>();
         md.root_kd = kd.root_kd;
         md.root_bf = kd.root_bf;
         md.n_tris = (uint32_t)(v.size() / 9);
@@ -184,6 +187,7 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         D.mesh_bufs.push_back(std::move(nodes));
         D.mesh_bufs.push_back(std::move(recs));
         D.mesh_bufs.push_back(std::move(tri64));
+        D.mesh_bufs.push_back(std::move(nb64));
     }
     for (const rt_object_desc& o : H.objects) {
         ObjDev d{};

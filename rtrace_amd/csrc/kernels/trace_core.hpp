@@ -134,6 +134,107 @@ constexpr float kEps32 = 5.9604645e-08f;  // 2^-24
 constexpr float kTolC = 64.0f * kEps32;   // slack factor for FP32 error bounds
 
 // ------------------------------------------------------------ mesh traversal
+// Per-ray FP32 state for a mesh: origin relative to the mesh centre, direction, guarded
+// reciprocal and the per-axis slack that covers rounding of the origin to f32.
+struct MeshRay {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz;
+    float sx, sy, sz;  // slack in t per axis
+};
+
+RT_HD MeshRay make_mesh_ray(const MeshDev& M, const RayD& r) {
+    MeshRay q;
+    q.ox = (float)(r.o.x - M.cx);
+    q.oy = (float)(r.o.y - M.cy);
+    q.oz = (float)(r.o.z - M.cz);
+    q.dx = (float)r.d.x;
+    q.dy = (float)r.d.y;
+    q.dz = (float)r.d.z;
+    // a zero component gets a tiny stand-in so 0 * inf never makes a NaN slab bound
+    q.ix = 1.0f / (fabsf(q.dx) < 1e-30f ? copysignf(1e-30f, q.dx) : q.dx);
+    q.iy = 1.0f / (fabsf(q.dy) < 1e-30f ? copysignf(1e-30f, q.dy) : q.dy);
+    q.iz = 1.0f / (fabsf(q.dz) < 1e-30f ? copysignf(1e-30f, q.dz) : q.dz);
+    float sp = kTolC * (fabsf(q.ox) + fabsf(q.oy) + fabsf(q.oz) + M.radius);
+    q.sx = sp * fabsf(q.ix);
+    q.sy = sp * fabsf(q.iy);
+    q.sz = sp * fabsf(q.iz);
+    return q;
+}
+
+// Conservative slab test of a node's tight box, clipped to [c0, c1]; false = empty.
+RT_HD bool box_clip(const MeshRay& q, const float* bmin, const float* bmax, float& c0, float& c1) {
+    float ax = (bmin[0] - q.ox) * q.ix, bx = (bmax[0] - q.ox) * q.ix;
+    float ay = (bmin[1] - q.oy) * q.iy, by = (bmax[1] - q.oy) * q.iy;
+    float az = (bmin[2] - q.oz) * q.iz, bz = (bmax[2] - q.oz) * q.iz;
+    float lo = fmaxf(fmaxf(fminf(ax, bx) - q.sx, fminf(ay, by) - q.sy), fminf(az, bz) - q.sz);
+    float hi = fminf(fminf(fmaxf(ax, bx) + q.sx, fmaxf(ay, by) + q.sy), fmaxf(az, bz) + q.sz);
+    lo -= kTolC * fabsf(lo);
+    hi += kTolC * fabsf(hi);
+    c0 = fmaxf(c0, lo);
+    c1 = fminf(c1, hi);
+    return c0 <= c1;
+}
+
+// ray_intersects_bounds (src/mesh.rs:218-255) in f64 on a node's exact bounds.
+RT_HD bool line_hits_box64(const RayD& r, const double* b) {
+    double t0 = -INFINITY, t1 = INFINITY;
+    for (int a = 0; a < 3; ++a) {
+        double da = a == 0 ? r.d.x : (a == 1 ? r.d.y : r.d.z);
+        double oa = a == 0 ? r.o.x : (a == 1 ? r.o.y : r.o.z);
+        if (fabs(da) < 1e-9) {
+            if (oa < b[a] || oa > b[3 + a]) return false;
+        } else {
+            double inv = 1.0 / da;
+            double ta = (b[a] - oa) * inv, tb = (b[3 + a] - oa) * inv;
+            if (ta > tb) {
+                double x = ta;
+                ta = tb;
+                tb = x;
+            }
+            t0 = fmax_r(t0, ta);
+            t1 = fmin_r(t1, tb);
+            if (t0 > t1) return false;
+        }
+    }
+    return t1 >= 0.0;
+}
+
+// Exact tie-break: several triangles of this mesh hit at exactly `best`. The reference
+// keeps the first one its traversal reaches (strict `<`, src/ray.rs:633), so walk the
+// tree in the reference's order (src/mesh.rs:399-465: origin side first, far side when
+// t_split >= 0, leaves gated by their line-vs-box test) without any pruning and return
+// the first triangle whose exact t equals `best`. Runs only for tied rays.
+RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, double best, StackRef stk) {
+    uint32_t node = M.root_kd;
+    int sp = 0;
+    for (;;) {
+        const KdNode& nd = M.nodes[node];
+        uint32_t ax = nd.b & 3u;
+        if (ax != kAxisLeaf) {
+            double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
+            double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
+            bool left_near = oa <= nd.split;
+            uint32_t nearc = left_near ? node + 1 : nd.a;
+            uint32_t farc = left_near ? nd.a : node + 1;
+            bool far_ok = !(fabs(da) < 1e-9) && ((oa == nd.split) || ((oa < nd.split) == (da > 0.0)));
+            if (far_ok) stk.node[(sp++) * stk.stride] = farc;
+            node = nearc;
+            continue;
+        }
+        if (line_hits_box64(r, M.nb64 + (size_t)node * 6)) {
+            uint32_t first = nd.a, cnt = nd.b >> 2;
+            for (uint32_t k = 0; k < cnt; ++k) {
+                uint32_t ti = f32_to_bits(M.recs[(size_t)(first + k) * 12 + 9]);
+                double tt, uu, vv;
+                d3 nn;
+                if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv) && tt == best)
+                    return (int32_t)ti;
+            }
+        }
+        if (sp == 0) return -1;
+        node = stk.node[(--sp) * stk.stride];
+    }
+}
+
 // kAny = shadow (any hit with tmin < t < tmax); otherwise closest hit improving `best`
 // strictly (t < best, first visited wins ties, src/ray.rs:620-649).
 //
@@ -141,148 +242,140 @@ constexpr float kTolC = 64.0f * kEps32;   // slack factor for FP32 error bounds
 // f64: origin side first (origin <= split -> left), far side only when t_split >= 0
 // (sign test, no division), only the origin side when |d_axis| < 1e-9. On top of that,
 // FP32 [t0, t1] intervals with conservative slack skip subtrees that cannot hold a hit
-// before `best`; a node whose child received a triangle by the empty-side rule
-// (kFlagMisplaced) passes its whole interval to both children.
+// before `best`: the split plane clips the interval (except below a node whose child
+// received a triangle by the empty-side rule, kFlagMisplaced) and every node's tight
+// box clips it again. Skipped subtrees hold no triangle the reference could accept
+// ahead of the current best, so the candidate set and its visit order are the
+// reference's.
 template <bool kAny>
 RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
                       StackRef stk, Work& w, bool count) {
-    const float ox = (float)(r.o.x - M.cx), oy = (float)(r.o.y - M.cy), oz = (float)(r.o.z - M.cz);
-    const float dx = (float)r.d.x, dy = (float)r.d.y, dz = (float)r.d.z;
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const MeshRay q = make_mesh_ray(M, r);
     const float R = M.radius;  // max |v - centre|_1 over the mesh
-
-    // mesh AABB (src/ray.rs:586-610), conservative
-    float c0 = (float)tmin, c1 = (float)best;
-    {
-        const float o3[3] = {ox, oy, oz}, d3_[3] = {dx, dy, dz}, i3[3] = {ix, iy, iz};
-        for (int a = 0; a < 3; ++a) {
-            float lo = M.bmin[a], hi = M.bmax[a];
-            if (d3_[a] == 0.0f) {
-                float tol = kTolC * (fabsf(o3[a]) + R);
-                if (o3[a] < lo - tol || o3[a] > hi + tol) return false;
-                continue;
-            }
-            float ta = (lo - o3[a]) * i3[a], tb = (hi - o3[a]) * i3[a];
-            if (ta > tb) {
-                float tmp = ta;
-                ta = tb;
-                tb = tmp;
-            }
-            float tol = kTolC * (fabsf(ta) + fabsf(tb) + (fabsf(o3[a]) + R) * fabsf(i3[a]));
-            c0 = fmaxf(c0, ta - tol);
-            c1 = fminf(c1, tb + tol);
-        }
-        c0 = fmaxf(c0, (float)tmin * 0.5f);
-        if (!(c0 <= c1)) return false;
-    }
     const bool par0 = fabs(r.d.x) < 1e-9, par1 = fabs(r.d.y) < 1e-9, par2 = fabs(r.d.z) < 1e-9;
     const float4* recs = reinterpret_cast<const float4*>(M.recs);
+    const float tmin_lo = (float)tmin * (1.0f - kTolC);
+    float best_hi = (float)best * (1.0f + kTolC) + kTolC;  // upper bound of best in f32
+    // the mesh AABB check (src/ray.rs:586-610) is the root's box test below
+    float c0 = tmin_lo * 0.5f, c1 = best_hi;
     uint32_t node = use_kd ? M.root_kd : M.root_bf;
     int sp = 0;
     bool found = false;
-    float best_hi = (float)best * (1.0f + kTolC) + kTolC;  // upper bound of best in f32
-    const float tmin_lo = (float)tmin * (1.0f - kTolC);
+    bool tie = false;  // another triangle of this mesh hit at exactly `best`
     for (;;) {
         bool descend = false;
         if (c0 <= best_hi) {
-            KdNode nd = M.nodes[node];
+            const float4* np = reinterpret_cast<const float4*>(M.nodes + node);
+            float4 n0 = np[0];
+            float4 n1 = np[1];
             if (count) w.v[kWorkNodes]++;
-            uint32_t ax = nd.b & 3u;
-            if (ax != kAxisLeaf) {
-                double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
-                double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
-                float ia = ax == 0 ? ix : (ax == 1 ? iy : iz);
-                bool par = ax == 0 ? par0 : (ax == 1 ? par1 : par2);
-                bool left_near = oa <= nd.split;
-                uint32_t nearc = left_near ? node + 1 : nd.a;
-                uint32_t farc = left_near ? nd.a : node + 1;
-                bool far_ok = !par && ((oa == nd.split) || ((oa < nd.split) == (da > 0.0)));
-                descend = true;
-                if (!far_ok) {
-                    node = nearc;
-                } else if (nd.b & kFlagMisplaced) {
-                    stk.node[sp * stk.stride] = farc;
-                    stk.t0[sp * stk.stride] = c0;
-                    stk.t1[sp * stk.stride] = c1;
-                    ++sp;
-                    node = nearc;
-                } else {
-                    float ts = (float)(nd.split - oa) * ia;
-                    float tol = kTolC * fabsf(ts) + 1e-30f;
-                    bool dn = c0 <= ts + tol;
-                    bool dfar = ts - tol <= c1;
-                    if (dn && dfar) {
+            const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
+            uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
+            uint32_t ax = nb & 3u;
+            if (box_clip(q, bmin, bmax, c0, c1) && c0 <= best_hi) {
+                if (ax != kAxisLeaf) {
+                    double split = M.nodes[node].split;
+                    double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
+                    double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
+                    float ia = ax == 0 ? q.ix : (ax == 1 ? q.iy : q.iz);
+                    bool par = ax == 0 ? par0 : (ax == 1 ? par1 : par2);
+                    bool left_near = oa <= split;
+                    uint32_t nearc = left_near ? node + 1 : na;
+                    uint32_t farc = left_near ? na : node + 1;
+                    bool far_ok = !par && ((oa == split) || ((oa < split) == (da > 0.0)));
+                    descend = true;
+                    if (!far_ok) {
+                        node = nearc;
+                    } else if (nb & kFlagMisplaced) {
                         stk.node[sp * stk.stride] = farc;
-                        stk.t0[sp * stk.stride] = fmaxf(c0, ts - tol);
+                        stk.t0[sp * stk.stride] = c0;
                         stk.t1[sp * stk.stride] = c1;
                         ++sp;
                         node = nearc;
-                        c1 = fminf(c1, ts + tol);
-                    } else if (dn) {
-                        node = nearc;
-                        c1 = fminf(c1, ts + tol);
-                    } else if (dfar) {
-                        node = farc;
-                        c0 = fmaxf(c0, ts - tol);
                     } else {
-                        descend = false;
-                    }
-                }
-            } else {
-                // leaf: FP32 Moller-Trumbore with error bounds; survivors re-tested in f64
-                uint32_t first = nd.a, cnt = nd.b >> 2;
-                for (uint32_t k = 0; k < cnt; ++k) {
-                    float4 r0 = recs[(size_t)(first + k) * 3 + 0];
-                    float4 r1 = recs[(size_t)(first + k) * 3 + 1];
-                    float4 r2 = recs[(size_t)(first + k) * 3 + 2];
-                    if (count) w.v[kWorkTris]++;
-                    float e1x = r0.w, e1y = r1.x, e1z = r1.y;
-                    float e2x = r1.z, e2y = r1.w, e2z = r2.x;
-                    float E = r2.z;  // max(|e1|_1, |e2|_1)
-                    float hx = dy * e2z - dz * e2y, hy = dz * e2x - dx * e2z, hz = dx * e2y - dy * e2x;
-                    float a = e1x * hx + e1y * hy + e1z * hz;
-                    float sx = ox - r0.x, sy = oy - r0.y, sz = oz - r0.z;
-                    float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
-                    float fa = fabsf(a);
-                    bool sure_in = false;
-                    bool maybe = true;
-                    if (fa > 4e-8f + kTolC * E * E) {
-                        float f = 1.0f / a;
-                        float g = E * fabsf(f);
-                        float du = kTolC * (S + E) * g + 1e-7f;
-                        float u = f * (sx * hx + sy * hy + sz * hz);
-                        if (u < -du || u > 1.0f + du) {
-                            maybe = false;
+                        float ts = (float)(split - oa) * ia;
+                        float tol = kTolC * fabsf(ts) + 1e-30f;
+                        bool dn = c0 <= ts + tol;
+                        bool dfar = ts - tol <= c1;
+                        if (dn && dfar) {
+                            stk.node[sp * stk.stride] = farc;
+                            stk.t0[sp * stk.stride] = fmaxf(c0, ts - tol);
+                            stk.t1[sp * stk.stride] = c1;
+                            ++sp;
+                            node = nearc;
+                            c1 = fminf(c1, ts + tol);
+                        } else if (dn) {
+                            node = nearc;
+                            c1 = fminf(c1, ts + tol);
+                        } else if (dfar) {
+                            node = farc;
+                            c0 = fmaxf(c0, ts - tol);
                         } else {
-                            float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
-                            float v = f * (dx * qx + dy * qy + dz * qz);
-                            if (v < -du || u + v > 1.0f + 2.0f * du) {
+                            descend = false;
+                        }
+                    }
+                } else {
+                    // leaf: FP32 Moller-Trumbore with error bounds; survivors re-tested in f64
+                    uint32_t first = na, cnt = nb >> 2;
+                    for (uint32_t k = 0; k < cnt; ++k) {
+                        float4 r0 = recs[(size_t)(first + k) * 3 + 0];
+                        float4 r1 = recs[(size_t)(first + k) * 3 + 1];
+                        float4 r2 = recs[(size_t)(first + k) * 3 + 2];
+                        if (count) w.v[kWorkTris]++;
+                        float e1x = r0.w, e1y = r1.x, e1z = r1.y;
+                        float e2x = r1.z, e2y = r1.w, e2z = r2.x;
+                        float E = r2.z;  // max(|e1|_1, |e2|_1)
+                        float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
+                        float a = e1x * hx + e1y * hy + e1z * hz;
+                        float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
+                        float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
+                        float fa = fabsf(a);
+                        bool sure_in = false;
+                        bool maybe = true;
+                        if (fa > 4e-8f + kTolC * E * E) {
+                            float f = 1.0f / a;
+                            float g = E * fabsf(f);
+                            float du = kTolC * (S + E) * g + 1e-7f;
+                            float u = f * (sx * hx + sy * hy + sz * hz);
+                            if (u < -du || u > 1.0f + du) {
                                 maybe = false;
                             } else {
-                                float t = f * (e2x * qx + e2y * qy + e2z * qz);
-                                float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
-                                if (t + dt <= tmin_lo || t - dt >= best_hi) {
+                                float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+                                float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
+                                if (v < -du || u + v > 1.0f + 2.0f * du) {
                                     maybe = false;
-                                } else if (kAny) {
-                                    sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
-                                              t - dt > (float)tmin * (1.0f + kTolC) &&
-                                              t + dt < (float)best * (1.0f - kTolC);
+                                } else {
+                                    float t = f * (e2x * qx + e2y * qy + e2z * qz);
+                                    float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
+                                    if (t + dt <= tmin_lo || t - dt >= best_hi) {
+                                        maybe = false;
+                                    } else if (kAny) {
+                                        sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
+                                                  t - dt > (float)tmin * (1.0f + kTolC) &&
+                                                  t + dt < (float)best * (1.0f - kTolC);
+                                    }
                                 }
                             }
                         }
-                    }
-                    if (!maybe) continue;
-                    if (kAny && sure_in) return true;
-                    uint32_t ti = f32_to_bits(r2.y);
-                    double tt, uu, vv;
-                    d3 nn;
-                    if (count) w.v[kWorkRetests]++;
-                    if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) {
-                        if (kAny) return true;
-                        best = tt;
-                        best_tri = (int32_t)ti;
-                        best_hi = (float)best * (1.0f + kTolC) + kTolC;
-                        found = true;
+                        if (!maybe) continue;
+                        if (kAny && sure_in) return true;
+                        uint32_t ti = f32_to_bits(r2.y);
+                        double tt, uu, vv;
+                        d3 nn;
+                        if (count) w.v[kWorkRetests]++;
+                        if (kAny) {
+                            if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) return true;
+                        } else if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
+                            if (tt < best) {
+                                best = tt;
+                                best_tri = (int32_t)ti;
+                                best_hi = (float)best * (1.0f + kTolC) + kTolC;
+                                found = true;
+                                tie = false;
+                            } else if (tt == best && found && (int32_t)ti != best_tri) {
+                                tie = true;
+                            }
+                        }
                     }
                 }
             }
@@ -301,8 +394,9 @@ RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin,
             }
         }
         if (!popped) break;
-        if (!kAny) c1 = fminf(c1, best_hi);
+        c1 = fminf(c1, best_hi);
     }
+    if (!kAny && tie && use_kd) best_tri = mesh_resolve_tie(M, r, tmin, best, stk);
     return found;
 }
 

@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit of traversal work (layout in rtrace_amd/csrc/common/layout.hpp)
-B_NODE, B_TRI, B_RETEST = 16, 48, 72
+B_NODE, B_TRI, B_RETEST = 40, 48, 72  # node: 32 B box+links, +8 B f64 split (inner)
 B_SPHERE, B_PLANE, B_CUBE = 32, 48, 304
 B_RAY_CLOSEST = 64 + 16  # RayRec in, HitRec out
 B_RAY_SHADOW = 48 + 24 + 1  # hit point + normal, light sample point in, visibility byte out
@@ -88,7 +88,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config2_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB (scale for FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the trace path has no multi-GPU mode in this build (asked for {args.gpus}).",
               file=sys.stderr)

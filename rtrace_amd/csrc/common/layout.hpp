@@ -68,6 +68,8 @@ struct ObjDev {
     double ntf[9];       // cube: (inv(w2l 3x3))^T row-major, applied to normals
     float cf[3];         // sphere centre (f32) for the prefilter
     float rf;            // sphere radius (f32)
+    float bmin[3];       // cube: world box of its geometry, rounded outward (prefilter)
+    float bmax[3];
 };
 
 struct MatDev {

@@ -100,6 +100,29 @@ void normal_matrix(const double* w2l, double out[9]) {
         for (int c = 0; c < 3; ++c) out[r * 3 + c] = inv[c][r];
 }
 
+// World box of a cube's hit geometry: Cube::hit intersects the box [-h, h] in the
+// local frame local = A^-1 (x - c) (src/ray.rs:413-421), A^-1 = upper 3x3 of
+// world_to_local, so the geometry is c + A [-h, h]^3 with A = ntf^T. Padded and rounded
+// outward for the FP32 prefilter.
+void cube_world_box(ObjDev& d) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int corner = 0; corner < 8; ++corner) {
+        double l[3] = {(corner & 1) ? d.half[0] : -d.half[0], (corner & 2) ? d.half[1] : -d.half[1],
+                       (corner & 4) ? d.half[2] : -d.half[2]};
+        for (int i = 0; i < 3; ++i) {
+            double x = d.c[i];
+            for (int j = 0; j < 3; ++j) x += d.ntf[j * 3 + i] * l[j];  // A[i][j] = ntf[j][i]
+            mn[i] = std::min(mn[i], x);
+            mx[i] = std::max(mx[i], x);
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        double pad = 1e-6 * (std::fabs(mn[i]) + std::fabs(mx[i]) + (mx[i] - mn[i])) + 1e-12;
+        d.bmin[i] = std::isfinite(mn[i]) ? round_down(mn[i] - pad) : -3.0e38f;
+        d.bmax[i] = std::isfinite(mx[i]) ? round_up(mx[i] + pad) : 3.0e38f;
+    }
+}
+
 bool is_identity(const double* m) {
     for (int i = 0; i < 16; ++i)
         if (m[i] != ((i % 5 == 0) ? 1.0 : 0.0)) return false;
@@ -119,8 +142,8 @@ struct SceneDevice {
 };
 
 struct FrameBuffers {
-    DevBuf path_seed, lvl, prim_obj, prim_tri, prim_t, ol_dn, q0, q1, counts, hits, hitq, hit_count, shade, spt, vis,
-        work, pix, rgb, px_dn, edge, edge2;
+    DevBuf path_seed, lvl, prim_obj, prim_tri, prim_t, ol_dn, q0, q1, counts, hits, hitq, hit_count, tieq, tie_count,
+        shade, spt, vis, work, pix, rgb, px_dn, edge, edge2;
 };
 
 }  // namespace rt
@@ -206,6 +229,7 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         memcpy(d.l2w, o.local_to_world, sizeof(d.l2w));
         d.t_identity = is_identity(o.world_to_local);
         normal_matrix(o.world_to_local, d.ntf);
+        if (o.kind == RT_OBJ_CUBE) cube_world_box(d);
         objs.push_back(d);
     }
     std::vector<MatDev> mats;
@@ -377,6 +401,8 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     B.hits.reserve(cap * sizeof(HitRec));
     B.hitq.reserve(cap * sizeof(int32_t));
     B.hit_count.reserve((depth + 1) * sizeof(int32_t));
+    B.tieq.reserve(cap * sizeof(int32_t));
+    B.tie_count.reserve((depth + 1) * sizeof(int32_t));
     B.shade.reserve(cap * sizeof(ShadeRec));
     B.spt.reserve(cap * K * 3 * sizeof(double));
     B.vis.reserve(cap * K);
@@ -397,6 +423,8 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     F.hits = B.hits.as<HitRec>();
     F.hitq = B.hitq.as<int32_t>();
     F.hit_count = B.hit_count.as<int32_t>();
+    F.tieq = B.tieq.as<int32_t>();
+    F.tie_count = B.tie_count.as<int32_t>();
     F.shade = B.shade.as<ShadeRec>();
     F.spt = B.spt.as<double>();
     F.vis = B.vis.as<uint8_t>();

@@ -134,6 +134,16 @@ struct RayD {
     d3 o, d;  // d is unit (Ray::new normalizes, src/ray.rs:13-18)
 };
 
+// Cold f64 paths stay out of line on the device so the traversal loops keep their
+// register budget (they run for a small fraction of the tests).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RT_COLD __host__ __device__ __attribute__((noinline))
+#elif defined(__HIPCC__)
+#define RT_COLD __host__ __device__ inline
+#else
+#define RT_COLD inline
+#endif
+
 // ---------------------------------------------------------------- exact f64 tests
 // Sphere::hit (src/ray.rs:89-120); t_max inclusive.
 RT_HD bool sphere_exact(const ObjDev& s, const RayD& r, double tmin, double tmax, double& t) {

@@ -17,7 +17,7 @@ namespace rt {
 
 namespace {
 
-constexpr int kBlock = 128;  // traversal kernels: 2 wavefronts, 24 KiB of LDS stack
+constexpr int kBlock = 128;  // traversal kernels: 2 wavefronts, 16 KiB of LDS stack
 constexpr int kXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // Workgroup id remapped so that each XCD (and its L2) walks one contiguous slice of
@@ -51,7 +51,10 @@ __device__ __forceinline__ void flush_work(const FrameView& F, const Work& w, in
 __global__ void k_init_counts(FrameView F) {
     for (int i = threadIdx.x; i <= F.max_depth; i += blockDim.x) {
         F.counts[i] = i == 0 ? F.n_q0 : 0;
-        if (i < F.max_depth) F.hit_count[i] = 0;
+        if (i < F.max_depth) {
+            F.hit_count[i] = 0;
+            F.tie_count[i] = 0;
+        }
     }
     if (threadIdx.x < 2 * kWorkCount) F.work[threadIdx.x] = 0;
 }
@@ -61,11 +64,11 @@ __global__ void k_raygen(SceneView S, FrameView F) {
     for (int q = xcd_block() * blockDim.x + threadIdx.x; q < F.n_q0; q += stride) raygen_item(S, F, q);
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_closest(SceneView S, FrameView F, int level) {
     __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
     __shared__ float s_t0[kMaxKdDepth * kBlock];
-    __shared__ float s_t1[kMaxKdDepth * kBlock];
-    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, s_t1 + threadIdx.x, kBlock};
+    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
     Work w;
     const int n = F.counts[level];
     const int stride = gridDim.x * kBlock;
@@ -75,12 +78,24 @@ __global__ __launch_bounds__(kBlock) void k_closest(SceneView S, FrameView F, in
     const int iters = (n + stride - 1) / stride;
     for (int it = 0; it < iters; ++it) {
         int slot = base + it * stride;
-        bool hit = false;
-        if (slot < n) hit = closest_item(S, F, slot, level, stk, w);
+        bool hit = false, tie = false;
+        if (slot < n) hit = closest_item<kCount>(S, F, slot, level, stk, w, &tie);
         int pos = wave_append(&F.hit_count[level], hit);
         if (hit) F.hitq[pos] = slot;
+        int tpos = wave_append(&F.tie_count[level], tie);
+        if (tie) F.tieq[tpos] = slot;
     }
-    flush_work(F, w, 0);
+    if (kCount) flush_work(F, w, 0);
+}
+
+// Reference-order tie resolution for the (rare) rays k_closest marked.
+__global__ __launch_bounds__(kBlock) void k_resolve(SceneView S, FrameView F, int level) {
+    __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
+    __shared__ float s_t0[kMaxKdDepth * kBlock];
+    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
+    const int n = F.tie_count[level];
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        resolve_item(S, F, F.tieq[i], level, stk);
 }
 
 __global__ void k_shade_prepare(SceneView S, FrameView F, int level) {
@@ -98,17 +113,17 @@ __global__ void k_shade_prepare(SceneView S, FrameView F, int level) {
     }
 }
 
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int level) {
     __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
     __shared__ float s_t0[kMaxKdDepth * kBlock];
-    __shared__ float s_t1[kMaxKdDepth * kBlock];
-    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, s_t1 + threadIdx.x, kBlock};
+    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
     Work w;
     const long long n = (long long)F.hit_count[level] * S.shadow_per_hit;
     const long long stride = (long long)gridDim.x * kBlock;
     for (long long t = (long long)xcd_block() * kBlock + threadIdx.x; t < n; t += stride)
-        shadow_item(S, F, (int)t, stk, w);
-    flush_work(F, w, 1);
+        shadow_item<kCount>(S, F, (int)t, stk, w);
+    if (kCount) flush_work(F, w, 1);
 }
 
 __global__ void k_shade_finish(SceneView S, FrameView F, int level) {
@@ -184,8 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_intersect(SceneView S, const double*
                                                      double tmax, int use_kd, int32_t* obj, int32_t* tri, double* t) {
     __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
     __shared__ float s_t0[kMaxKdDepth * kBlock];
-    __shared__ float s_t1[kMaxKdDepth * kBlock];
-    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, s_t1 + threadIdx.x, kBlock};
+    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
     Work w;
     int stride = gridDim.x * kBlock;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
@@ -194,7 +208,8 @@ __global__ __launch_bounds__(kBlock) void k_intersect(SceneView S, const double*
         r.d = unit(mk(d[3 * i], d[3 * i + 1], d[3 * i + 2]));
         double tt;
         int32_t ob, tr;
-        bool h = world_closest(S, r, tmin, tmax, use_kd != 0, stk, w, false, tt, ob, tr);
+        bool h = world_closest<false>(S, r, tmin, tmax, use_kd != 0, stk, w, tt, ob, tr);
+        if (h && tr == kTiePending) tr = mesh_resolve_tie(S.meshes[S.objs[ob].mesh], r, tmin, tt, stk);
         obj[i] = h ? ob : -1;
         tri[i] = h ? tr : -1;
         t[i] = h ? tt : 0.0;
@@ -228,14 +243,23 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
         // the device-side counts bound the work; grids are sized for the largest case
         long long cap_rays = level == 0 ? F.n_q0 : F.n_paths;
         mark(1, true);
-        hipLaunchKernelGGL(k_closest, dim3(grid_for(cap_rays, kBlock, kCap * 2)), dim3(kBlock), 0, st, S, F, level);
+        if (F.count_work)
+            hipLaunchKernelGGL(k_closest<true>, dim3(grid_for(cap_rays, kBlock, kCap * 2)), dim3(kBlock), 0, st, S, F,
+                               level);
+        else
+            hipLaunchKernelGGL(k_closest<false>, dim3(grid_for(cap_rays, kBlock, kCap * 2)), dim3(kBlock), 0, st, S, F,
+                               level);
+        hipLaunchKernelGGL(k_resolve, dim3(64), dim3(kBlock), 0, st, S, F, level);
         mark(1, false);
         mark(2, true);
         hipLaunchKernelGGL(k_shade_prepare, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);
         mark(2, false);
         mark(3, true);
-        hipLaunchKernelGGL(k_shadow, dim3(grid_for((long long)F.n_paths * S.shadow_per_hit, kBlock, kCap * 2)),
-                           dim3(kBlock), 0, st, S, F, level);
+        dim3 gs(grid_for((long long)F.n_paths * S.shadow_per_hit, kBlock, kCap * 2));
+        if (F.count_work)
+            hipLaunchKernelGGL(k_shadow<true>, gs, dim3(kBlock), 0, st, S, F, level);
+        else
+            hipLaunchKernelGGL(k_shadow<false>, gs, dim3(kBlock), 0, st, S, F, level);
         mark(3, false);
         mark(4, true);
         hipLaunchKernelGGL(k_shade_finish, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);

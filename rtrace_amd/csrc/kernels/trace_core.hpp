@@ -83,6 +83,8 @@ struct FrameView {
     HitRec* hits;         // [queue capacity]
     int32_t* hitq;        // [queue capacity] compacted slots that hit
     int32_t* hit_count;   // [max_depth]
+    int32_t* tieq;        // [queue capacity] slots whose mesh tie is pending
+    int32_t* tie_count;   // [max_depth]
     ShadeRec* shade;      // [queue capacity]
     double* spt;          // [queue capacity * shadow_per_hit * 3] sample points
     uint8_t* vis;         // [queue capacity * shadow_per_hit]
@@ -107,7 +109,6 @@ struct Work {
 struct StackRef {
     uint32_t* node;
     float* t0;
-    float* t1;
     int stride;
 };
 
@@ -134,6 +135,8 @@ constexpr float kEps32 = 5.9604645e-08f;  // 2^-24
 constexpr float kTolC = 64.0f * kEps32;   // slack factor for FP32 error bounds
 
 // ------------------------------------------------------------ mesh traversal
+constexpr int32_t kTiePending = -2;  // HitRec.tri of a mesh hit whose tie is not yet resolved
+
 // Per-ray FP32 state for a mesh: origin relative to the mesh centre, direction, guarded
 // reciprocal and the per-axis slack that covers rounding of the origin to f32.
 struct MeshRay {
@@ -158,6 +161,12 @@ RT_HD MeshRay make_mesh_ray(const MeshDev& M, const RayD& r) {
     q.sy = sp * fabsf(q.iy);
     q.sz = sp * fabsf(q.iz);
     return q;
+}
+
+// The same FP32 ray in absolute world coordinates (for object boxes).
+RT_HD MeshRay make_world_ray(const RayD& r) {
+    MeshDev z{};
+    return make_mesh_ray(z, r);
 }
 
 // Conservative slab test of a node's tight box, clipped to [c0, c1]; false = empty.
@@ -247,167 +256,188 @@ RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, dou
 // box clips it again. Skipped subtrees hold no triangle the reference could accept
 // ahead of the current best, so the candidate set and its visit order are the
 // reference's.
-template <bool kAny>
+template <bool kAny, bool count>
 RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
-                      StackRef stk, Work& w, bool count) {
+                      StackRef stk, Work& w) {
     const MeshRay q = make_mesh_ray(M, r);
     const float R = M.radius;  // max |v - centre|_1 over the mesh
     const bool par0 = fabs(r.d.x) < 1e-9, par1 = fabs(r.d.y) < 1e-9, par2 = fabs(r.d.z) < 1e-9;
     const float4* recs = reinterpret_cast<const float4*>(M.recs);
     const float tmin_lo = (float)tmin * (1.0f - kTolC);
     float best_hi = (float)best * (1.0f + kTolC) + kTolC;  // upper bound of best in f32
-    // the mesh AABB check (src/ray.rs:586-610) is the root's box test below
+    // the mesh AABB check (src/ray.rs:586-610) is the root's box test
     float c0 = tmin_lo * 0.5f, c1 = best_hi;
     uint32_t node = use_kd ? M.root_kd : M.root_bf;
+    bool have_node = true;
     int sp = 0;
     bool found = false;
     bool tie = false;  // another triangle of this mesh hit at exactly `best`
+    // "while-while" traversal: each lane walks inner nodes until it holds its next leaf,
+    // then the wavefront tests leaves together, so lanes at inner nodes do not wait
+    // through other lanes' triangle loops. Leaves are still tested in this lane's
+    // visit order.
     for (;;) {
-        bool descend = false;
-        if (c0 <= best_hi) {
-            const float4* np = reinterpret_cast<const float4*>(M.nodes + node);
-            float4 n0 = np[0];
-            float4 n1 = np[1];
-            if (count) w.v[kWorkNodes]++;
-            const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
-            uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
-            uint32_t ax = nb & 3u;
-            if (box_clip(q, bmin, bmax, c0, c1) && c0 <= best_hi) {
-                if (ax != kAxisLeaf) {
-                    double split = M.nodes[node].split;
-                    double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
-                    double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
-                    float ia = ax == 0 ? q.ix : (ax == 1 ? q.iy : q.iz);
-                    bool par = ax == 0 ? par0 : (ax == 1 ? par1 : par2);
-                    bool left_near = oa <= split;
-                    uint32_t nearc = left_near ? node + 1 : na;
-                    uint32_t farc = left_near ? na : node + 1;
-                    bool far_ok = !par && ((oa == split) || ((oa < split) == (da > 0.0)));
-                    descend = true;
-                    if (!far_ok) {
-                        node = nearc;
-                    } else if (nb & kFlagMisplaced) {
-                        stk.node[sp * stk.stride] = farc;
-                        stk.t0[sp * stk.stride] = c0;
-                        stk.t1[sp * stk.stride] = c1;
-                        ++sp;
-                        node = nearc;
+        uint32_t leaf = 0xffffffffu;
+        float leaf_c0 = 0.0f;
+        while (have_node && leaf == 0xffffffffu) {
+            bool next = false;  // `node` already holds the next node to visit
+            if (c0 <= best_hi) {
+                const float4* np = reinterpret_cast<const float4*>(M.nodes + node);
+                float4 n0 = np[0];
+                float4 n1 = np[1];
+                if (count) w.v[kWorkNodes]++;
+                const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
+                uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
+                uint32_t ax = nb & 3u;
+                if (box_clip(q, bmin, bmax, c0, c1) && c0 <= best_hi) {
+                    if (ax == kAxisLeaf) {
+                        leaf = node;
+                        leaf_c0 = c0;
                     } else {
-                        float ts = (float)(split - oa) * ia;
-                        float tol = kTolC * fabsf(ts) + 1e-30f;
-                        bool dn = c0 <= ts + tol;
-                        bool dfar = ts - tol <= c1;
-                        if (dn && dfar) {
+                        double split = M.nodes[node].split;
+                        double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
+                        double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
+                        float ia = ax == 0 ? q.ix : (ax == 1 ? q.iy : q.iz);
+                        bool par = ax == 0 ? par0 : (ax == 1 ? par1 : par2);
+                        bool left_near = oa <= split;
+                        uint32_t nearc = left_near ? node + 1 : na;
+                        uint32_t farc = left_near ? na : node + 1;
+                        bool far_ok = !par && ((oa == split) || ((oa < split) == (da > 0.0)));
+                        next = true;
+                        if (!far_ok) {
+                            node = nearc;
+                        } else if (nb & kFlagMisplaced) {
                             stk.node[sp * stk.stride] = farc;
-                            stk.t0[sp * stk.stride] = fmaxf(c0, ts - tol);
-                            stk.t1[sp * stk.stride] = c1;
+                            stk.t0[sp * stk.stride] = c0;
                             ++sp;
                             node = nearc;
-                            c1 = fminf(c1, ts + tol);
-                        } else if (dn) {
-                            node = nearc;
-                            c1 = fminf(c1, ts + tol);
-                        } else if (dfar) {
-                            node = farc;
-                            c0 = fmaxf(c0, ts - tol);
                         } else {
-                            descend = false;
-                        }
-                    }
-                } else {
-                    // leaf: FP32 Moller-Trumbore with error bounds; survivors re-tested in f64
-                    uint32_t first = na, cnt = nb >> 2;
-                    for (uint32_t k = 0; k < cnt; ++k) {
-                        float4 r0 = recs[(size_t)(first + k) * 3 + 0];
-                        float4 r1 = recs[(size_t)(first + k) * 3 + 1];
-                        float4 r2 = recs[(size_t)(first + k) * 3 + 2];
-                        if (count) w.v[kWorkTris]++;
-                        float e1x = r0.w, e1y = r1.x, e1z = r1.y;
-                        float e2x = r1.z, e2y = r1.w, e2z = r2.x;
-                        float E = r2.z;  // max(|e1|_1, |e2|_1)
-                        float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
-                        float a = e1x * hx + e1y * hy + e1z * hz;
-                        float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
-                        float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
-                        float fa = fabsf(a);
-                        bool sure_in = false;
-                        bool maybe = true;
-                        if (fa > 4e-8f + kTolC * E * E) {
-                            float f = 1.0f / a;
-                            float g = E * fabsf(f);
-                            float du = kTolC * (S + E) * g + 1e-7f;
-                            float u = f * (sx * hx + sy * hy + sz * hz);
-                            if (u < -du || u > 1.0f + du) {
-                                maybe = false;
+                            float ts = (float)(split - oa) * ia;
+                            float tol = kTolC * fabsf(ts) + 1e-30f;
+                            bool dn = c0 <= ts + tol;
+                            bool dfar = ts - tol <= c1;
+                            if (dn && dfar) {
+                                stk.node[sp * stk.stride] = farc;
+                                stk.t0[sp * stk.stride] = fmaxf(c0, ts - tol);
+                                ++sp;
+                                node = nearc;
+                                c1 = fminf(c1, ts + tol);
+                            } else if (dn) {
+                                node = nearc;
+                                c1 = fminf(c1, ts + tol);
+                            } else if (dfar) {
+                                node = farc;
+                                c0 = fmaxf(c0, ts - tol);
                             } else {
-                                float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
-                                float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
-                                if (v < -du || u + v > 1.0f + 2.0f * du) {
-                                    maybe = false;
-                                } else {
-                                    float t = f * (e2x * qx + e2y * qy + e2z * qz);
-                                    float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
-                                    if (t + dt <= tmin_lo || t - dt >= best_hi) {
-                                        maybe = false;
-                                    } else if (kAny) {
-                                        sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
-                                                  t - dt > (float)tmin * (1.0f + kTolC) &&
-                                                  t + dt < (float)best * (1.0f - kTolC);
-                                    }
-                                }
-                            }
-                        }
-                        if (!maybe) continue;
-                        if (kAny && sure_in) return true;
-                        uint32_t ti = f32_to_bits(r2.y);
-                        double tt, uu, vv;
-                        d3 nn;
-                        if (count) w.v[kWorkRetests]++;
-                        if (kAny) {
-                            if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) return true;
-                        } else if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
-                            if (tt < best) {
-                                best = tt;
-                                best_tri = (int32_t)ti;
-                                best_hi = (float)best * (1.0f + kTolC) + kTolC;
-                                found = true;
-                                tie = false;
-                            } else if (tt == best && found && (int32_t)ti != best_tri) {
-                                tie = true;
+                                next = false;
                             }
                         }
                     }
                 }
             }
-        }
-        if (descend) continue;
-        // pop the next pending subtree that can still beat `best`
-        bool popped = false;
-        while (sp > 0) {
-            --sp;
-            node = stk.node[sp * stk.stride];
-            c0 = stk.t0[sp * stk.stride];
-            c1 = stk.t1[sp * stk.stride];
-            if (c0 <= best_hi) {
-                popped = true;
-                break;
+            if (!next) {
+                // pop the next pending subtree that can still beat `best`
+                have_node = false;
+                while (sp > 0) {
+                    --sp;
+                    node = stk.node[sp * stk.stride];
+                    c0 = stk.t0[sp * stk.stride];
+                    if (c0 <= best_hi) {
+                        have_node = true;
+                        break;
+                    }
+                }
+                c1 = best_hi;  // the popped node's own box clips the far end again
             }
         }
-        if (!popped) break;
-        c1 = fminf(c1, best_hi);
+        if (leaf == 0xffffffffu) break;  // this lane has no node left
+        if (leaf_c0 > best_hi) continue;  // `best` improved since the leaf was found
+        // leaf: FP32 Moller-Trumbore with error bounds; survivors re-tested in f64
+        const KdNode& ln = M.nodes[leaf];
+        uint32_t first = ln.a, cnt = ln.b >> 2;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            float4 r0 = recs[(size_t)(first + k) * 3 + 0];
+            float4 r1 = recs[(size_t)(first + k) * 3 + 1];
+            float4 r2 = recs[(size_t)(first + k) * 3 + 2];
+            if (count) w.v[kWorkTris]++;
+            float e1x = r0.w, e1y = r1.x, e1z = r1.y;
+            float e2x = r1.z, e2y = r1.w, e2z = r2.x;
+            float E = r2.z;  // max(|e1|_1, |e2|_1)
+            float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
+            float a = e1x * hx + e1y * hy + e1z * hz;
+            float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
+            float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
+            float fa = fabsf(a);
+            bool sure_in = false;
+            bool maybe = true;
+            if (fa > 4e-8f + kTolC * E * E) {
+                float f = 1.0f / a;
+                float g = E * fabsf(f);
+                float du = kTolC * (S + E) * g + 1e-7f;
+                float u = f * (sx * hx + sy * hy + sz * hz);
+                if (u < -du || u > 1.0f + du) {
+                    maybe = false;
+                } else {
+                    float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+                    float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
+                    if (v < -du || u + v > 1.0f + 2.0f * du) {
+                        maybe = false;
+                    } else {
+                        float t = f * (e2x * qx + e2y * qy + e2z * qz);
+                        float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
+                        if (t + dt <= tmin_lo || t - dt >= best_hi) {
+                            maybe = false;
+                        } else if (kAny) {
+                            sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
+                                      t - dt > (float)tmin * (1.0f + kTolC) && t + dt < (float)best * (1.0f - kTolC);
+                        }
+                    }
+                }
+            }
+            if (!maybe) continue;
+            if (kAny && sure_in) return true;
+            uint32_t ti = f32_to_bits(r2.y);
+            double tt, uu, vv;
+            d3 nn;
+            if (count) w.v[kWorkRetests]++;
+            if (kAny) {
+                if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) return true;
+            } else if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
+                if (tt < best) {
+                    best = tt;
+                    best_tri = (int32_t)ti;
+                    best_hi = (float)best * (1.0f + kTolC) + kTolC;
+                    found = true;
+                    tie = false;
+                } else if (tt == best && found && (int32_t)ti != best_tri) {
+                    tie = true;
+                }
+            }
+        }
     }
-    if (!kAny && tie && use_kd) best_tri = mesh_resolve_tie(M, r, tmin, best, stk);
+    // a tie between triangles of this mesh: the reference-order walk (mesh_resolve_tie)
+    // picks the winner in a separate pass; kTiePending marks the hit as pending
+    if (!kAny && tie && use_kd) best_tri = kTiePending;
     return found;
 }
 
 // ------------------------------------------------------------ World::hit
 // Closest hit over all objects in scene order (src/ray.rs:699-711): analytic objects
 // accept t == closest (and so replace an equal earlier hit), meshes need t < closest.
+// A cube's test runs in its local space and returns a local t (src/ray.rs:413-421), so its
+// prefilter is the line against the cube's world box, with no t range.
+RT_HD bool cube_may_hit(const MeshRay& wq, const ObjDev& ob) {
+    float lo = -3.0e38f, hi = 3.0e38f;
+    return box_clip(wq, ob.bmin, ob.bmax, lo, hi);
+}
+
+template <bool count>
 RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk,
-                         Work& w, bool count, double& t_out, int32_t& obj_out, int32_t& tri_out) {
+                         Work& w, double& t_out, int32_t& obj_out, int32_t& tri_out) {
     double closest = tmax;
     int32_t bo = -1, bt = -1;
     RayF f = make_rayf(r);
+    const MeshRay wq = make_world_ray(r);
     for (int i = 0; i < S.n_obj; ++i) {
         const ObjDev& ob = S.objs[i];
         double t;
@@ -426,9 +456,10 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
                 bt = -1;
             }
         } else if (ob.kind == kCube) {
-            if (count) w.v[kWorkCube]++;
             d3 p, n;
-            if (cube_exact(ob, r, tmin, closest, t, p, n)) {
+            bool may = cube_may_hit(wq, ob);
+            if (count && may) w.v[kWorkCube]++;
+            if (may && cube_exact(ob, r, tmin, closest, t, p, n)) {
                 closest = t;
                 bo = i;
                 bt = -1;
@@ -436,7 +467,7 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
         } else {
             int32_t tri = -1;
             double best = closest;
-            if (mesh_trace<false>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w, count)) {
+            if (mesh_trace<false, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) {
                 closest = best;
                 bo = i;
                 bt = tri;
@@ -450,9 +481,10 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
 }
 
 // Shadow query: World::hit(shadow_ray, 0.001, dist).is_some() (src/lighting.rs:120,185).
-RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w,
-                     bool count) {
+template <bool count>
+RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w) {
     RayF f = make_rayf(r);
+    const MeshRay wq = make_world_ray(r);
     for (int i = 0; i < S.n_obj; ++i) {
         const ObjDev& ob = S.objs[i];
         double t;
@@ -463,13 +495,14 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
             if (count) w.v[kWorkPlane]++;
             if (plane_exact(ob, r, tmin, tmax, t)) return true;
         } else if (ob.kind == kCube) {
-            if (count) w.v[kWorkCube]++;
             d3 p, n;
-            if (cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
+            bool may = cube_may_hit(wq, ob);
+            if (count && may) w.v[kWorkCube]++;
+            if (may && cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
         } else {
             int32_t tri = -1;
             double best = tmax;
-            if (mesh_trace<true>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w, count)) return true;
+            if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
         }
     }
     return false;
@@ -604,13 +637,16 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
 }
 
 // ------------------------------------------------------------ closest hit stage
-// Returns true when the ray hit (the caller appends the slot to the hit queue).
-RT_HD bool closest_item(const SceneView& S, const FrameView& F, int slot, int level, StackRef stk, Work& w) {
+// Returns true when the ray hit (the caller appends the slot to the hit queue);
+// *tie is set when the hit waits for the tie resolver.
+template <bool count>
+RT_HD bool closest_item(const SceneView& S, const FrameView& F, int slot, int level, StackRef stk, Work& w, bool* tie) {
     const RayRec& rr = F.rays[slot];
     HitRec h;
     h.t = 0.0;
     h.obj = -1;
     h.tri = -1;
+    *tie = false;
     if (rr.path < 0) {
         F.hits[slot] = h;
         return false;
@@ -620,11 +656,12 @@ RT_HD bool closest_item(const SceneView& S, const FrameView& F, int slot, int le
     r.d = ld3(rr.d);
     double t;
     int32_t obj, tri;
-    bool hit = world_closest(S, r, 0.001, INFINITY, F.use_kd != 0, stk, w, F.count_work != 0, t, obj, tri);
+    bool hit = world_closest<count>(S, r, 0.001, INFINITY, F.use_kd != 0, stk, w, t, obj, tri);
     if (hit) {
         h.t = t;
         h.obj = obj;
         h.tri = tri;
+        *tie = tri == kTiePending;
     }
     F.hits[slot] = h;
     if (level == 0) {
@@ -649,6 +686,17 @@ RT_HD bool closest_item(const SceneView& S, const FrameView& F, int slot, int le
         }
     }
     return hit;
+}
+
+// Tie pass: the reference-order winner for a hit marked kTiePending.
+RT_HD void resolve_item(const SceneView& S, const FrameView& F, int slot, int level, StackRef stk) {
+    const RayRec& rr = F.rays[slot];
+    HitRec& h = F.hits[slot];
+    RayD r;
+    r.o = ld3(rr.o);
+    r.d = ld3(rr.d);
+    h.tri = mesh_resolve_tie(S.meshes[S.objs[h.obj].mesh], r, 0.001, h.t, stk);
+    if (level == 0) F.prim_tri[rr.path] = h.tri;
 }
 
 // ------------------------------------------------------------ shade, part 1
@@ -785,6 +833,7 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
 }
 
 // ------------------------------------------------------------ shadow stage
+template <bool count>
 RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRef stk, Work& w) {
     int hq = task / S.shadow_per_hit;
     const ShadeRec& sr = F.shade[hq];
@@ -795,7 +844,7 @@ RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRe
     RayD r;
     r.o = add(p, scale(0.001, n));
     r.d = unit(ld);
-    bool blocked = world_any(S, r, 0.001, dist, F.use_kd != 0, stk, w, F.count_work != 0);
+    bool blocked = world_any<count>(S, r, 0.001, dist, F.use_kd != 0, stk, w);
     F.vis[task] = blocked ? 0 : 1;
 }
 

@@ -81,6 +81,63 @@ def cpu_baseline(path, cfg, budget_s=20.0):
                       f"{threads} threads)"}
 
 
+COPY_BYTES = 1 << 30
+
+
+def traffic_child(workload):
+    """Profiled child: one frame of the workload, then a plain copy of COPY_BYTES (the
+    calibration kernel for the memory counters), in one process."""
+    import torch
+
+    import rtrace_amd as R
+    path, cfg = make_workload(workload, os.path.join(tempfile.gettempdir(), "rtrace_amd_bench"))
+    scene = R.Scene.from_json_file(path)
+    r = R.Renderer(cfg["width"], cfg["height"])
+    configure(R, r, cfg, scene)
+    out = torch.empty(cfg["width"] * cfg["height"] * 3, dtype=torch.uint8, device="cuda")
+    r.render_device(scene, out)
+    src = torch.ones(COPY_BYTES, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+
+
+def measure_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc passes (FETCH_SIZE, then
+    WRITE_SIZE; KiB per dispatch), each scaled by bytes / counter of the copy kernel."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    res = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="rt_pmc_")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--workload", workload, "--traffic-child"]
+        try:
+            subprocess.run(cmd, check=True, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        except Exception as e:  # noqa: BLE001 - report and carry on without traffic
+            return None, f"{counter} pass failed: {e}"
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows += list(csv.DictReader(open(f)))
+        kern = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"] or "copy" in r["Kernel_Name"].lower()]
+        copy = [c for c in copy if c > 1000]  # the 256 MiB copy, not small staging copies
+        shutil.rmtree(d, ignore_errors=True)
+        if not kern or not copy:
+            return None, f"{counter}: no samples"
+        scale = COPY_BYTES / (max(copy) * 1024.0)
+        res[counter] = (sum(kern) / len(kern)) * 1024.0 * scale
+        res[counter + "_scale"] = scale
+    return res, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,18 +145,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="config2_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
-                    help="only run a plain device-to-device copy of MIB MiB (scale for FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
-    if args.copy_calibration:
-        import torch
-        n = args.copy_calibration << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": n, "launches": 3}))
+    if args.traffic_child:
+        traffic_child(args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the trace path has no multi-GPU mode in this build (asked for {args.gpus}).",
@@ -169,6 +219,15 @@ def main():
         "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
         "work": wk,
     }
+    if not args.no_traffic:
+        t, err = measure_traffic(args.workload, dom)
+        if t:
+            line["roofline"]["traffic"] = int(t["FETCH_SIZE"] + t["WRITE_SIZE"])
+            line["roofline"]["traffic_note"] = (
+                f"HBM bytes/launch = FETCH_SIZE*{t['FETCH_SIZE_scale']:.3f} + WRITE_SIZE*{t['WRITE_SIZE_scale']:.3f} "
+                f"(KiB counters scaled by a {COPY_BYTES >> 20} MiB copy)")
+        else:
+            line["roofline"]["traffic_note"] = err
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(path, cfg)
     print(json.dumps(line))

@@ -59,7 +59,7 @@ __global__ void k_init_counts(FrameView F) {
     if (threadIdx.x < 2 * kWorkCount) F.work[threadIdx.x] = 0;
 }
 
-__global__ void k_raygen(SceneView S, FrameView F) {
+__global__ __launch_bounds__(256) void k_raygen(SceneView S, FrameView F) {
     int stride = gridDim.x * blockDim.x;
     for (int q = xcd_block() * blockDim.x + threadIdx.x; q < F.n_q0; q += stride) raygen_item(S, F, q);
 }
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_resolve(SceneView S, FrameView F, in
         resolve_item(S, F, F.tieq[i], level, stk);
 }
 
-__global__ void k_shade_prepare(SceneView S, FrameView F, int level) {
+__global__ __launch_bounds__(256) void k_shade_prepare(SceneView S, FrameView F, int level) {
     const int n = F.hit_count[level];
     const int stride = gridDim.x * blockDim.x;
     const int base = xcd_block() * blockDim.x + (int)threadIdx.x;
@@ -126,13 +126,13 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
     if (kCount) flush_work(F, w, 1);
 }
 
-__global__ void k_shade_finish(SceneView S, FrameView F, int level) {
+__global__ __launch_bounds__(256) void k_shade_finish(SceneView S, FrameView F, int level) {
     const int n = F.hit_count[level];
     const int stride = gridDim.x * blockDim.x;
     for (int hq = xcd_block() * blockDim.x + threadIdx.x; hq < n; hq += stride) shade_finish_item(S, F, hq);
 }
 
-__global__ void k_pixel(FrameView F) {
+__global__ __launch_bounds__(256) void k_pixel(FrameView F) {
     int n = F.W * F.H;
     int stride = gridDim.x * blockDim.x;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) pixel_item(F, p);
@@ -181,14 +181,14 @@ __global__ __launch_bounds__(kOT * kOT) void k_outline_edges(FrameView F) {
     F.edge[(size_t)y * F.W + x] = edge_mask_value(F, e);
 }
 
-__global__ void k_outline_dilate(FrameView F) {
+__global__ __launch_bounds__(256) void k_outline_dilate(FrameView F) {
     int n = F.W * F.H;
     int stride = gridDim.x * blockDim.x;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride)
         F.edge2[p] = dilate_item(F, F.edge, p % F.W, p / F.W);
 }
 
-__global__ void k_finalize(FrameView F, const double* mask) {
+__global__ __launch_bounds__(256) void k_finalize(FrameView F, const double* mask) {
     int n = F.W * F.H;
     int stride = gridDim.x * blockDim.x;
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += stride) finalize_item(F, mask, p);

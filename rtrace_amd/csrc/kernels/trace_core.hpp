@@ -699,6 +699,51 @@ RT_HD void resolve_item(const SceneView& S, const FrameView& F, int slot, int le
     if (level == 0) F.prim_tri[rr.path] = h.tri;
 }
 
+// ------------------------------------------------------------ area-light samples
+// 16 x sample_disk_light_point (src/lighting.rs:69-97, SAMPLES = 16 at :158): rejection
+// sampling with two gen_range(-r..r) draws per trial from StdRng::seed_from_u64(seed).
+// Each draw is one u64 = two consecutive ChaCha12 words, so a trial takes exactly 4 words
+// and a 16-word block holds exactly 4 trials; the block is consumed with static indices.
+// A draw that lands on `high` (gen_range retries with a smaller scale, probability
+// ~2^-52) makes the fast path give up so the sequential reader redoes the light.
+RT_HD bool disk_samples_fast(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb, double* out) {
+    uint32_t key[8];
+    stdrng_key(seed, key);
+    const double low = -radius, high = radius, scl = high - low;
+    int n = 0;
+    for (uint64_t blk = 0; n < 16; ++blk) {
+        uint32_t b[16];
+        chacha12_block(key, blk, b);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            uint64_t wx = (uint64_t)b[4 * t] | ((uint64_t)b[4 * t + 1] << 32);
+            uint64_t wy = (uint64_t)b[4 * t + 2] | ((uint64_t)b[4 * t + 3] << 32);
+            double x = (bits_to_f64((wx >> 12) | (1023ull << 52)) - 1.0) * scl + low;
+            double y = (bits_to_f64((wy >> 12) | (1023ull << 52)) - 1.0) * scl + low;
+            if (!(x < high) || !(y < high)) return false;
+            if (n < 16 && x * x + y * y <= radius * radius) {
+                st3(out + 3 * n, add(add(lp, scale(x, ub)), scale(y, vb)));
+                ++n;
+            }
+        }
+    }
+    return true;
+}
+
+RT_COLD void disk_samples_slow(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb, double* out) {
+    StdRngDev rng;
+    rng.init(seed, 0);
+    for (int s = 0; s < 16; ++s) {
+        double x, y;
+        for (;;) {  // sample_disk_point (src/lighting.rs:57-66)
+            x = rng.gen_range(-radius, radius);
+            y = rng.gen_range(-radius, radius);
+            if (x * x + y * y <= radius * radius) break;
+        }
+        st3(out + 3 * s, add(add(lp, scale(x, ub)), scale(y, vb)));
+    }
+}
+
 // ------------------------------------------------------------ shade, part 1
 // Exact hit record, effective material, shadow sample points, reflection ray.
 // Returns true when a reflection ray was written to *refl.
@@ -798,24 +843,13 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
         ls = ls + to_u64_sat(p.x * 1000.0) * 0x85EBCA6Bull;
         ls = ls + to_u64_sat(p.y * 1000.0) * 0xC2B2AE35ull;
         ls = ls + to_u64_sat(p.z * 1000.0) * 0x6C8E9CF5ull;
-        StdRngDev rng;
-        rng.init(ls, 0);
         double radius = L.diameter / 2.0;
         d3 ldir = unit(sub(lp, p));
         d3 up = fabs(ldir.x) < 0.9 ? mk(1, 0, 0) : mk(0, 1, 0);
         d3 ub = unit(cross(up, ldir));
         d3 vb = unit(cross(ldir, ub));
-        for (int s = 0; s < 16; ++s) {
-            double x, y;
-            for (;;) {  // sample_disk_point (src/lighting.rs:57-66)
-                x = rng.gen_range(-radius, radius);
-                y = rng.gen_range(-radius, radius);
-                if (x * x + y * y <= radius * radius) break;
-            }
-            d3 sp = add(add(lp, scale(x, ub)), scale(y, vb));
-            st3(spt + 3 * j, sp);
-            ++j;
-        }
+        if (!disk_samples_fast(ls, radius, lp, ub, vb, spt + 3 * j)) disk_samples_slow(ls, radius, lp, ub, vb, spt + 3 * j);
+        j += 16;
     }
     // reflection (src/lighting.rs:469-494): mirror of the view direction from the camera
     if (sr.spawned) {

@@ -54,6 +54,18 @@ struct MeshDev {
 
 enum : int32_t { kSphere = 0, kPlane = 1, kCube = 2, kMesh = 3 };
 
+// Object-level BVH over the scene's spheres (scenes with many spheres): 32 B nodes,
+// f32 boxes rounded outward. Inner: a = left child, b = right child. Leaf: b has the top
+// bit set, a = first entry of the leaf's sphere list, b & 0x7fffffff = count.
+struct alignas(16) BvhNode {
+    float bmin[3];
+    float bmax[3];
+    uint32_t a;
+    uint32_t b;
+};
+static_assert(sizeof(BvhNode) == 32, "BvhNode is 32 bytes");
+constexpr uint32_t kBvhLeaf = 0x80000000u;
+
 struct ObjDev {
     int32_t kind;
     int32_t mat;

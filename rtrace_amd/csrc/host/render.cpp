@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -123,6 +124,49 @@ void cube_world_box(ObjDev& d) {
     }
 }
 
+// Object BVH over spheres: median split of the centroids on the widest axis, leaves of
+// at most 4 spheres, boxes = centre +- radius rounded outward.
+struct SphereBox {
+    float lo[3], hi[3];
+    double c[3];
+    int32_t obj;
+};
+uint32_t build_obvh(std::vector<SphereBox>& items, size_t begin, size_t end, std::vector<BvhNode>& nodes,
+                    std::vector<int32_t>& order) {
+    uint32_t me = (uint32_t)nodes.size();
+    nodes.push_back(BvhNode{});
+    BvhNode nd{};
+    for (int k = 0; k < 3; ++k) {
+        nd.bmin[k] = INFINITY;
+        nd.bmax[k] = -INFINITY;
+    }
+    double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = begin; i < end; ++i)
+        for (int k = 0; k < 3; ++k) {
+            nd.bmin[k] = std::min(nd.bmin[k], items[i].lo[k]);
+            nd.bmax[k] = std::max(nd.bmax[k], items[i].hi[k]);
+            cmin[k] = std::min(cmin[k], items[i].c[k]);
+            cmax[k] = std::max(cmax[k], items[i].c[k]);
+        }
+    if (end - begin <= 4) {
+        nd.a = (uint32_t)order.size();
+        nd.b = kBvhLeaf | (uint32_t)(end - begin);
+        for (size_t i = begin; i < end; ++i) order.push_back(items[i].obj);
+        nodes[me] = nd;
+        return me;
+    }
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+        if (cmax[k] - cmin[k] > cmax[axis] - cmin[axis]) axis = k;
+    size_t mid = (begin + end) / 2;
+    std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
+                     [axis](const SphereBox& x, const SphereBox& y) { return x.c[axis] < y.c[axis]; });
+    nd.a = build_obvh(items, begin, mid, nodes, order);
+    nd.b = build_obvh(items, mid, end, nodes, order);
+    nodes[me] = nd;
+    return me;
+}
+
 bool is_identity(const double* m) {
     for (int i = 0; i < 16; ++i)
         if (m[i] != ((i % 5 == 0) ? 1.0 : 0.0)) return false;
@@ -134,7 +178,9 @@ bool is_identity(const double* m) {
 struct SceneDevice {
     HostScene host;
     int device = 0;
-    DevBuf objs, meshes, mats, lights;
+    DevBuf objs, meshes, mats, lights, obvh, obvh_items, others;
+    bool has_obvh = false, obvh_closest = false;
+    int n_others = 0;
     std::vector<std::unique_ptr<DevBuf>> mesh_bufs;
     int n_obj = 0, n_lights = 0, shadow_per_hit = 0;
     double ms_upload = 0;
@@ -263,6 +309,37 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         D.shadow_per_hit += l.is_area ? 16 : 1;
         lights.push_back(d);
     }
+    // spheres through an object BVH once there are enough of them
+    std::vector<SphereBox> sph;
+    std::vector<int32_t> others;
+    bool has_cube = false;
+    for (size_t i = 0; i < objs.size(); ++i) {
+        if (objs[i].kind == kSphere) {
+            SphereBox b{};
+            for (int k = 0; k < 3; ++k) {
+                double pad = 1e-6 * (std::fabs(objs[i].c[k]) + std::fabs(objs[i].r)) + 1e-12;
+                b.lo[k] = round_down(objs[i].c[k] - std::fabs(objs[i].r) - pad);
+                b.hi[k] = round_up(objs[i].c[k] + std::fabs(objs[i].r) + pad);
+                b.c[k] = objs[i].c[k];
+            }
+            b.obj = (int32_t)i;
+            sph.push_back(b);
+        } else {
+            others.push_back((int32_t)i);
+            has_cube |= objs[i].kind == kCube;
+        }
+    }
+    D.has_obvh = sph.size() >= 16;
+    if (D.has_obvh) {
+        std::vector<BvhNode> nodes;
+        std::vector<int32_t> order;
+        build_obvh(sph, 0, sph.size(), nodes, order);
+        upload(D.obvh, nodes, st);
+        upload(D.obvh_items, order, st);
+        upload(D.others, others, st);
+        D.n_others = (int)others.size();
+        D.obvh_closest = !has_cube;
+    }
     upload(D.objs, objs, st);
     upload(D.meshes, meshes, st);
     upload(D.mats, mats, st);
@@ -343,6 +420,13 @@ SceneView make_view(const SceneDevice& D, const CameraDev& cam) {
     S.fog_start = H.fog_start;
     S.fog_end = H.fog_end;
     memcpy(S.bg, H.background, sizeof(S.bg));
+    if (D.has_obvh) {
+        S.obvh = D.obvh.as<BvhNode>();
+        S.obvh_items = D.obvh_items.as<int32_t>();
+        S.others = D.others.as<int32_t>();
+        S.n_others = D.n_others;
+        S.obvh_closest = D.obvh_closest;
+    }
     return S;
 }
 

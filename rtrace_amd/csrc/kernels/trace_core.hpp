@@ -28,6 +28,12 @@ struct SceneView {
     double fog_color[3];
     double fog_density, fog_start, fog_end;
     double bg[3];
+    // object BVH over spheres (null when the scene does not use one)
+    const BvhNode* obvh;
+    const int32_t* obvh_items;  // sphere object indices in leaf order
+    const int32_t* others;      // every non-sphere object index, in scene order
+    int32_t n_others;
+    int32_t obvh_closest;       // closest-hit may use the BVH (the scene has no cube)
 };
 
 struct RayRec {  // 64 B queue entry
@@ -431,9 +437,131 @@ RT_HD bool cube_may_hit(const MeshRay& wq, const ObjDev& ob) {
     return box_clip(wq, ob.bmin, ob.bmax, lo, hi);
 }
 
+// ------------------------------------------------------------ object BVH
+// Visit the spheres whose boxes the ray meets within [tmin, hi()], near child first, and
+// call fn(object index) for each; fn returns true to stop. Uses the LDS stack.
+template <class Hi, class Fn>
+RT_HD void obvh_visit(const SceneView& S, const MeshRay& wq, double tmin, Hi hi, StackRef stk, Fn fn) {
+    uint32_t node = 0;
+    int sp = 0;
+    float c0 = (float)tmin * 0.5f, c1 = hi();
+    for (;;) {
+        bool next = false;
+        if (c0 <= hi()) {
+            const BvhNode& nd = S.obvh[node];
+            if (box_clip(wq, nd.bmin, nd.bmax, c0, c1) && c0 <= hi()) {
+                if (nd.b & kBvhLeaf) {
+                    uint32_t first = nd.a, cnt = nd.b & ~kBvhLeaf;
+                    for (uint32_t k = 0; k < cnt; ++k)
+                        if (fn(S.obvh_items[first + k])) return;
+                } else {
+                    // near child first by the entry of its box
+                    const BvhNode& L = S.obvh[nd.a];
+                    const BvhNode& R = S.obvh[nd.b];
+                    float l0 = c0, l1 = hi(), r0 = c0, r1 = hi();
+                    bool hl = box_clip(wq, L.bmin, L.bmax, l0, l1);
+                    bool hr = box_clip(wq, R.bmin, R.bmax, r0, r1);
+                    if (hl && hr) {
+                        bool left_first = l0 <= r0;
+                        stk.node[sp * stk.stride] = left_first ? nd.b : nd.a;
+                        stk.t0[sp * stk.stride] = left_first ? r0 : l0;
+                        ++sp;
+                        node = left_first ? nd.a : nd.b;
+                        c0 = left_first ? l0 : r0;
+                        next = true;
+                    } else if (hl || hr) {
+                        node = hl ? nd.a : nd.b;
+                        c0 = hl ? l0 : r0;
+                        next = true;
+                    }
+                    c1 = hi();
+                }
+            }
+        }
+        if (next) continue;
+        bool popped = false;
+        while (sp > 0) {
+            --sp;
+            node = stk.node[sp * stk.stride];
+            c0 = stk.t0[sp * stk.stride];
+            if (c0 <= hi()) {
+                popped = true;
+                break;
+            }
+        }
+        if (!popped) return;
+        c1 = hi();
+    }
+}
+
+// World::hit in any object order (spheres through the BVH, the rest linearly): among the
+// objects reaching the smallest t, the reference's scene-order fold (analytic objects
+// accept t == closest and replace, meshes need t < closest) keeps the largest analytic
+// index above the first object reaching it, else that first object. Spheres, planes and
+// meshes have hit distances independent of the running `closest`, so the rule is exact
+// for them; scenes with cubes use the scene-order fold below instead.
+struct ClosestState {
+    double best;
+    double tmax;
+    int32_t first, first_tri, max_analytic;
+    RT_HD void offer(double t, int32_t i, int32_t tri, bool analytic) {
+        if (!(analytic ? t <= tmax : t < tmax)) return;
+        if (first < 0 || t < best) {
+            best = t;
+            first = i;
+            first_tri = tri;
+            max_analytic = analytic ? i : -1;
+        } else if (t == best) {
+            if (i < first) {
+                first = i;
+                first_tri = tri;
+            }
+            if (analytic && i > max_analytic) max_analytic = i;
+        }
+    }
+};
+
+template <bool count>
+RT_HD bool world_closest_bvh(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk,
+                             Work& w, double& t_out, int32_t& obj_out, int32_t& tri_out) {
+    ClosestState cs{tmax, tmax, -1, -1, -1};
+    RayF f = make_rayf(r);
+    const MeshRay wq = make_world_ray(r);
+    auto hi = [&]() { return (float)cs.best * (1.0f + kTolC) + kTolC; };
+    obvh_visit(S, wq, tmin, hi, stk, [&](int32_t i) {
+        const ObjDev& ob = S.objs[i];
+        double t;
+        if (count) w.v[kWorkSphere]++;
+        if (sphere_may_hit(ob, r, f, tmin, cs.best) && sphere_exact(ob, r, tmin, INFINITY, t))
+            cs.offer(t, i, -1, true);
+        return false;
+    });
+    for (int k = 0; k < S.n_others; ++k) {
+        int32_t i = S.others[k];
+        const ObjDev& ob = S.objs[i];
+        double t;
+        if (ob.kind == kPlane) {
+            if (count) w.v[kWorkPlane]++;
+            if (plane_exact(ob, r, tmin, INFINITY, t)) cs.offer(t, i, -1, true);
+        } else {
+            int32_t tri = -1;
+            double best = cs.best;
+            if (mesh_trace<false, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w))
+                cs.offer(best, i, tri, false);
+        }
+    }
+    if (cs.first < 0) return false;
+    bool analytic_wins = cs.max_analytic > cs.first;
+    obj_out = analytic_wins ? cs.max_analytic : cs.first;
+    tri_out = analytic_wins ? -1 : cs.first_tri;
+    t_out = cs.best;
+    return true;
+}
+
 template <bool count>
 RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk,
                          Work& w, double& t_out, int32_t& obj_out, int32_t& tri_out) {
+    if (S.obvh_closest) return world_closest_bvh<count>(S, r, tmin, tmax, use_kd, stk, w, t_out, obj_out, tri_out);
     double closest = tmax;
     int32_t bo = -1, bt = -1;
     RayF f = make_rayf(r);
@@ -485,6 +613,37 @@ template <bool count>
 RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w) {
     RayF f = make_rayf(r);
     const MeshRay wq = make_world_ray(r);
+    if (S.obvh) {
+        // any hit in [tmin, tmax] over the spheres, then the other objects
+        bool hit = false;
+        const float hi_c = (float)tmax * (1.0f + kTolC) + kTolC;
+        obvh_visit(S, wq, tmin, [&]() { return hi_c; }, stk, [&](int32_t i) {
+            const ObjDev& ob = S.objs[i];
+            double t;
+            if (count) w.v[kWorkSphere]++;
+            hit = sphere_may_hit(ob, r, f, tmin, tmax) && sphere_exact(ob, r, tmin, tmax, t);
+            return hit;
+        });
+        if (hit) return true;
+        for (int k = 0; k < S.n_others; ++k) {
+            const ObjDev& ob = S.objs[S.others[k]];
+            double t;
+            if (ob.kind == kPlane) {
+                if (count) w.v[kWorkPlane]++;
+                if (plane_exact(ob, r, tmin, tmax, t)) return true;
+            } else if (ob.kind == kCube) {
+                d3 p, n;
+                bool may = cube_may_hit(wq, ob);
+                if (count && may) w.v[kWorkCube]++;
+                if (may && cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
+            } else {
+                int32_t tri = -1;
+                double best = tmax;
+                if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
+            }
+        }
+        return false;
+    }
     for (int i = 0; i < S.n_obj; ++i) {
         const ObjDev& ob = S.objs[i];
         double t;

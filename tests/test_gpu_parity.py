@@ -224,3 +224,52 @@ def test_full_size_config2_invariants(R, config_dir):
     assert st.rays_shadow % 16 == 0
     assert 0 < st.rays_reflect <= st.rays_shadow // 16
     assert st.rays_total == st2.rays_total
+
+
+def _tie_scene(tmp_path, with_cube):
+    """24 spheres (object BVH on) where several are exact duplicates, a plane through
+    sphere centres and a mesh: exact cross-object ties resolved by World::hit's
+    scene-order rule (src/ray.rs:699-711)."""
+    objs = []
+    for i in range(24):
+        x, y = (i % 6) - 2.5, (i // 6) - 1.5
+        objs.append({"kind": "sphere", "center": [x, y, 0.5], "radius": 0.45,
+                     "material": {"color": "#%02X%02X80" % (i * 10, 255 - i * 10), "ambient": 0.1, "diffuse": 0.7,
+                                  "specular": 0.3, "shininess": 16, "reflectivity": 0.3 if i % 3 == 0 else None}})
+    # exact duplicates: a later analytic object at equal t replaces an earlier one
+    objs.append(dict(objs[3], material=dict(objs[3]["material"], color="#00FF00")))
+    objs.append(dict(objs[10], material=dict(objs[10]["material"], color="#0000FF")))
+    objs.append({"kind": "plane", "point": [0, 0, 0.5], "normal": [0, 0, 1],
+                 "material": {"color": "#CCCCCC", "ambient": 0.1, "diffuse": 0.6, "specular": 0.1, "shininess": 8}})
+    objs.append({"kind": "mesh", "filename": "icosahedron.stl",
+                 "material": {"color": "#FFAA00", "ambient": 0.1, "diffuse": 0.7, "specular": 0.3, "shininess": 32},
+                 "transform": ["scale(0.8, 0.8, 0.8)", "translate(0, 0, 2)"]})
+    if with_cube:
+        objs.append({"kind": "cube", "center": [0, 2.5, 0.5], "size": [1, 1, 1],
+                     "material": {"color": "#FF00FF", "ambient": 0.1, "diffuse": 0.7, "specular": 0.3,
+                                  "shininess": 32}})
+    sc = {"camera": {"kind": "perspective", "position": [0.3, -7, 6], "target": [0, 0, 0.5], "up": [0, 0, 1],
+                     "width": 4, "height": 3, "fov": 45},
+          "objects": objs,
+          "lights": [{"position": [3, -3, 6], "color": "#FFFFFF", "intensity": 0.8},
+                     {"position": [-3, 2, 5], "color": "#FFFFFF", "intensity": 0.5, "diameter": 0.6}],
+          "scene_settings": {"ambient_illumination": {"color": "#FFFFFF", "intensity": 0.1},
+                             "background_color": "#101010"}}
+    import shutil
+    shutil.copy(os.path.join(REF_SCENES, "icosahedron.stl"), str(tmp_path))
+    p = tmp_path / ("ties_cube.json" if with_cube else "ties.json")
+    p.write_text(json.dumps(sc))
+    return str(p)
+
+
+@pytest.mark.parametrize("with_cube", [False, True], ids=["bvh-closest", "scene-order-fold"])
+def test_object_bvh_exact_ties(R, tmp_path, with_cube):
+    path = _tie_scene(tmp_path, with_cube)
+    for aa, spp in (("quincunx", 1), ("stochastic", 2)):
+        opts = O.make_opts(200, 150, aa=aa, samples=spp, max_depth=4)
+        img, st, gh = gpu_render(R, path, opts)
+        ref, ost, rh = O.render(path, opts, want_hits=True)
+        d, mism = compare(f"ties {aa} cube={with_cube}", img, ref, gh, rh)
+        assert mism == 0 and int(d.max()) == 0
+        for k in ("rays_primary", "rays_reflect", "rays_shadow"):
+            assert getattr(st, k) == ost[k], k

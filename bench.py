@@ -35,7 +35,9 @@ def kernel_bytes(work, rays, per_ray):
 
 def make_workload(name, out_dir):
     from rtrace_amd import scenes
-    if name.startswith("config2"):
+    if name.startswith("config2_square"):
+        path = scenes.config2(out_dir, subdiv=6, light="square", light_size=1.0, light_samples=16)
+    elif name.startswith("config2"):
         path = scenes.config2(out_dir, subdiv=6)
     elif name.startswith("config3"):
         path = scenes.config3(out_dir, 1000)

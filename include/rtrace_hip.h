@@ -84,13 +84,23 @@ typedef struct {
     int32_t mesh;         /* mesh: index into rt_scene_desc.meshes */
 } rt_object_desc;
 
-/* Light (src/scene.rs:348-353); is_area when `diameter` is Some. */
+/* Light (src/scene.rs:348-353); is_area when `diameter` is Some.
+ * Extension fields (no reference counterpart; the reference ignores the JSON keys):
+ *   shape   RT_LIGHT_DISK (reference) or RT_LIGHT_SQUARE (JSON "shape": "square");
+ *   size    square side length (JSON "size"; falls back to "diameter");
+ *   samples shadow samples per hit (JSON "samples"; default 16, the reference's
+ *           hard-coded SAMPLES at src/lighting.rs:158). Disks keep the reference's
+ *           rejection sampling for any count; squares use a jittered grid. */
+enum { RT_LIGHT_DISK = 0, RT_LIGHT_SQUARE = 1 };
 typedef struct {
     double position[3];
     double color[3];
     double intensity;
     int32_t is_area;
     double diameter;
+    int32_t shape;
+    int32_t samples;
+    double size;
 } rt_light_desc;
 
 /* Triangle mesh, vertices in f64 after the object's transform (src/renderer.rs:270-296).
@@ -136,6 +146,7 @@ typedef struct {
     /* backend knobs */
     int32_t count_work;  /* 1 = also collect traversal counters into rt_render_stats */
     int32_t profile;     /* 1 = time each pipeline stage with hipEvents (rt_render_stats.ms_stage) */
+    int32_t light_samples; /* > 0: shadow samples per hit for every area light (extension) */
 } rt_render_opts;
 
 /* Pipeline stages timed when rt_render_opts.profile is set. */

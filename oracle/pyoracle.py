@@ -29,7 +29,7 @@ class OracleOpts(ctypes.Structure):
         ("outline_enabled", ctypes.c_int32), ("outline_use_8_neighbors", ctypes.c_int32),
         ("outline_depth_weight", ctypes.c_double), ("outline_normal_weight", ctypes.c_double),
         ("outline_threshold", ctypes.c_double), ("outline_color", ctypes.c_double * 3),
-        ("outline_thickness", ctypes.c_double),
+        ("outline_thickness", ctypes.c_double), ("light_samples", ctypes.c_int32),
     ]
 
 
@@ -59,6 +59,12 @@ def lib():
         L.oracle_stdrng_u64.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
         L.oracle_stdrng_f64.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
         L.oracle_sample_disk.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        L.oracle_area_samples.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
+        L.oracle_splitmix64_nth.restype = ctypes.c_uint64
+        L.oracle_splitmix64_nth.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_jitter_cols.restype = ctypes.c_int
+        L.oracle_jitter_cols.argtypes = [ctypes.c_int]
         L.oracle_parse_transforms.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
         L.oracle_hex_to_color.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
         L.oracle_kd_stats.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int32)] * 4 + [ctypes.c_char_p, ctypes.c_int]
@@ -109,8 +115,9 @@ def hex_to_color(s):
 
 
 def make_opts(width, height, max_depth=10, use_kdtree=True, threads=0, samples=1, aa="quincunx",
-              seed=0, outline=None):
+              seed=0, outline=None, light_samples=0):
     o = OracleOpts()
+    o.light_samples = light_samples
     o.width, o.height, o.max_depth = width, height, max_depth
     o.use_kdtree, o.threads, o.samples = int(use_kdtree), threads, samples
     o.aa_mode = AA_BY_NAME[aa] if isinstance(aa, str) else aa
@@ -191,6 +198,24 @@ def sample_disk(seed, radius, n):
     out = np.zeros(2 * n, dtype=np.float64)
     lib().oracle_sample_disk(seed, radius, n, out.ctypes.data)
     return out.reshape(n, 2)
+
+
+def area_samples(path_seed, p, light_pos, square, extent, n):
+    """n shadow-sample points (n x 3) of a disk (square=False, extent=diameter) or
+    square (extent=side) light for a hit at p."""
+    out = np.zeros(3 * n, dtype=np.float64)
+    pa = np.asarray(p, dtype=np.float64)
+    la = np.asarray(light_pos, dtype=np.float64)
+    lib().oracle_area_samples(path_seed, pa.ctypes.data, la.ctypes.data, int(square), extent, n, out.ctypes.data)
+    return out.reshape(n, 3)
+
+
+def splitmix64_nth(seed, k):
+    return lib().oracle_splitmix64_nth(seed, k)
+
+
+def jitter_cols(n):
+    return lib().oracle_jitter_cols(n)
 
 
 def parse_transforms(strings):

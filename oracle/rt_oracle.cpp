@@ -1099,6 +1099,11 @@ struct Light {
     double intensity;
     bool has_d = false;
     double diameter = 0;
+    // extension keys (the reference's serde model ignores them): "shape": "square",
+    // "size", "samples"
+    bool square = false;
+    double size = 0;
+    int samples = 16;
 };
 struct Fog {
     std::string color;
@@ -1121,6 +1126,7 @@ struct Ctx {
     Fog fog;
     V3 cam_pos;
     V3 background;
+    int light_samples = 0;  // > 0 overrides every area light's sample count (extension)
 };
 
 V3 reflect(V3 inc, V3 n) {  // :291-294
@@ -1172,29 +1178,61 @@ V3 point_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, 
     }
     return diffuse + spec;
 }
-V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V3 lc, V3 mc, u64 seed, Counters& C) {  // :145-217
-    const int SAMPLES = 16;
+// Square-light extension (no reference counterpart): n samples on a jittered grid of
+// cols x n/cols cells over the side x side square centred on the light, in the disk's
+// light-facing basis; cell (s % cols, s / cols) is jittered by SplitMix64 outputs 2s+1 and
+// 2s+2 of the stream seeded with the light seed, each turned into [0,1) as (x >> 11) * 2^-53.
+u64 splitmix64_nth(u64 seed, u64 k) {  // k-th output (k >= 1) of SplitMix64 from `seed`
+    u64 z = seed + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+int jitter_cols(int n) {  // largest divisor of n that is <= sqrt(n)
+    int c = 1;
+    for (int d = 1; (long long)d * d <= n; ++d)
+        if (n % d == 0) c = d;
+    return c;
+}
+u64 light_seed(u64 seed, V3 p) {  // src/lighting.rs:161-165
     u64 ls = seed * 0x9E3779B97F4A7C15ull;
-    ls = ls + sat_u64(h.p.x * 1000.0) * 0x85EBCA6Bull;
-    ls = ls + sat_u64(h.p.y * 1000.0) * 0xC2B2AE35ull;
-    ls = ls + sat_u64(h.p.z * 1000.0) * 0x6C8E9CF5ull;
-    StdRng rng(ls);
-    V3 total = v3(0, 0, 0);
-    int visible = 0;
-    for (int s = 0; s < SAMPLES; ++s) {
-        // sample_disk_light_point (:69-97)
-        double radius = L.diameter / 2.0;
-        V3 ldir = normalize(L.pos - h.p);
-        V3 up = std::fabs(ldir.x) < 0.9 ? v3(1, 0, 0) : v3(0, 1, 0);
-        V3 u = normalize(cross(up, ldir));
-        V3 v = normalize(cross(ldir, u));
-        double dx, dy;
+    ls = ls + sat_u64(p.x * 1000.0) * 0x85EBCA6Bull;
+    ls = ls + sat_u64(p.y * 1000.0) * 0xC2B2AE35ull;
+    ls = ls + sat_u64(p.z * 1000.0) * 0x6C8E9CF5ull;
+    return ls;
+}
+// Sample s of an area light seen from hp: sample_disk_light_point (:69-97) drawing from
+// rng, or the square extension's jittered cell s.
+V3 area_sample(const Light& L, V3 hp, StdRng& rng, u64 ls, int s, int cols, int rows) {
+    double radius = L.diameter / 2.0;
+    V3 ldir = normalize(L.pos - hp);
+    V3 up = std::fabs(ldir.x) < 0.9 ? v3(1, 0, 0) : v3(0, 1, 0);
+    V3 u = normalize(cross(up, ldir));
+    V3 v = normalize(cross(ldir, u));
+    double dx, dy;
+    if (L.square) {
+        double jx = (double)(splitmix64_nth(ls, 2 * (u64)s + 1) >> 11) * 0x1.0p-53;
+        double jy = (double)(splitmix64_nth(ls, 2 * (u64)s + 2) >> 11) * 0x1.0p-53;
+        dx = (((double)(s % cols) + jx) / (double)cols - 0.5) * L.size;
+        dy = (((double)(s / cols) + jy) / (double)rows - 0.5) * L.size;
+    } else {
         for (;;) {  // sample_disk_point (:57-66)
             dx = rng.gen_range(-radius, radius);
             dy = rng.gen_range(-radius, radius);
             if (dx * dx + dy * dy <= radius * radius) break;
         }
-        V3 sp = (L.pos + dx * u) + dy * v;
+    }
+    return (L.pos + dx * u) + dy * v;
+}
+V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V3 lc, V3 mc, u64 seed, Counters& C) {  // :145-217
+    const int SAMPLES = X.light_samples > 0 ? X.light_samples : L.samples;  // reference: 16 (:158)
+    u64 ls = light_seed(seed, h.p);
+    StdRng rng(ls);
+    const int cols = jitter_cols(SAMPLES), rows = SAMPLES / cols;
+    V3 total = v3(0, 0, 0);
+    int visible = 0;
+    for (int s = 0; s < SAMPLES; ++s) {
+        V3 sp = area_sample(L, h.p, rng, ls, s, cols, rows);
         V3 ld = normalize(sp - h.p);
         double dist = norm(sp - h.p);
         Ray sr = make_ray(h.p + 0.001 * h.n, ld);
@@ -1224,7 +1262,7 @@ V3 phong(const Ctx& X, const Hit& h, const Material& mat, u64 seed, Counters& C)
     V3 color = (eff->ambient * X.amb_intensity) * cmul(ac, mc);
     for (const Light& L : X.lights) {
         V3 lc = hex_or(L.color, v3(1, 1, 1));
-        V3 contrib = L.has_d ? area_light(X, h, *eff, L, lc, mc, seed, C) : point_light(X, h, *eff, L, lc, mc, C);
+        V3 contrib = (L.has_d || L.square) ? area_light(X, h, *eff, L, lc, mc, seed, C) : point_light(X, h, *eff, L, lc, mc, C);
         color = color + contrib;
     }
     return color;
@@ -1506,6 +1544,22 @@ void build_world(const J& root, const std::string& scene_path, bool use_kd, Load
         li.color = strv(req(l, "color"), "color");
         li.intensity = num(req(l, "intensity"), "intensity");
         li.has_d = optnum(l, "diameter", li.diameter);
+        const J* shp = l.get("shape");
+        if (shp && shp->t != J::Null) {
+            std::string k = strv(*shp, "shape");
+            if (k == "square") li.square = true;
+            else if (k != "disk") throw Err(-2, "unknown variant `" + k + "`, expected `disk` or `square`");
+        }
+        if (li.square && !optnum(l, "size", li.size)) {
+            if (!li.has_d) throw Err(-2, "missing field `size`");
+            li.size = li.diameter;
+        }
+        double ns = 0;
+        if (optnum(l, "samples", ns)) {
+            if (!(ns >= 1.0 && ns <= 65536.0) || ns != std::floor(ns))
+                throw Err(-2, "invalid value for `samples`, expected an integer in 1..=65536");
+            li.samples = (int)ns;
+        }
         L.ctx.lights.push_back(li);
     }
     const J& amb = req(ss, "ambient_illumination");
@@ -1639,6 +1693,7 @@ int do_render(const char* scene_path, const oracle_opts* op, uint8_t* rgb, int32
     Loaded L;
     build_world(root, scene_path, op->use_kdtree != 0, L);
     L.ctx.world = &L.world;
+    L.ctx.light_samples = op->light_samples;
     const int W = op->width, H = op->height;
     double aspect = (double)W / (double)H;
     Camera cam = make_camera(*L.camj, aspect);
@@ -1818,6 +1873,27 @@ void oracle_stdrng_f64(uint64_t seed, int n, double* out) {
     StdRng r(seed);
     for (int i = 0; i < n; ++i) out[i] = r.gen_f64();
 }
+void oracle_area_samples(uint64_t path_seed, const double p[3], const double light_pos[3], int square,
+                         double extent, int n, double* out) {
+    Light L;
+    L.pos = v3(light_pos[0], light_pos[1], light_pos[2]);
+    L.square = square != 0;
+    L.has_d = !L.square;
+    L.diameter = extent;
+    L.size = extent;
+    V3 hp = v3(p[0], p[1], p[2]);
+    u64 ls = light_seed(path_seed, hp);
+    StdRng rng(ls);
+    const int cols = jitter_cols(n), rows = n / cols;
+    for (int s = 0; s < n; ++s) {
+        V3 q = area_sample(L, hp, rng, ls, s, cols, rows);
+        out[3 * s] = q.x;
+        out[3 * s + 1] = q.y;
+        out[3 * s + 2] = q.z;
+    }
+}
+uint64_t oracle_splitmix64_nth(uint64_t seed, uint64_t k) { return splitmix64_nth(seed, k); }
+int oracle_jitter_cols(int n) { return jitter_cols(n); }
 void oracle_sample_disk(uint64_t seed, double radius, int n, double* out) {
     StdRng r(seed);
     for (int i = 0; i < n; ++i) {

@@ -37,6 +37,8 @@ typedef struct {
     double outline_threshold;
     double outline_color[3];
     double outline_thickness;
+    /* extension: > 0 overrides every area light's shadow samples per hit */
+    int32_t light_samples;
 } oracle_opts;
 
 typedef struct {
@@ -69,6 +71,14 @@ void oracle_stdrng_u64(uint64_t seed, int n, uint64_t* out);
 void oracle_stdrng_f64(uint64_t seed, int n, double* out);
 /* n calls of sample_disk_point(rng, radius); writes 2n doubles */
 void oracle_sample_disk(uint64_t seed, double radius, int n, double* out_xy);
+/* n shadow-sample points of an area light for a hit at p with path seed `path_seed`
+ * (calculate_diffuse_light_contribution seeding, src/lighting.rs:161-166): the reference's
+ * disk (square = 0, extent = diameter) or the square extension (square = 1, extent = side). */
+void oracle_area_samples(uint64_t path_seed, const double p[3], const double light_pos[3], int square,
+                         double extent, int n, double* out);
+/* k-th output (k >= 1) of SplitMix64 seeded with `seed`; jittered-grid columns for n. */
+uint64_t oracle_splitmix64_nth(uint64_t seed, uint64_t k);
+int oracle_jitter_cols(int n);
 int oracle_parse_transforms(const char* joined, double m_out[16]); /* '\n'-separated */
 int oracle_hex_to_color(const char* hex, double out[3]);
 int oracle_kd_stats(const char* stl_path, int32_t* n_tris, int32_t* n_leaves, int32_t* max_leaf,

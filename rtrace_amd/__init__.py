@@ -45,6 +45,7 @@ class _Opts(ctypes.Structure):
         ("outline_depth_weight", ctypes.c_double), ("outline_normal_weight", ctypes.c_double),
         ("outline_threshold", ctypes.c_double), ("outline_color", ctypes.c_double * 3),
         ("outline_thickness", ctypes.c_double), ("count_work", ctypes.c_int32), ("profile", ctypes.c_int32),
+        ("light_samples", ctypes.c_int32),
     ]
 
 
@@ -198,6 +199,7 @@ class Renderer:
         self.outline_config = None
         self.count_work = False
         self.profile = False
+        self.light_samples = 0  # extension: > 0 overrides every area light's shadow samples
         self.last_stats = None
 
     new = classmethod(lambda cls, w, h: cls(w, h))
@@ -230,6 +232,7 @@ class Renderer:
             o.outline_thickness = oc.line_thickness
         o.count_work = int(bool(self.count_work))
         o.profile = int(bool(self.profile))
+        o.light_samples = int(self.light_samples)
         return o
 
     def render(self, scene):

@@ -102,8 +102,12 @@ struct LightDev {
     double intensity;
     double diameter;
     int32_t is_area;
-    int32_t pad;
+    int32_t shape;    // kLightDisk / kLightSquare
+    int32_t samples;  // shadow samples per hit (area lights)
+    int32_t cols;     // square: jittered-grid columns for `samples` (largest divisor <= sqrt)
+    double size;      // square side
 };
+enum : int32_t { kLightDisk = 0, kLightSquare = 1 };
 
 struct CameraDev {
     double origin[3], horizontal[3], vertical[3], llc[3], view[3];

@@ -63,6 +63,14 @@ void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
     if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
 
+// Columns of the jittered grid for n square-light samples: the largest divisor of n that
+// is at most sqrt(n) (16 -> 4x4, 8 -> 2x4, 7 -> 1x7).
+int grid_cols(int n) {
+    int c = (int)std::sqrt((double)n);
+    while (c > 1 && n % c != 0) --c;
+    return c < 1 ? 1 : c;
+}
+
 float round_down(double x) {
     float f = (float)x;
     if ((double)f > x) f = std::nextafter(f, -INFINITY);
@@ -182,7 +190,7 @@ struct SceneDevice {
     bool has_obvh = false, obvh_closest = false;
     int n_others = 0;
     std::vector<std::unique_ptr<DevBuf>> mesh_bufs;
-    int n_obj = 0, n_lights = 0, shadow_per_hit = 0;
+    int n_obj = 0, n_lights = 0;
     double ms_upload = 0;
     std::vector<FlatKd> kd;  // host copies (stats)
 };
@@ -296,7 +304,6 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         mats.push_back(d);
     }
     std::vector<LightDev> lights;
-    D.shadow_per_hit = 0;
     for (const rt_light_desc& l : H.lights) {
         LightDev d{};
         memcpy(d.pos, l.position, sizeof(d.pos));
@@ -304,9 +311,14 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         d.intensity = l.intensity;
         d.is_area = l.is_area;
         d.diameter = l.diameter;
-        if (l.is_area && !(l.diameter / 2.0 > -l.diameter / 2.0))
+        d.shape = l.shape == RT_LIGHT_SQUARE ? kLightSquare : kLightDisk;
+        d.samples = l.samples > 0 ? l.samples : 16;
+        d.size = l.size;
+        d.cols = grid_cols(d.samples);
+        if (l.is_area && d.shape == kLightDisk && !(l.diameter / 2.0 > -l.diameter / 2.0))
             throw Error(RT_E_INVALID_ARG, "cannot sample empty range");  // gen_range(-r..r) with r <= 0
-        D.shadow_per_hit += l.is_area ? 16 : 1;
+        if (d.shape == kLightSquare && !(l.size > 0.0 && std::isfinite(l.size)))
+            throw Error(RT_E_INVALID_ARG, "square light size must be positive");
         lights.push_back(d);
     }
     // spheres through an object BVH once there are enough of them
@@ -400,7 +412,7 @@ CameraDev make_camera(const rt_camera_desc& c, double aspect) {
     return cam;
 }
 
-SceneView make_view(const SceneDevice& D, const CameraDev& cam) {
+SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_samples) {
     const HostScene& H = D.host;
     SceneView S{};
     S.objs = D.objs.as<ObjDev>();
@@ -409,7 +421,21 @@ SceneView make_view(const SceneDevice& D, const CameraDev& cam) {
     S.lights = D.lights.as<LightDev>();
     S.n_obj = D.n_obj;
     S.n_lights = D.n_lights;
-    S.shadow_per_hit = D.shadow_per_hit > 0 ? D.shadow_per_hit : 1;
+    // shadow-ray slots per hit: 1 per point light, `samples` per area light; sample points
+    // are stored for point and disk lights (square samples are regenerated from the seed)
+    int slots = 0, spt = 0;
+    bool square = false;
+    for (const rt_light_desc& l : H.lights) {
+        int n = l.is_area ? (light_samples > 0 ? light_samples : (l.samples > 0 ? l.samples : 16)) : 1;
+        slots += n;
+        if (l.shape == RT_LIGHT_SQUARE) square = true;
+        else spt += n;
+    }
+    S.shadow_per_hit = slots > 0 ? slots : 1;
+    S.spt_per_hit = spt;
+    S.has_square = square;
+    S.light_samples = light_samples;
+    S.light_cols = grid_cols(light_samples > 0 ? light_samples : 1);
     S.has_fog = H.has_fog;
     S.cam = cam;
     memcpy(S.cam_pos, H.camera.position, sizeof(S.cam_pos));
@@ -440,7 +466,8 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     const bool quin = o->aa_mode == RT_AA_QUINCUNX;
     if (o->aa_mode < 0 || o->aa_mode > 2) throw Error(RT_E_INVALID_ARG, "unknown anti-aliasing mode");
     CameraDev cam = make_camera(D.host.camera, (double)W / (double)H);
-    SceneView S = make_view(D, cam);
+    if (o->light_samples < 0 || o->light_samples > 65536) throw Error(RT_E_INVALID_ARG, "light_samples out of range");
+    SceneView S = make_view(D, cam, o->light_samples);
     FrameBuffers& B = ctx->fb;
     hipStream_t st = ctx->stream;
 
@@ -488,7 +515,8 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     B.tieq.reserve(cap * sizeof(int32_t));
     B.tie_count.reserve((depth + 1) * sizeof(int32_t));
     B.shade.reserve(cap * sizeof(ShadeRec));
-    B.spt.reserve(cap * K * 3 * sizeof(double));
+    if (cap * K >= (1ull << 31)) throw Error(RT_E_INVALID_ARG, "too many shadow samples per frame for one launch");
+    B.spt.reserve(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
     B.vis.reserve(cap * K);
     B.work.reserve(2 * kWorkCount * sizeof(unsigned long long));
     B.pix.reserve((size_t)W * H * 3 * sizeof(double));
@@ -551,7 +579,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         memset(stats, 0, sizeof(*stats));
         stats->rays_primary = depth > 0 ? (uint64_t)n_paths : 0;
         for (int l = 1; l < depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
-        for (int l = 0; l < depth; ++l) stats->rays_shadow += (uint64_t)hit_count[l] * (uint64_t)D.shadow_per_hit;
+        for (int l = 0; l < depth; ++l) stats->rays_shadow += (uint64_t)hit_count[l] * (uint64_t)(S.n_lights > 0 ? S.shadow_per_hit : 0);
         for (int k = 0; k < 2; ++k) {
             const unsigned long long* wk = work + k * kWorkCount;
             stats->work[k].nodes = wk[kWorkNodes];
@@ -744,7 +772,7 @@ int rt_intersect(rt_ctx* ctx, rt_scene* scene, const double* origins, const doub
     if (n > (size_t)INT32_MAX) return fail(ctx, RT_E_INVALID_ARG, "too many rays");
     return guarded(ctx, [&] {
         SceneDevice& D = *scene->dev;
-        SceneView S = make_view(D, CameraDev{});  // World::hit needs no camera
+        SceneView S = make_view(D, CameraDev{}, 0);  // World::hit needs no camera
         DevBuf o, d, ob, tr, tt;
         o.reserve(n * 3 * sizeof(double));
         d.reserve(n * 3 * sizeof(double));

@@ -516,6 +516,27 @@ HostScene load_scene_json(const std::string& text, const std::string& base_dir) 
             d.is_area = 1;
             d.diameter = as_f64(*dm, "diameter");
         }
+        // extension keys (ignored by the reference's serde model)
+        d.shape = RT_LIGHT_DISK;
+        d.samples = 16;
+        if (const Value* sh = opt_field(l, "shape")) {
+            std::string k = as_str(*sh, "shape");
+            if (k == "square") d.shape = RT_LIGHT_SQUARE;
+            else if (k != "disk") throw Error(RT_E_SCENE, "unknown variant `" + k + "`, expected `disk` or `square`");
+        }
+        if (d.shape == RT_LIGHT_SQUARE) {
+            const Value* sz = opt_field(l, "size");
+            if (!sz) sz = opt_field(l, "diameter");
+            if (!sz) throw Error(RT_E_SCENE, "missing field `size`");
+            d.size = as_f64(*sz, "size");
+            d.is_area = 1;
+        }
+        if (const Value* ns = opt_field(l, "samples")) {
+            double v = as_f64(*ns, "samples");
+            if (!(v >= 1.0 && v <= 65536.0) || v != std::floor(v))
+                throw Error(RT_E_SCENE, "invalid value for `samples`, expected an integer in 1..=65536");
+            d.samples = (int32_t)v;
+        }
         S.lights.push_back(d);
     }
 
@@ -595,6 +616,12 @@ HostScene from_desc(const rt_scene_desc& d) {
     S.objects.assign(d.objects, d.objects + d.n_objects);
     S.materials.assign(d.materials, d.materials + d.n_materials);
     S.lights.assign(d.lights, d.lights + d.n_lights);
+    for (auto& l : S.lights) {
+        if (l.samples == 0) l.samples = 16;  // zero-initialised descriptors get the reference count
+        if (l.samples < 0 || l.samples > 65536) throw Error(RT_E_INVALID_ARG, "light samples out of range");
+        if (l.shape != RT_LIGHT_DISK && l.shape != RT_LIGHT_SQUARE) throw Error(RT_E_INVALID_ARG, "unknown light shape");
+        if (l.shape == RT_LIGHT_SQUARE) l.is_area = 1;
+    }
     for (int i = 0; i < d.n_meshes; ++i) {
         if (d.meshes[i].n_tris < 0 || (d.meshes[i].n_tris > 0 && !d.meshes[i].vertices))
             throw Error(RT_E_INVALID_ARG, "mesh without vertices");

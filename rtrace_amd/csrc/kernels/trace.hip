@@ -256,10 +256,13 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
         mark(2, false);
         mark(3, true);
         dim3 gs(grid_for((long long)F.n_paths * S.shadow_per_hit, kBlock, kCap * 2));
-        if (F.count_work)
+        if (S.n_lights == 0) {
+            // no shadow rays
+        } else if (F.count_work) {
             hipLaunchKernelGGL(k_shadow<true>, gs, dim3(kBlock), 0, st, S, F, level);
-        else
+        } else {
             hipLaunchKernelGGL(k_shadow<false>, gs, dim3(kBlock), 0, st, S, F, level);
+        }
         mark(3, false);
         mark(4, true);
         hipLaunchKernelGGL(k_shade_finish, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);

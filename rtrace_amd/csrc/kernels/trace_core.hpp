@@ -19,7 +19,11 @@ struct SceneView {
     const LightDev* lights;
     int32_t n_obj;
     int32_t n_lights;
-    int32_t shadow_per_hit;  // sum over lights of (area ? 16 : 1)
+    int32_t shadow_per_hit;  // shadow-ray slots per hit: 1 per point light, samples per area light
+    int32_t spt_per_hit;     // stored sample points per hit (point and disk lights)
+    int32_t has_square;      // some light is a square (its samples are regenerated, not stored)
+    int32_t light_samples;   // > 0 overrides every area light's sample count
+    int32_t light_cols;      // jittered-grid columns for light_samples
     int32_t has_fog;
     CameraDev cam;
     double cam_pos[3];
@@ -859,18 +863,19 @@ RT_HD void resolve_item(const SceneView& S, const FrameView& F, int slot, int le
 }
 
 // ------------------------------------------------------------ area-light samples
-// 16 x sample_disk_light_point (src/lighting.rs:69-97, SAMPLES = 16 at :158): rejection
-// sampling with two gen_range(-r..r) draws per trial from StdRng::seed_from_u64(seed).
-// Each draw is one u64 = two consecutive ChaCha12 words, so a trial takes exactly 4 words
-// and a 16-word block holds exactly 4 trials; the block is consumed with static indices.
-// A draw that lands on `high` (gen_range retries with a smaller scale, probability
-// ~2^-52) makes the fast path give up so the sequential reader redoes the light.
-RT_HD bool disk_samples_fast(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb, double* out) {
+// n x sample_disk_light_point (src/lighting.rs:69-97; the reference's n is SAMPLES = 16 at
+// :158): rejection sampling with two gen_range(-r..r) draws per trial from
+// StdRng::seed_from_u64(seed). Each draw is one u64 = two consecutive ChaCha12 words, so a
+// trial takes exactly 4 words and a 16-word block holds exactly 4 trials; the block is
+// consumed with static indices. A draw that lands on `high` (gen_range retries with a
+// smaller scale, probability ~2^-52) makes the fast path give up so the sequential reader
+// redoes the light.
+RT_HD bool disk_samples_fast(uint64_t seed, int count, double radius, d3 lp, d3 ub, d3 vb, double* out) {
     uint32_t key[8];
     stdrng_key(seed, key);
     const double low = -radius, high = radius, scl = high - low;
     int n = 0;
-    for (uint64_t blk = 0; n < 16; ++blk) {
+    for (uint64_t blk = 0; n < count; ++blk) {
         uint32_t b[16];
         chacha12_block(key, blk, b);
 #pragma unroll
@@ -880,7 +885,7 @@ RT_HD bool disk_samples_fast(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb, 
             double x = (bits_to_f64((wx >> 12) | (1023ull << 52)) - 1.0) * scl + low;
             double y = (bits_to_f64((wy >> 12) | (1023ull << 52)) - 1.0) * scl + low;
             if (!(x < high) || !(y < high)) return false;
-            if (n < 16 && x * x + y * y <= radius * radius) {
+            if (n < count && x * x + y * y <= radius * radius) {
                 st3(out + 3 * n, add(add(lp, scale(x, ub)), scale(y, vb)));
                 ++n;
             }
@@ -889,10 +894,10 @@ RT_HD bool disk_samples_fast(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb, 
     return true;
 }
 
-RT_COLD void disk_samples_slow(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb, double* out) {
+RT_COLD void disk_samples_slow(uint64_t seed, int count, double radius, d3 lp, d3 ub, d3 vb, double* out) {
     StdRngDev rng;
     rng.init(seed, 0);
-    for (int s = 0; s < 16; ++s) {
+    for (int s = 0; s < count; ++s) {
         double x, y;
         for (;;) {  // sample_disk_point (src/lighting.rs:57-66)
             x = rng.gen_range(-radius, radius);
@@ -901,6 +906,68 @@ RT_COLD void disk_samples_slow(uint64_t seed, double radius, d3 lp, d3 ub, d3 vb
         }
         st3(out + 3 * s, add(add(lp, scale(x, ub)), scale(y, vb)));
     }
+}
+
+// The per-hit light seed and the light-facing basis of calculate_diffuse_light_contribution
+// (src/lighting.rs:161-166) and sample_disk_light_point (:74-82); squares reuse both.
+struct AreaFrame {
+    uint64_t seed;
+    d3 ub, vb;
+};
+RT_HD AreaFrame area_frame(uint64_t path_seed, d3 p, d3 lp) {
+    AreaFrame a;
+    uint64_t ls = path_seed * 0x9E3779B97F4A7C15ull;
+    ls = ls + to_u64_sat(p.x * 1000.0) * 0x85EBCA6Bull;
+    ls = ls + to_u64_sat(p.y * 1000.0) * 0xC2B2AE35ull;
+    ls = ls + to_u64_sat(p.z * 1000.0) * 0x6C8E9CF5ull;
+    a.seed = ls;
+    d3 ldir = unit(sub(lp, p));
+    d3 up = fabs(ldir.x) < 0.9 ? mk(1, 0, 0) : mk(0, 1, 0);
+    a.ub = unit(cross(up, ldir));
+    a.vb = unit(cross(ldir, a.ub));
+    return a;
+}
+
+// Square light (extension, no reference counterpart): sample s of n on a cols x (n/cols)
+// jittered grid over the side x side square centred on the light, in the same
+// light-facing basis as the disk. The jitter of sample s is outputs 2s+1 and 2s+2 of a
+// SplitMix64 stream seeded with the light seed, so any sample is computed on its own.
+RT_HD uint64_t splitmix_at(uint64_t seed, uint64_t k) {
+    uint64_t z = seed + k * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+RT_HD d3 square_sample(const LightDev& L, const AreaFrame& a, int s, int n, int cols) {
+    const int rows = n / cols;
+    const int ix = s % cols, iy = s / cols;
+    const double jx = (double)(splitmix_at(a.seed, 2ull * (uint64_t)s + 1) >> 11) * 0x1.0p-53;
+    const double jy = (double)(splitmix_at(a.seed, 2ull * (uint64_t)s + 2) >> 11) * 0x1.0p-53;
+    const double x = (((double)ix + jx) / (double)cols - 0.5) * L.size;
+    const double y = (((double)iy + jy) / (double)rows - 0.5) * L.size;
+    return add(add(ld3(L.pos), scale(x, a.ub)), scale(y, a.vb));
+}
+
+RT_HD int light_count(const SceneView& S, const LightDev& L) {
+    return L.is_area ? (S.light_samples > 0 ? S.light_samples : L.samples) : 1;
+}
+RT_HD int light_cols(const SceneView& S, const LightDev& L) {
+    return S.light_samples > 0 ? S.light_cols : L.cols;
+}
+
+// Phong term of one visible light sample (src/lighting.rs:123-138 and :184-208).
+RT_HD d3 phong_sample(const MatDev& m, const LightDev& L, d3 lc, d3 mc, d3 p, d3 n, d3 cam, d3 sp) {
+    d3 ld = unit(sub(sp, p));
+    double ds = fmax_r(dot(n, ld), 0.0);
+    d3 diffuse = scale((m.diffuse * ds) * L.intensity, hadamard(lc, mc));
+    d3 spec = mk(0, 0, 0);
+    if (ds > 0.0) {
+        d3 vd = unit(sub(cam, p));
+        d3 rd = reflect_dir(neg(ld), n);
+        double ss = pow(fmax_r(dot(vd, rd), 0.0), m.shininess);
+        spec = scale((m.specular * ss) * L.intensity, lc);
+    }
+    return add(diffuse, spec);
 }
 
 // ------------------------------------------------------------ shade, part 1
@@ -985,9 +1052,9 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
         o[2] = n.y;
         o[3] = n.z;
     }
-    // shadow sample points, lights in scene order (src/lighting.rs:251-285)
+    // shadow sample points of point and disk lights, in scene order (src/lighting.rs:251-285)
     uint64_t seed = F.path_seed[rr.path];
-    double* spt = F.spt + (size_t)hq * S.shadow_per_hit * 3;
+    double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
     int j = 0;
     for (int li = 0; li < S.n_lights; ++li) {
         const LightDev& L = S.lights[li];
@@ -997,18 +1064,13 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
             ++j;
             continue;
         }
-        // calculate_diffuse_light_contribution seeding (src/lighting.rs:161-166)
-        uint64_t ls = seed * 0x9E3779B97F4A7C15ull;
-        ls = ls + to_u64_sat(p.x * 1000.0) * 0x85EBCA6Bull;
-        ls = ls + to_u64_sat(p.y * 1000.0) * 0xC2B2AE35ull;
-        ls = ls + to_u64_sat(p.z * 1000.0) * 0x6C8E9CF5ull;
+        if (L.shape == kLightSquare) continue;  // regenerated where needed
+        const int cnt = light_count(S, L);
+        AreaFrame a = area_frame(seed, p, lp);
         double radius = L.diameter / 2.0;
-        d3 ldir = unit(sub(lp, p));
-        d3 up = fabs(ldir.x) < 0.9 ? mk(1, 0, 0) : mk(0, 1, 0);
-        d3 ub = unit(cross(up, ldir));
-        d3 vb = unit(cross(ldir, ub));
-        if (!disk_samples_fast(ls, radius, lp, ub, vb, spt + 3 * j)) disk_samples_slow(ls, radius, lp, ub, vb, spt + 3 * j);
-        j += 16;
+        if (!disk_samples_fast(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j))
+            disk_samples_slow(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j);
+        j += cnt;
     }
     // reflection (src/lighting.rs:469-494): mirror of the view direction from the camera
     if (sr.spawned) {
@@ -1031,7 +1093,25 @@ RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRe
     int hq = task / S.shadow_per_hit;
     const ShadeRec& sr = F.shade[hq];
     d3 p = ld3(sr.p), n = ld3(sr.n);
-    d3 sp = ld3(F.spt + (size_t)task * 3);
+    d3 sp;
+    if (!S.has_square) {
+        sp = ld3(F.spt + (size_t)task * 3);  // slots == stored points
+    } else {
+        int j = task - hq * S.shadow_per_hit, k = 0;
+        for (int li = 0;; ++li) {
+            const LightDev& L = S.lights[li];
+            const int cnt = light_count(S, L);
+            if (j < cnt) {
+                if (L.shape == kLightSquare)
+                    sp = square_sample(L, area_frame(F.path_seed[sr.path], p, ld3(L.pos)), j, cnt, light_cols(S, L));
+                else
+                    sp = ld3(F.spt + ((size_t)hq * S.spt_per_hit + k + j) * 3);
+                break;
+            }
+            j -= cnt;
+            if (L.shape != kLightSquare) k += cnt;
+        }
+    }
     d3 ld = unit(sub(sp, p));
     double dist = length(sub(sp, p));
     RayD r;
@@ -1050,52 +1130,45 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
     const MatDev& m = S.mats[sr.mat];
     d3 color = scale(m.ambient * S.amb_i, hadamard(ld3(S.amb), mc));
     d3 cam = ld3(S.cam_pos);
-    const double* spt = F.spt + (size_t)hq * S.shadow_per_hit * 3;
+    const double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
     const uint8_t* vis = F.vis + (size_t)hq * S.shadow_per_hit;
-    int j = 0;
+    uint64_t path_seed = 0;
+    if (S.has_square) path_seed = F.path_seed[sr.path];
+    int j = 0, k = 0;
     for (int li = 0; li < S.n_lights; ++li) {
         const LightDev& L = S.lights[li];
         d3 lc = ld3(L.color);
         if (!L.is_area) {
             // calculate_point_light_contribution (src/lighting.rs:100-142)
-            if (vis[j]) {
-                d3 ld = unit(sub(ld3(L.pos), p));
-                double ds = fmax_r(dot(n, ld), 0.0);
-                d3 diffuse = scale((m.diffuse * ds) * L.intensity, hadamard(lc, mc));
-                d3 spec = mk(0, 0, 0);
-                if (ds > 0.0) {
-                    d3 vd = unit(sub(cam, p));
-                    d3 rd = reflect_dir(neg(ld), n);
-                    double ss = pow(fmax_r(dot(vd, rd), 0.0), m.shininess);
-                    spec = scale((m.specular * ss) * L.intensity, lc);
-                }
-                color = add(color, add(diffuse, spec));
-            } else {
-                color = add(color, mk(0, 0, 0));
-            }
+            if (vis[j]) color = add(color, phong_sample(m, L, lc, mc, p, n, cam, ld3(L.pos)));
+            else color = add(color, mk(0, 0, 0));
             ++j;
+            ++k;
             continue;
         }
-        // calculate_diffuse_light_contribution (src/lighting.rs:145-217)
+        // calculate_diffuse_light_contribution (src/lighting.rs:145-217), n samples
+        const int cnt = light_count(S, L);
         d3 total = mk(0, 0, 0);
         int visible = 0;
-        for (int s = 0; s < 16; ++s, ++j) {
-            if (!vis[j]) continue;
-            ++visible;
-            d3 ld = unit(sub(ld3(spt + 3 * j), p));
-            double ds = fmax_r(dot(n, ld), 0.0);
-            d3 diffuse = scale((m.diffuse * ds) * L.intensity, hadamard(lc, mc));
-            d3 spec = mk(0, 0, 0);
-            if (ds > 0.0) {
-                d3 vd = unit(sub(cam, p));
-                d3 rd = reflect_dir(neg(ld), n);
-                double ss = pow(fmax_r(dot(vd, rd), 0.0), m.shininess);
-                spec = scale((m.specular * ss) * L.intensity, lc);
+        if (L.shape == kLightSquare) {
+            AreaFrame a = area_frame(path_seed, p, ld3(L.pos));
+            const int cols = light_cols(S, L);
+            for (int s = 0; s < cnt; ++s) {
+                if (!vis[j + s]) continue;
+                ++visible;
+                total = add(total, phong_sample(m, L, lc, mc, p, n, cam, square_sample(L, a, s, cnt, cols)));
             }
-            total = add(total, add(diffuse, spec));
+        } else {
+            for (int s = 0; s < cnt; ++s) {
+                if (!vis[j + s]) continue;
+                ++visible;
+                total = add(total, phong_sample(m, L, lc, mc, p, n, cam, ld3(spt + 3 * (k + s))));
+            }
+            k += cnt;
         }
-        // (total / 16) * (visible / 16)
-        color = add(color, scale((double)visible / 16.0, divs(total, 16.0)));
+        j += cnt;
+        // (total / n) * (visible / n)
+        color = add(color, scale((double)visible / (double)cnt, divs(total, (double)cnt)));
     }
     if (S.has_fog) {
         double dist = length(sub(p, cam));

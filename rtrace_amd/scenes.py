@@ -129,8 +129,12 @@ def config1(out_dir):
     return _write(out_dir, "config1", sc)
 
 
-def config2(out_dir, subdiv=6, light="disk"):
-    """Procedural mesh scene. subdiv=6 gives 81,920 triangles (BASELINE: ~80k)."""
+def config2(out_dir, subdiv=6, light="disk", light_size=1.0, light_samples=16):
+    """Procedural mesh scene. subdiv=6 gives 81,920 triangles (~80k).
+
+    light="disk": the reference's area light (diameter `light_size`, 16 rejection samples,
+    byte-comparable with the reference); light="square": the extension's square light of
+    side `light_size` with `light_samples` jittered-grid samples."""
     os.makedirs(out_dir, exist_ok=True)
     rng = SplitMix64(2)
     tris = icosphere(subdiv, 1.5, 0.05, seed=2)
@@ -151,11 +155,19 @@ def config2(out_dir, subdiv=6, light="disk"):
                     "material": material("#D8D8D0", 0.1, 0.7, 0.1, 8, reflectivity=0.2,
                                          texture={"type": "grid", "line_color": "#404040", "line_width": 0.04,
                                                   "cell_size": 1.0})})
-    lights = [{"position": [3, -4, 6], "color": "#FFFFFF", "intensity": 1.0, "diameter": 1.0}]
+    if light == "disk":
+        lt = {"diameter": light_size}
+        name = f"config2_s{subdiv}"
+    elif light == "square":
+        lt = {"shape": "square", "size": light_size, "samples": light_samples}
+        name = f"config2_s{subdiv}_square{light_size:g}_n{light_samples}"
+    else:
+        raise ValueError(f"unknown light {light!r}")
+    lights = [dict({"position": [3, -4, 6], "color": "#FFFFFF", "intensity": 1.0}, **lt)]
     sc = {"camera": _camera_persp([0, -7.5, 3.5], [0, 0, 0], 50), "objects": objects, "lights": lights,
           "scene_settings": _settings("#202838", fog={"color": "#B0C4DE", "density": 0.05, "start": 2.0,
                                                        "end": 20.0})}
-    return _write(out_dir, f"config2_s{subdiv}", sc)
+    return _write(out_dir, name, sc)
 
 
 def config3(out_dir, n_spheres=1000):
@@ -198,5 +210,6 @@ RENDER = {
     "config1": dict(width=800, height=600, aa="quincunx", samples=1, max_depth=10),
     "config2": dict(width=1280, height=720, aa="quincunx", samples=1, max_depth=4),
     "config2_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
+    "config2_square_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
     "config3": dict(width=1920, height=1080, aa="stochastic", samples=4, max_depth=10),
 }

@@ -44,6 +44,7 @@ def gpu_render(R, path, opts, with_hits=True):
     r.samples = opts.samples
     r.anti_aliasing_mode = opts.aa_mode
     r.seed = opts.seed if opts.has_seed else None
+    r.light_samples = opts.light_samples
     if opts.outline_enabled:
         r.outline_config = R.OutlineConfig(opts.outline_depth_weight, opts.outline_normal_weight,
                                            opts.outline_threshold, tuple(opts.outline_color),
@@ -128,6 +129,36 @@ def test_config_scenes_vs_oracle(R, config_dir, name, w, h, aa, spp, depth):
     img, st, gh = gpu_render(R, path, opts)
     ref, ost, rh = O.render(path, opts, want_hits=True)
     compare(name, img, ref, gh, rh)
+    for k in ("rays_primary", "rays_reflect", "rays_shadow"):
+        assert getattr(st, k) == ost[k], k
+
+
+@pytest.mark.parametrize("variant,light,ls,w,h", [
+    ("square16", {"shape": "square", "size": 1.0}, 0, 160, 90),
+    ("square8", {"shape": "square", "size": 1.0, "samples": 8}, 0, 160, 90),
+    ("square7_stochastic", {"shape": "square", "size": 1.5, "samples": 7}, 0, 128, 72),
+    ("disk8", {"samples": 8}, 0, 160, 90),
+    ("square_override4", {"shape": "square", "size": 2.0}, 4, 160, 90),
+    ("square_override256", {"shape": "square", "size": 2.0}, 256, 64, 36),
+    ("disk_override5", {}, 5, 160, 90),
+])
+def test_light_extension_vs_oracle(R, config_dir, variant, light, ls, w, h):
+    """Square lights, per-light sample counts and the per-render override on the mesh
+    scene: the same pixels, hit ids and shadow-ray counts as the oracle."""
+    src = config_dir["config2_s4"]
+    with open(src) as f:
+        sc = json.load(f)
+    for lt in sc["lights"]:
+        if "diameter" in lt:
+            lt.update(light)
+    path = os.path.join(os.path.dirname(src), f"light_{variant}.json")
+    with open(path, "w") as f:
+        json.dump(sc, f)
+    aa = "stochastic" if "stochastic" in variant else "quincunx"
+    opts = O.make_opts(w, h, aa=aa, samples=2 if aa == "stochastic" else 1, max_depth=4, light_samples=ls)
+    img, st, gh = gpu_render(R, path, opts)
+    ref, ost, rh = O.render(path, opts, want_hits=True)
+    compare(variant, img, ref, gh, rh)
     for k in ("rays_primary", "rays_reflect", "rays_shadow"):
         assert getattr(st, k) == ost[k], k
 

@@ -216,6 +216,27 @@ int rt_render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, uint8_t*
 /* Same, into a device buffer (no host copy); the call still blocks until done. */
 int rt_render_device(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
                      rt_render_stats* stats);
+/* ---- auxiliary outputs (extension; no reference counterpart) ----
+ * Per-pixel first-hit features for the denoiser and tile-masked re-rendering. Features are
+ * taken from the pixel's centre ray (quincunx) or its sample 0; misses leave zeros:
+ *   0-2 radiance RGB (the pixel's colour before quantisation), 3-5 albedo RGB (material
+ *   colour after textures), 6-8 normal XYZ (facing the ray), 9 depth (distance to the
+ *   camera position), 10 mean visibility of the area-light samples (of all shadow rays
+ *   when no light is an area light), 11 its binomial variance mean*(1-mean)/n, 12 fog
+ *   factor. */
+enum { RT_FEAT_COUNT = 13 };
+typedef struct {
+    float* features;           /* device, W*H*RT_FEAT_COUNT f32, or NULL */
+    float* radiance;           /* device, W*H*3 f32 linear colour before quantisation, or NULL */
+    const uint8_t* tile_mask;  /* device, ceil(W/T)*ceil(H/T) bytes, row-major; only pixels of tiles
+                                  != 0 are traced and written (others keep the output's contents);
+                                  NULL = every pixel. Not combinable with the outline pass. */
+    int32_t mask_tile;         /* T, tile edge in pixels (> 0 with a mask) */
+} rt_render_aux;
+/* rt_render_device plus auxiliary outputs. */
+int rt_render_device_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
+                         const rt_render_aux* aux, rt_render_stats* stats);
+
 /* Primary-ray hit records of the last render on ctx, in canonical order
  * (quincunx: [W*H centres][(W+1)*(H+1) corners]; else pixel-major, sample-minor):
  * object index (-1 miss), triangle index (-1 non-mesh), t. n = capacity. */

@@ -54,6 +54,9 @@ def lib():
         L.oracle_render.argtypes = [ctypes.c_char_p, ctypes.POINTER(OracleOpts), ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.POINTER(OracleStats), ctypes.c_char_p, ctypes.c_int]
+        L.oracle_render_aux.restype = ctypes.c_int
+        L.oracle_render_aux.argtypes = [ctypes.c_char_p, ctypes.POINTER(OracleOpts), ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.POINTER(OracleStats), ctypes.c_char_p, ctypes.c_int]
         L.oracle_chacha_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         L.oracle_seed_key.argtypes = [ctypes.c_uint64, ctypes.c_void_p]
         L.oracle_stdrng_u64.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
@@ -167,6 +170,24 @@ def render(scene_path, opts, want_hits=False):
         raise RuntimeError(f"oracle_render failed ({rc}): {err.value.decode()}")
     stats = {k: getattr(st, k) for k, _ in OracleStats._fields_}
     return rgb, stats, hits
+
+
+FEAT_COUNT = 13
+
+
+def render_aux(scene_path, opts):
+    """Returns (rgb HxWx3 uint8, features HxWx13 f32, radiance HxWx3 f32, stats dict)."""
+    W, H = opts.width, opts.height
+    rgb = np.zeros((H, W, 3), dtype=np.uint8)
+    feat = np.zeros((H, W, FEAT_COUNT), dtype=np.float32)
+    rad = np.zeros((H, W, 3), dtype=np.float32)
+    st = OracleStats()
+    err = ctypes.create_string_buffer(512)
+    rc = lib().oracle_render_aux(scene_path.encode(), ctypes.byref(opts), rgb.ctypes.data, feat.ctypes.data,
+                                 rad.ctypes.data, ctypes.byref(st), err, 512)
+    if rc != 0:
+        raise RuntimeError(f"oracle_render_aux failed ({rc}): {err.value.decode()}")
+    return rgb, feat, rad, {k: getattr(st, k) for k, _ in OracleStats._fields_}
 
 
 def chacha_block(words16, double_rounds):

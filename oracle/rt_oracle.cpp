@@ -1160,13 +1160,24 @@ const Material* apply_texture(const Material& m, double u, double v, Material& s
     }
     return &m;
 }
-V3 point_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V3 lc, V3 mc, Counters& C) {  // :100-142
+// First-hit feature probe (extension outputs for the denoiser; no reference counterpart):
+// shadow-sample visibility counts, effective material colour and fog factor.
+struct Probe {
+    int nv = 0, nt = 0;  // area-light samples visible / taken
+    int pv = 0, pt = 0;  // point lights visible / taken
+    V3 mc{};
+    double fog = 0.0;
+};
+V3 point_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V3 lc, V3 mc, Counters& C,
+               Probe* pr = nullptr) {  // :100-142
     V3 ld = normalize(L.pos - h.p);
     Ray sr = make_ray(h.p + 0.001 * h.n, ld);
     double dist = norm(L.pos - h.p);
     Hit tmp;
     C.shadow++;
+    if (pr) pr->pt++;
     if (X.world->hit(sr, 0.001, dist, tmp, C)) return v3(0, 0, 0);
+    if (pr) pr->pv++;
     double ds = rmax(dot(h.n, ld), 0.0);
     V3 diffuse = ((mat.diffuse * ds) * L.intensity) * cmul(lc, mc);
     V3 spec = v3(0, 0, 0);
@@ -1224,7 +1235,8 @@ V3 area_sample(const Light& L, V3 hp, StdRng& rng, u64 ls, int s, int cols, int 
     }
     return (L.pos + dx * u) + dy * v;
 }
-V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V3 lc, V3 mc, u64 seed, Counters& C) {  // :145-217
+V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V3 lc, V3 mc, u64 seed, Counters& C,
+              Probe* pr = nullptr) {  // :145-217
     const int SAMPLES = X.light_samples > 0 ? X.light_samples : L.samples;  // reference: 16 (:158)
     u64 ls = light_seed(seed, h.p);
     StdRng rng(ls);
@@ -1251,9 +1263,13 @@ V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V
         }
         total = total + (diffuse + spec);
     }
+    if (pr) {
+        pr->nt += SAMPLES;
+        pr->nv += visible;
+    }
     return (total / (double)SAMPLES) * ((double)visible / (double)SAMPLES);
 }
-V3 phong(const Ctx& X, const Hit& h, const Material& mat, u64 seed, Counters& C) {  // :220-288
+V3 phong(const Ctx& X, const Hit& h, const Material& mat, u64 seed, Counters& C, Probe* pr = nullptr) {  // :220-288
     Material scratch;
     const Material* eff = &mat;
     if (mat.tex != 0 && h.has_uv) eff = apply_texture(mat, h.u, h.v, scratch);
@@ -1262,12 +1278,14 @@ V3 phong(const Ctx& X, const Hit& h, const Material& mat, u64 seed, Counters& C)
     V3 color = (eff->ambient * X.amb_intensity) * cmul(ac, mc);
     for (const Light& L : X.lights) {
         V3 lc = hex_or(L.color, v3(1, 1, 1));
-        V3 contrib = (L.has_d || L.square) ? area_light(X, h, *eff, L, lc, mc, seed, C) : point_light(X, h, *eff, L, lc, mc, C);
+        V3 contrib = (L.has_d || L.square) ? area_light(X, h, *eff, L, lc, mc, seed, C, pr)
+                                           : point_light(X, h, *eff, L, lc, mc, C, pr);
         color = color + contrib;
     }
+    if (pr) pr->mc = mc;
     return color;
 }
-V3 apply_fog(const Ctx& X, V3 c, double dist) {  // :297-319
+V3 apply_fog(const Ctx& X, V3 c, double dist, double* f_out = nullptr) {  // :297-319
     if (!X.has_fog) return c;
     V3 fc = hex_or(X.fog.color, v3(0.5, 0.5, 0.5));
     double f;
@@ -1276,6 +1294,7 @@ V3 apply_fog(const Ctx& X, V3 c, double dist) {  // :297-319
     else f = (dist - X.fog.start) / (X.fog.end - X.fog.start);
     f = 1.0 - std::exp(-X.fog.density * f);
     f = rclamp(f, 0.0, 1.0);
+    if (f_out) *f_out = f;
     return c * (1.0 - f) + fc * f;
 }
 bool grid_color(const Camera& cam, const Ray& r, V3& out) {  // src/camera.rs:171-253
@@ -1317,8 +1336,12 @@ struct PrimRec {
 };
 // ray_color_with_camera / ray_color / ray_color_with_data (:323-506). `cam` is
 // null for ray_color (quincunx). depth/normal are the first hit's (with_data).
+// Features of the first hit (extension; zeros on a miss), in the device's channel order.
+struct Feat {
+    float v[13] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+};
 V3 ray_color(const Ctx& X, const Ray& r, int max_depth, const Camera* cam, u64 seed, Counters& C, bool primary,
-             PrimRec* rec, bool* has_dn, double* depth_out, V3* normal_out) {
+             PrimRec* rec, bool* has_dn, double* depth_out, V3* normal_out, Feat* feat = nullptr) {
     if (max_depth <= 0) return v3(0, 0, 0);
     Hit h;
     if (primary) C.primary++; else C.reflect++;
@@ -1330,9 +1353,22 @@ V3 ray_color(const Ctx& X, const Ray& r, int max_depth, const Camera* cam, u64 s
             *normal_out = h.n;
         }
         const Material& mat = X.mats[h.obj];
-        V3 color = phong(X, h, mat, seed, C);
+        Probe pr;
+        V3 color = phong(X, h, mat, seed, C, feat ? &pr : nullptr);
         double dist = norm(h.p - X.cam_pos);
-        color = apply_fog(X, color, dist);
+        color = apply_fog(X, color, dist, &pr.fog);
+        if (feat) {
+            float* f = feat->v;
+            f[3] = (float)pr.mc.x; f[4] = (float)pr.mc.y; f[5] = (float)pr.mc.z;
+            f[6] = (float)h.n.x; f[7] = (float)h.n.y; f[8] = (float)h.n.z;
+            f[9] = (float)dist;
+            int nv = pr.nv, nt = pr.nt;
+            if (nt == 0) { nv = pr.pv; nt = pr.pt; }
+            float mean = nt > 0 ? (float)nv / (float)nt : 1.0f;
+            f[10] = mean;
+            f[11] = nt > 0 ? mean * (1.0f - mean) / (float)nt : 0.0f;
+            f[12] = (float)pr.fog;
+        }
         if (mat.has_refl && mat.reflectivity > 0.0 && max_depth > 1) {
             V3 vd = normalize(X.cam_pos - h.p);
             V3 rd = reflect(-vd, h.n);
@@ -1682,7 +1718,7 @@ void parallel_rows(int rows, int threads, F fn) {
 }
 
 int do_render(const char* scene_path, const oracle_opts* op, uint8_t* rgb, int32_t* hobj, int32_t* htri, double* ht,
-              oracle_stats* st) {
+              oracle_stats* st, float* feat_out = nullptr, float* rad_out = nullptr) {
     if (op->samples == 0) throw Err(-1, "Samples must be greater than 0");
     std::ifstream f(scene_path, std::ios::binary);
     if (!f) throw Err(-3, std::string("No such file or directory (os error 2): ") + scene_path);
@@ -1730,8 +1766,11 @@ int do_render(const char* scene_path, const oracle_opts* op, uint8_t* rgb, int32
                 Ray r = get_ray(cam, u, v);
                 u64 s = seed * 0x9E3779B97F4A7C15ull + (u64)x * 0x85EBCA6Bull + (u64)y * 0xC2B2AE35ull + 0x12345678ull;
                 PrimRec rec;
-                V3 c = ray_color(X, r, op->max_depth, nullptr, s, cnt[y], true, &rec, nullptr, nullptr, nullptr);
+                Feat fe;
+                V3 c = ray_color(X, r, op->max_depth, nullptr, s, cnt[y], true, &rec, nullptr, nullptr, nullptr,
+                                 feat_out ? &fe : nullptr);
                 size_t k = (size_t)y * W + x;
+                if (feat_out) memcpy(feat_out + k * 13, fe.v, sizeof(fe.v));
                 if (hobj) { hobj[k] = rec.obj; htri[k] = rec.tri; ht[k] = rec.t; }
                 V3 tot = c + corners[(size_t)y * (W + 1) + x];
                 tot = tot + corners[(size_t)y * (W + 1) + x + 1];
@@ -1790,7 +1829,10 @@ int do_render(const char* scene_path, const oracle_opts* op, uint8_t* rgb, int32
                     bool hd = false;
                     double dd = 0;
                     V3 dn{};
-                    V3 c = ray_color(X, r, op->max_depth, &cam, ss, cnt[y], true, &rec, outline ? &hd : nullptr, &dd, &dn);
+                    Feat fe;
+                    V3 c = ray_color(X, r, op->max_depth, &cam, ss, cnt[y], true, &rec, outline ? &hd : nullptr, &dd, &dn,
+                                     (feat_out && s == 0) ? &fe : nullptr);
+                    if (feat_out && s == 0) memcpy(feat_out + ((size_t)y * W + x) * 13, fe.v, sizeof(fe.v));
                     if (hobj) {
                         size_t o = ((size_t)y * W + x) * N + s;
                         hobj[o] = rec.obj; htri[o] = rec.tri; ht[o] = rec.t;
@@ -1816,6 +1858,11 @@ int do_render(const char* scene_path, const oracle_opts* op, uint8_t* rgb, int32
         if (outline) outline_apply(W, H, depth, nrm, fg, *op, color);
     }
     auto t1 = std::chrono::steady_clock::now();
+    for (size_t i = 0; i < color.size(); ++i) {
+        float c3[3] = {(float)color[i].x, (float)color[i].y, (float)color[i].z};
+        if (rad_out) memcpy(rad_out + 3 * i, c3, sizeof(c3));
+        if (feat_out) memcpy(feat_out + 13 * i, c3, sizeof(c3));
+    }
     // create_image_from_data (src/renderer.rs:899-912)
     for (size_t i = 0; i < color.size(); ++i) {
         rgb[3 * i + 0] = sat_u8(rclamp(color[i].x, 0.0, 1.0) * 255.0);
@@ -1847,6 +1894,19 @@ void set_err(char* err, int errlen, const std::string& m) {
 }  // namespace
 
 extern "C" {
+
+int oracle_render_aux(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_out, float* features,
+                      float* radiance, oracle_stats* stats, char* err, int errlen) {
+    try {
+        return do_render(scene_path, opts, rgb_out, nullptr, nullptr, nullptr, stats, features, radiance);
+    } catch (const Err& e) {
+        set_err(err, errlen, e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return -99;
+    }
+}
 
 int oracle_render(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_out, int32_t* hit_obj,
                   int32_t* hit_tri, double* hit_t, oracle_stats* stats, char* err, int errlen) {

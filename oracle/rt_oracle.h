@@ -62,6 +62,13 @@ int oracle_render(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_
                   int32_t* hit_obj, int32_t* hit_tri, double* hit_t,
                   oracle_stats* stats, char* err, int errlen);
 
+/* Render plus the extension's per-pixel outputs (the device's rt_render_aux):
+ * features W*H*13 f32 (radiance RGB, albedo RGB, normal XYZ, depth, area-light visibility
+ * mean and binomial variance, fog factor; first hit of the centre ray or sample 0; zeros
+ * on a miss) and radiance W*H*3 f32 (colour before quantisation). Either may be NULL. */
+int oracle_render_aux(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_out, float* features,
+                      float* radiance, oracle_stats* stats, char* err, int errlen);
+
 /* Unit-level hooks used by the tests (each restates one reference function). */
 void oracle_chacha_block(const uint32_t in[16], int double_rounds, uint32_t out[16]);
 void oracle_seed_key(uint64_t seed, uint32_t key_out[8]);

@@ -49,6 +49,13 @@ class _Opts(ctypes.Structure):
     ]
 
 
+class _Aux(ctypes.Structure):
+    _fields_ = [("features", ctypes.c_void_p), ("radiance", ctypes.c_void_p), ("tile_mask", ctypes.c_void_p),
+                ("mask_tile", ctypes.c_int32)]
+
+
+FEAT_COUNT = 13  # rt_render_aux feature channels (include/rtrace_hip.h)
+
 STAGES = ["raygen", "closest", "shade", "shadow", "finish", "post"]
 KERNEL_CLOSEST, KERNEL_SHADOW = 0, 1
 
@@ -105,6 +112,8 @@ def lib():
         L.rt_render_opts_default.argtypes = [ctypes.POINTER(_Opts), ctypes.c_int32, ctypes.c_int32]
         L.rt_render.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
         L.rt_render_device.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
+        L.rt_render_device_aux.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(_Aux),
+                                           ctypes.POINTER(RenderStats)]
         L.rt_last_primary_hits.argtypes = [P, P, P, P, ctypes.c_size_t]
         L.rt_intersect.argtypes = [P, P, P, P, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                    P, P, P]
@@ -250,6 +259,22 @@ class Renderer:
         st = RenderStats()
         _check(lib().rt_render_device(_context(), scene._h, ctypes.byref(o), ctypes.c_void_p(out_tensor.data_ptr()),
                                       ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def render_device_aux(self, scene, out_tensor, features=None, radiance=None, tile_mask=None, mask_tile=16):
+        """render_device plus per-pixel features (H*W*13 f32), radiance (H*W*3 f32) and an
+        optional per-tile mask (uint8, ceil(H/T) x ceil(W/T)): only masked tiles are traced
+        and written into out_tensor. Tensors are device tensors."""
+        o = self._opts()
+        a = _Aux()
+        a.features = features.data_ptr() if features is not None else None
+        a.radiance = radiance.data_ptr() if radiance is not None else None
+        a.tile_mask = tile_mask.data_ptr() if tile_mask is not None else None
+        a.mask_tile = mask_tile
+        st = RenderStats()
+        _check(lib().rt_render_device_aux(_context(), scene._h, ctypes.byref(o), ctypes.c_void_p(out_tensor.data_ptr()),
+                                          ctypes.byref(a), ctypes.byref(st)))
         self.last_stats = st
         return st
 

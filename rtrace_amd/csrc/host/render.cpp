@@ -456,8 +456,27 @@ SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_sample
     return S;
 }
 
+// Camera rays traced: every path, or with a tile mask the masked pixels' rays (quincunx:
+// their centres and every corner that touches one).
+uint64_t primary_rays(const FrameView& F, long long n_paths) {
+    if (!F.tmask) return (uint64_t)n_paths;
+    const int tx = F.mtiles_x, ty = (F.H + F.mtile - 1) / F.mtile;
+    std::vector<uint8_t> m((size_t)tx * ty);
+    HIP_TRY(hipMemcpy(m.data(), F.tmask, m.size(), hipMemcpyDeviceToHost));
+    auto in = [&](int x, int y) {
+        return x >= 0 && y >= 0 && x < F.W && y < F.H && m[(size_t)(y / F.mtile) * tx + x / F.mtile] != 0;
+    };
+    uint64_t n = 0;
+    for (int y = 0; y < F.H; ++y)
+        for (int x = 0; x < F.W; ++x) n += in(x, y) ? (F.aa == 1 ? 1 : F.N) : 0;
+    if (F.aa == 1)
+        for (int y = 0; y <= F.H; ++y)
+            for (int x = 0; x <= F.W; ++x) n += (in(x - 1, y - 1) || in(x, y - 1) || in(x - 1, y) || in(x, y)) ? 1 : 0;
+    return n;
+}
+
 void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host_rgb, void* dev_rgb,
-            rt_render_stats* stats) {
+            rt_render_stats* stats, const rt_render_aux* aux = nullptr) {
     if (!scene || !scene->dev || !o) throw Error(RT_E_INVALID_ARG, "null scene or options");
     if (o->samples == 0) throw Error(RT_E_INVALID_ARG, "Samples must be greater than 0");
     if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
@@ -553,6 +572,18 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     memcpy(F.ol_color, o->outline_color, sizeof(F.ol_color));
     F.ol_8 = o->outline_use_8_neighbors;
     F.max_depth = depth;
+    if (aux) {
+        F.feat = aux->features;
+        F.rad = aux->radiance;
+        if (aux->tile_mask) {
+            if (aux->mask_tile <= 0) throw Error(RT_E_INVALID_ARG, "mask_tile must be positive");
+            if (F.outline) throw Error(RT_E_INVALID_ARG, "a tile mask cannot be combined with the outline pass");
+            if (host_rgb) throw Error(RT_E_INVALID_ARG, "a tile mask needs a device output");
+            F.tmask = aux->tile_mask;
+            F.mtile = aux->mask_tile;
+            F.mtiles_x = (W + aux->mask_tile - 1) / aux->mask_tile;
+        }
+    }
 
     if (depth == 0) {
         // ray_color returns black at max_depth <= 0 before any World::hit; mark the
@@ -577,7 +608,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         memset(stats, 0, sizeof(*stats));
-        stats->rays_primary = depth > 0 ? (uint64_t)n_paths : 0;
+        stats->rays_primary = depth > 0 ? primary_rays(F, n_paths) : 0;
         for (int l = 1; l < depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
         for (int l = 0; l < depth; ++l) stats->rays_shadow += (uint64_t)hit_count[l] * (uint64_t)(S.n_lights > 0 ? S.shadow_per_hit : 0);
         for (int k = 0; k < 2; ++k) {
@@ -646,6 +677,17 @@ rt_scene* make_scene(rt_ctx* ctx, HostScene&& hs) {
 }
 
 }  // namespace
+}  // namespace rt
+
+namespace rt {
+// Context access for the other host modules (host/context.hpp).
+hipStream_t ctx_stream(rt_ctx* ctx) { return ctx->stream; }
+int ctx_device(rt_ctx* ctx) { return ctx->device; }
+int ctx_fail(rt_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
+void ctx_render_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
+                    rt_render_stats* stats) {
+    render(ctx, scene, o, nullptr, dev_rgb, stats, aux);
+}
 }  // namespace rt
 
 using namespace rt;
@@ -752,6 +794,12 @@ int rt_render_device(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, v
                      rt_render_stats* stats) {
     if (!ctx || !d_rgb_out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
     return guarded(ctx, [&] { render(ctx, scene, opts, nullptr, d_rgb_out, stats); });
+}
+
+int rt_render_device_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
+                         const rt_render_aux* aux, rt_render_stats* stats) {
+    if (!ctx || !d_rgb_out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] { render(ctx, scene, opts, nullptr, d_rgb_out, stats, aux); });
 }
 
 int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, size_t n) {

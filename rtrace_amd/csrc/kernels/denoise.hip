@@ -1,0 +1,492 @@
+// denoise.hip — BF16 MFMA kernels of the shadow denoiser (a 4-level U-Net of 3x3 convs).
+//
+// Tensors are NHWC: [batch][height][width][channels] with channels a multiple of 32, so a
+// 16-byte vector is 8 consecutive channels of one pixel. Activations are bf16, gradients
+// f32, and every product runs on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.
+//
+// A 3x3 convolution is an implicit GEMM: M = pixels, N = output channels, K = 9 taps x
+// input channels. A workgroup owns an 8x16-pixel tile and 32 output channels; per K chunk
+// of 32 input channels it stages the 10x18 input halo and the 9x32x32 weight block in LDS
+// and each of its four waves runs 9 taps x (2 pixel rows x 2 channel halves) MFMAs. The
+// same kernel computes backward-data (input = dZ, weights flipped and transposed, f32
+// output). Weight gradients stage the halo and dZ transposed (channel-major, one copy per
+// x shift) so that both MFMA operands are 16-byte aligned 8-pixel runs.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "denoise_kernels.hpp"
+
+namespace rt {
+namespace dn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+constexpr int kTH = 8, kTW = 16;                   // output tile (pixels)
+constexpr int kHH = kTH + 2, kHW = kTW + 2;        // input halo
+constexpr int kCK = 32;                            // input channels per K chunk
+constexpr int kPS = 40;                            // LDS pixel stride in bf16 (80 B)
+constexpr int kNT = 32;                            // output channels per workgroup
+
+// 16-byte vector of channels [c, c+8) of the (virtual) input pixel (b, y, x) of a conv whose
+// input is concat(src0 (optionally at half resolution, nearest upsampling), src1).
+__device__ __forceinline__ uint4 gather8(const ConvIn& in, int b, int y, int x, int H, int W, int c) {
+    if (y < 0 || x < 0 || y >= H || x >= W) return make_uint4(0, 0, 0, 0);
+    if (c < in.c0) {
+        const uint16_t* p;
+        if (in.up0) {
+            const int h2 = H >> 1, w2 = W >> 1;
+            p = in.src0 + (((size_t)b * h2 + (y >> 1)) * w2 + (x >> 1)) * in.c0;
+        } else {
+            p = in.src0 + (((size_t)b * H + y) * W + x) * in.c0;
+        }
+        return *reinterpret_cast<const uint4*>(p + c);
+    }
+    const uint16_t* p = in.src1 + (((size_t)b * H + y) * W + x) * in.c1;
+    return *reinterpret_cast<const uint4*>(p + (c - in.c0));
+}
+
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[kHH * kHW * kPS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[9 * kNT * kPS];
+    const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+    int t = blockIdx.x;
+    const int b = t / (tiles_x * tiles_y);
+    t -= b * tiles_x * tiles_y;
+    const int y0 = (t / tiles_x) * kTH, x0 = (t % tiles_x) * kTW;
+    const int co0 = blockIdx.y * kNT;
+    const int cin = a.in.c0 + a.in.c1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int ci0 = 0; ci0 < cin; ci0 += kCK) {
+        for (int v = threadIdx.x; v < kHH * kHW * 4; v += 256) {
+            const int px = v >> 2, q = v & 3;
+            const int hy = px / kHW, hx = px - hy * kHW;
+            uint4 val = gather8(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8);
+            *reinterpret_cast<uint4*>(s_in + px * kPS + q * 8) = val;
+        }
+        for (int v = threadIdx.x; v < 9 * kNT * 4; v += 256) {
+            const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
+            uint4 val = *reinterpret_cast<const uint4*>(a.wt + ((size_t)tap * a.coutp + co0 + n) * cin + ci0 + q * 8);
+            *reinterpret_cast<uint4*>(s_w + (tap * kNT + n) * kPS + q * 8) = val;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap - dy * 3;
+            bf16x8 bfr[2];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+                bfr[nt] = *reinterpret_cast<const bf16x8*>(s_w + (tap * kNT + nt * 16 + r16) * kPS + g * 8);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int row = wave * 2 + mt;
+                bf16x8 afr = *reinterpret_cast<const bf16x8*>(s_in + ((row + dy) * kHW + r16 + dx) * kPS + g * 8);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // D layout: lane holds rows (pixels) 4g..4g+3 of column (channel) r16
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int y = y0 + wave * 2 + mt;
+        if (y >= a.H) continue;
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int co = co0 + nt * 16 + r16;
+            if (co >= a.cout) continue;
+            const float bias = a.bias ? a.bias[co] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int x = x0 + 4 * g + i;
+                if (x >= a.W) continue;
+                float v = acc[mt][nt][i] + bias;
+                const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.out_pitch + co;
+                if (a.out_f32) {
+                    a.out_f32[o] = v;
+                } else {
+                    if (a.relu) v = fmaxf(v, 0.0f);
+                    a.out_bf16[o] = f2bf(v);
+                }
+            }
+        }
+    }
+}
+
+// Weight-gradient partial sums: part[split][tap][ci][co] = sum over this split's tiles of
+// X(p + tap)[ci] * dZ(p)[co]. grid.y enumerates (32-ci block, 32-co block).
+__global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
+    __shared__ __attribute__((aligned(16))) uint16_t s_x[3][kCK][kHH][kTW];   // x-shifted, channel-major
+    __shared__ __attribute__((aligned(16))) uint16_t s_z[kNT][kTH][kTW];      // channel-major
+    const int cin = a.in.c0 + a.in.c1;
+    const int nci = cin / kCK;
+    const int ci0 = (blockIdx.y % nci) * kCK, co0 = (blockIdx.y / nci) * kNT;
+    const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+    const int n_tiles = a.B * tiles_x * tiles_y;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+    const int cit = wave >> 1, cot = wave & 1;  // this wave's 16x16 (ci, co) block, all 9 taps
+    f32x4 acc[9];
+    for (int j = 0; j < 9; ++j) acc[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        int t = tile;
+        const int b = t / (tiles_x * tiles_y);
+        t -= b * tiles_x * tiles_y;
+        const int y0 = (t / tiles_x) * kTH, x0 = (t % tiles_x) * kTW;
+        for (int v = threadIdx.x; v < kHH * kHW * 4; v += 256) {
+            const int px = v >> 2, q = v & 3;
+            const int hy = px / kHW, hx = px - hy * kHW;
+            uint4 val = gather8(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8);
+            const uint16_t* h = reinterpret_cast<const uint16_t*>(&val);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int x = hx - dx;
+                if (x < 0 || x >= kTW) continue;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) s_x[dx][q * 8 + k][hy][x] = h[k];
+            }
+        }
+        for (int v = threadIdx.x; v < kTH * kTW * 4; v += 256) {
+            const int px = v >> 2, q = v & 3;
+            const int ty = px / kTW, tx = px - ty * kTW;
+            const int y = y0 + ty, x = x0 + tx;
+            uint4 val = make_uint4(0, 0, 0, 0);
+            if (y < a.H && x < a.W)
+                val = *reinterpret_cast<const uint4*>(a.dz + (((size_t)b * a.H + y) * a.W + x) * a.coutp + co0 + q * 8);
+            const uint16_t* h = reinterpret_cast<const uint16_t*>(&val);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) s_z[q * 8 + k][ty][tx] = h[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int rowk = 2 * kk + (g >> 1), xk = 8 * (g & 1);
+            const bf16x8 bfr = *reinterpret_cast<const bf16x8*>(&s_z[cot * 16 + r16][rowk][xk]);
+#pragma unroll
+            for (int j = 0; j < 9; ++j) {
+                const int dy = j / 3, dx = j - dy * 3;
+                const bf16x8 afr = *reinterpret_cast<const bf16x8*>(&s_x[dx][cit * 16 + r16][rowk + dy][xk]);
+                acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr, acc[j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* out = a.part + (size_t)blockIdx.x * 9 * cin * a.coutp;
+#pragma unroll
+    for (int j = 0; j < 9; ++j)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            out[((size_t)j * cin + ci0 + cit * 16 + 4 * g + i) * a.coutp + co0 + cot * 16 + r16] = acc[j][i];
+}
+
+// Bias-gradient partial sums: part[split][c] = sum of dz over this split's pixels.
+__global__ __launch_bounds__(256) void k_bias_grad(const uint16_t* dz, long long n_px, int c, float* part) {
+    __shared__ float s[256];
+    const int lanes = 256 / c;  // c in {32, 64, 128}
+    const int ch = threadIdx.x % c, pl = threadIdx.x / c;
+    float sum = 0.0f;
+    for (long long p = (long long)blockIdx.x * lanes + pl; p < n_px; p += (long long)gridDim.x * lanes)
+        sum += bf2f(dz[p * c + ch]);
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    if (pl == 0) {
+        for (int k = 1; k < lanes; ++k) sum += s[k * c + ch];
+        part[(size_t)blockIdx.x * c + ch] = sum;
+    }
+}
+
+// out[i] (=|+=) sum over s of part[s][i], in split order.
+__global__ void k_reduce_splits(const float* part, int splits, long long n, float* out, int accumulate) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int k = 0; k < splits; ++k) s += part[(size_t)k * n + i];
+        out[i] = accumulate ? out[i] + s : s;
+    }
+}
+
+// Input features -> 32-channel bf16 network input (channels 13..31 zero), zero outside the
+// frame (the network runs on a size rounded up to a multiple of 8).
+__global__ void k_prep(const float* feat, int B, int H, int W, int Hp, int Wp, uint16_t* x0) {
+    const long long n = (long long)B * Hp * Wp;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % Wp);
+        const int y = (int)((i / Wp) % Hp);
+        const int b = (int)(i / ((long long)Wp * Hp));
+        uint16_t* o = x0 + i * 32;
+        float v[kFeatIn];
+        if (y < H && x < W) {
+            const float* f = feat + (((size_t)b * H + y) * W + x) * kFeatIn;
+            for (int c = 0; c < kFeatIn; ++c) v[c] = f[c];
+            for (int c = 0; c < 3; ++c) v[c] = fminf(fmaxf(v[c], 0.0f), 4.0f);  // radiance
+            v[9] = log1pf(fmaxf(v[9], 0.0f)) * 0.25f;                             // depth
+            v[11] = sqrtf(fmaxf(v[11], 0.0f)) * 4.0f;                             // visibility std
+        } else {
+            for (int c = 0; c < kFeatIn; ++c) v[c] = 0.0f;
+        }
+        for (int c = 0; c < 32; ++c) o[c] = f2bf(c < kFeatIn ? v[c] : 0.0f);
+    }
+}
+
+// 2x2 average pooling of a bf16 tensor (H, W even).
+__global__ void k_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out) {
+    const int h2 = H >> 1, w2 = W >> 1;
+    const long long n = (long long)B * h2 * w2 * C;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long long p = i / C;
+        const int x = (int)(p % w2);
+        const int y = (int)((p / w2) % h2);
+        const int b = (int)(p / ((long long)w2 * h2));
+        const uint16_t* q = in + (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+        float s = (bf2f(q[0]) + bf2f(q[C])) + (bf2f(q[(size_t)W * C]) + bf2f(q[(size_t)W * C + C]));
+        out[i] = f2bf(s * 0.25f);
+    }
+}
+
+// g_full[b][y][x][c] += 0.25 * g_half[b][y/2][x/2][c]
+__global__ void k_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf) {
+    const long long n = (long long)B * H * W * C;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        long long p = i / C;
+        const int x = (int)(p % W);
+        const int y = (int)((p / W) % H);
+        const int b = (int)(p / ((long long)W * H));
+        gf[i] += 0.25f * gh[(((size_t)b * (H >> 1) + (y >> 1)) * (W >> 1) + (x >> 1)) * C + c];
+    }
+}
+
+// Gradient of concat(up(low), skip): g_low = sum over each 2x2 block of dx[..., :c0],
+// g_skip = dx[..., c0:] (written, it is the skip's first contribution).
+__global__ void k_split(const float* dx, int B, int H, int W, int c0, int c1, float* g_low, float* g_skip) {
+    const int C = c0 + c1;
+    const long long n_skip = (long long)B * H * W * c1;
+    const int h2 = H >> 1, w2 = W >> 1;
+    const long long n_low = (long long)B * h2 * w2 * c0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_skip + n_low;
+         i += (long long)gridDim.x * blockDim.x) {
+        if (i < n_skip) {
+            const long long p = i / c1;
+            g_skip[i] = dx[p * C + c0 + (i % c1)];
+        } else {
+            const long long j = i - n_skip;
+            const int c = (int)(j % c0);
+            long long p = j / c0;
+            const int x = (int)(p % w2);
+            const int y = (int)((p / w2) % h2);
+            const int b = (int)(p / ((long long)w2 * h2));
+            const float* q = dx + (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
+            g_low[j] = (q[0] + q[C]) + (q[(size_t)W * C] + q[(size_t)W * C + C]);
+        }
+    }
+}
+
+// dz = a > 0 ? bf16(g) : 0 (ReLU backward; `a` is the stored post-ReLU activation)
+__global__ void k_relu_bwd(const float* g, const uint16_t* a, long long n, uint16_t* dz) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dz[i] = bf2f(a[i]) > 0.0f ? f2bf(g[i]) : (uint16_t)0;
+}
+
+// Residual output, loss and its gradient. y: [B][Hp][Wp][3] network output; the result is
+// noisy + y on the H x W frame. dz: [B][Hp][Wp][32] bf16 = dLoss/dy in channels 0..2
+// (loss = mean squared error over B*H*W*3 values); part[block] = partial sum of squares.
+__global__ __launch_bounds__(256) void k_loss(const float* y, const float* feat, const float* target, int B, int H,
+                                              int W, int Hp, int Wp, uint16_t* dz, float* part) {
+    __shared__ float s[256];
+    const long long n = (long long)B * Hp * Wp;
+    const float scale = 2.0f / (float)((long long)B * H * W * 3);
+    float acc = 0.0f;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % Wp);
+        const int yy = (int)((i / Wp) % Hp);
+        const int b = (int)(i / ((long long)Wp * Hp));
+        uint16_t* d = dz + i * 32;
+        for (int c = 3; c < 32; ++c) d[c] = 0;
+        if (yy < H && x < W) {
+            const size_t p = ((size_t)b * H + yy) * W + x;
+            for (int c = 0; c < 3; ++c) {
+                const float diff = (feat[p * kFeatIn + c] + y[i * 3 + c]) - target[p * 3 + c];
+                acc += diff * diff;
+                d[c] = f2bf(scale * diff);
+            }
+        } else {
+            d[0] = d[1] = d[2] = 0;
+        }
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+// Denoised frame: rgb = noisy + y (f32 and/or quantised like create_image_from_data).
+__global__ void k_compose(const float* y, const float* feat, int H, int W, int Wp, float* out_f32, uint8_t* out_u8) {
+    const long long n = (long long)H * W;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(i % W), yy = (int)(i / W);
+        const size_t q = ((size_t)yy * Wp + x) * 3;
+        for (int c = 0; c < 3; ++c) {
+            const float v = feat[i * kFeatIn + c] + y[q + c];
+            if (out_f32) out_f32[i * 3 + c] = v;
+            if (out_u8) {
+                const float cl = fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f;
+                out_u8[i * 3 + c] = (uint8_t)(cl >= 255.0f ? 255 : (int)cl);
+            }
+        }
+    }
+}
+
+// Per-tile fallback decision: a tile is re-rendered when the mean over its pixels of the
+// binomial variance of the visibility estimate exceeds `threshold`.
+__global__ __launch_bounds__(256) void k_tile_mask(const float* feat, int H, int W, int T, float threshold,
+                                                   uint8_t* mask, int* n_flagged) {
+    __shared__ float s[256];
+    const int tx = (W + T - 1) / T;
+    const int x0 = (blockIdx.x % tx) * T, y0 = (blockIdx.x / tx) * T;
+    float acc = 0.0f;
+    for (int k = threadIdx.x; k < T * T; k += 256) {
+        const int x = x0 + k % T, y = y0 + k / T;
+        if (x < W && y < H) acc += feat[((size_t)y * W + x) * kFeatIn + 11];
+    }
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int w = min(T, W - x0), h = min(T, H - y0);
+        const bool f = s[0] / (float)(w * h) > threshold;
+        mask[blockIdx.x] = f ? 1 : 0;
+        if (f) atomicAdd(n_flagged, 1);
+    }
+}
+
+// Adam (Kingma & Ba 2015) on the f32 master parameters.
+__global__ void k_adam(float* w, const float* grad, float* m, float* v, long long n, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float gi = grad[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        w[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    }
+}
+
+// Master [9][cin][coutp] f32 -> forward bf16 [9][coutp][cin] and backward-data bf16
+// [9][cin][coutp] with the taps flipped (tap t reads master tap 8 - t).
+__global__ void k_pack(const float* w, int cin, int coutp, uint16_t* wf, uint16_t* wb) {
+    const long long n = 9ll * cin * coutp;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const int co = (int)(i % coutp);
+        const int ci = (int)((i / coutp) % cin);
+        const int tap = (int)(i / ((long long)coutp * cin));
+        const uint16_t h = f2bf(w[i]);
+        wf[((size_t)tap * coutp + co) * cin + ci] = h;
+        if (wb) wb[((size_t)(8 - tap) * cin + ci) * coutp + co] = h;
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+namespace {
+int grid1(long long n, int block, int cap) {
+    long long g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    return (int)(g < cap ? g : cap);
+}
+}  // namespace
+
+hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
+    const int tiles = a.B * ((a.H + kTH - 1) / kTH) * ((a.W + kTW - 1) / kTW);
+    hipLaunchKernelGGL(k_conv3x3, dim3(tiles, a.coutp / kNT), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st) {
+    const int cin = a.in.c0 + a.in.c1;
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(splits, (cin / kCK) * (a.coutp / kNT)), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+hipError_t launch_bias_grad(const uint16_t* dz, long long n_px, int c, float* part, int splits, hipStream_t st) {
+    hipLaunchKernelGGL(k_bias_grad, dim3(splits), dim3(256), 0, st, dz, n_px, c, part);
+    return hipGetLastError();
+}
+hipError_t launch_reduce(const float* part, int splits, long long n, float* out, bool acc, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_splits, dim3(grid1(n, 256, 4096)), dim3(256), 0, st, part, splits, n, out, acc ? 1 : 0);
+    return hipGetLastError();
+}
+hipError_t launch_prep(const float* feat, int B, int H, int W, int Hp, int Wp, uint16_t* x0, hipStream_t st) {
+    hipLaunchKernelGGL(k_prep, dim3(grid1((long long)B * Hp * Wp, 256, 8192)), dim3(256), 0, st, feat, B, H, W, Hp, Wp,
+                       x0);
+    return hipGetLastError();
+}
+hipError_t launch_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_pool, dim3(grid1((long long)B * H * W * C / 4, 256, 8192)), dim3(256), 0, st, in, B, H, W, C,
+                       out);
+    return hipGetLastError();
+}
+hipError_t launch_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf, hipStream_t st) {
+    hipLaunchKernelGGL(k_pool_bwd, dim3(grid1((long long)B * H * W * C, 256, 8192)), dim3(256), 0, st, gh, B, H, W, C,
+                       gf);
+    return hipGetLastError();
+}
+hipError_t launch_split(const float* dx, int B, int H, int W, int c0, int c1, float* g_low, float* g_skip,
+                        hipStream_t st) {
+    const long long n = (long long)B * H * W * c1 + (long long)B * (H / 2) * (W / 2) * c0;
+    hipLaunchKernelGGL(k_split, dim3(grid1(n, 256, 8192)), dim3(256), 0, st, dx, B, H, W, c0, c1, g_low, g_skip);
+    return hipGetLastError();
+}
+hipError_t launch_relu_bwd(const float* g, const uint16_t* a, long long n, uint16_t* dz, hipStream_t st) {
+    hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(n, 256, 8192)), dim3(256), 0, st, g, a, n, dz);
+    return hipGetLastError();
+}
+hipError_t launch_loss(const float* y, const float* feat, const float* target, int B, int H, int W, int Hp, int Wp,
+                       uint16_t* dz, float* part, int blocks, hipStream_t st) {
+    hipLaunchKernelGGL(k_loss, dim3(blocks), dim3(256), 0, st, y, feat, target, B, H, W, Hp, Wp, dz, part);
+    return hipGetLastError();
+}
+hipError_t launch_compose(const float* y, const float* feat, int H, int W, int Wp, float* out_f32, uint8_t* out_u8,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_compose, dim3(grid1((long long)H * W, 256, 8192)), dim3(256), 0, st, y, feat, H, W, Wp,
+                       out_f32, out_u8);
+    return hipGetLastError();
+}
+hipError_t launch_tile_mask(const float* feat, int H, int W, int T, float threshold, uint8_t* mask, int* n_flagged,
+                            hipStream_t st) {
+    const int tiles = ((W + T - 1) / T) * ((H + T - 1) / T);
+    hipLaunchKernelGGL(k_tile_mask, dim3(tiles), dim3(256), 0, st, feat, H, W, T, threshold, mask, n_flagged);
+    return hipGetLastError();
+}
+hipError_t launch_adam(float* w, const float* grad, float* m, float* v, long long n, float lr, float b1, float b2,
+                       float eps, float bc1, float bc2, hipStream_t st) {
+    hipLaunchKernelGGL(k_adam, dim3(grid1(n, 256, 4096)), dim3(256), 0, st, w, grad, m, v, n, lr, b1, b2, eps, bc1, bc2);
+    return hipGetLastError();
+}
+hipError_t launch_pack(const float* w, int cin, int coutp, uint16_t* wf, uint16_t* wb, hipStream_t st) {
+    hipLaunchKernelGGL(k_pack, dim3(grid1(9ll * cin * coutp, 256, 4096)), dim3(256), 0, st, w, cin, coutp, wf, wb);
+    return hipGetLastError();
+}
+
+}  // namespace dn
+}  // namespace rt

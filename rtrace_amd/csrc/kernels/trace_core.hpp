@@ -108,7 +108,29 @@ struct FrameView {
     double ol_wd, ol_wn, ol_thr, ol_thick;
     double ol_color[3];
     int32_t ol_8;
+    // auxiliary outputs (null when not requested)
+    float* feat;              // [W*H][kFeatCount] first-hit features of each pixel
+    float* rad;               // [W*H][3] linear colour before quantisation
+    const uint8_t* tmask;     // [mtiles_y][mtiles_x]: trace only pixels of tiles != 0
+    int32_t mtile, mtiles_x;  // mask tile size in pixels, tiles per row
 };
+
+// Feature channels: noisy radiance RGB, albedo RGB, normal XYZ, depth, mean and variance
+// of the area-light visibility, fog factor; taken from the centre ray (quincunx) or
+// sample 0 of the pixel. Misses leave zeros.
+enum : int32_t { kFeatRad = 0, kFeatAlbedo = 3, kFeatNormal = 6, kFeatDepth = 9, kFeatVisMean = 10,
+                 kFeatVisVar = 11, kFeatFog = 12, kFeatCount = 13 };
+
+// Pixel that path `path` provides the features of, or -1.
+RT_HD int feature_pixel(const FrameView& F, int path) {
+    if (F.aa == 1) return path < F.W * F.H ? path : -1;
+    return path % F.N == 0 ? path / F.N : -1;
+}
+RT_HD bool pixel_in_mask(const FrameView& F, int x, int y) {
+    if (!F.tmask) return true;
+    if (x < 0 || y < 0 || x >= F.W || y >= F.H) return false;
+    return F.tmask[(y / F.mtile) * F.mtiles_x + x / F.mtile] != 0;
+}
 
 // --------------------------------------------------------------- helpers
 struct Work {
@@ -731,7 +753,12 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
         int tx = (gw + kTile - 1) / kTile;
         int tile = qq / (kTile * kTile), in = qq % (kTile * kTile);
         int x = (tile % tx) * kTile + in % kTile, y = (tile / tx) * kTile + in / kTile;
-        if (x < gw && y < gh) {
+        bool want = x < gw && y < gh;
+        if (want && F.tmask)
+            want = corner ? (pixel_in_mask(F, x - 1, y - 1) || pixel_in_mask(F, x, y - 1) ||
+                             pixel_in_mask(F, x - 1, y) || pixel_in_mask(F, x, y))
+                          : pixel_in_mask(F, x, y);
+        if (want) {
             double pw = 1.0 / (double)F.W, ph = 1.0 / (double)F.H;
             double u, v;
             uint64_t s;
@@ -753,6 +780,10 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
             st3(rec.d, r.d);
             rec.path = path;
             F.path_seed[path] = s;
+            if (F.feat && !corner) {
+                float* fp = F.feat + (size_t)path * kFeatCount;
+                for (int c = 0; c < kFeatCount; ++c) fp[c] = 0.0f;
+            }
         }
     } else {
         // render_standard / render_standard_with_outline (src/renderer.rs:445-749);
@@ -761,7 +792,7 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
         int smp = q / tiled, qq = q % tiled;
         int tile = qq / (kTile * kTile), in = qq % (kTile * kTile);
         int x = (tile % F.tiles_x) * kTile + in % kTile, y = (tile / F.tiles_x) * kTile + in / kTile;
-        if (x < F.W && y < F.H && smp < F.N) {
+        if (x < F.W && y < F.H && smp < F.N && pixel_in_mask(F, x, y)) {
             double pu = (double)x / (double)(F.W - 1);
             double pv = (double)(F.H - 1 - y) / (double)(F.H - 1);
             double pw = 1.0 / (double)(F.W - 1), ph = 1.0 / (double)(F.H - 1);
@@ -794,6 +825,10 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
             int path = (y * F.W + x) * F.N + smp;
             rec.path = path;
             F.path_seed[path] = ps + (uint64_t)smp * 0x1F845FEDull;
+            if (F.feat && smp == 0) {
+                float* fp = F.feat + (size_t)(y * F.W + x) * kFeatCount;
+                for (int c = 0; c < kFeatCount; ++c) fp[c] = 0.0f;
+            }
         }
     }
     F.rays[q] = rec;
@@ -1170,6 +1205,7 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
         // (total / n) * (visible / n)
         color = add(color, scale((double)visible / (double)cnt, divs(total, (double)cnt)));
     }
+    double fog_f = 0.0;
     if (S.has_fog) {
         double dist = length(sub(p, cam));
         double f;
@@ -1180,6 +1216,43 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
         f = clamp_r(f, 0.0, 1.0);
         d3 fc = ld3(S.fog_color);
         color = add(scale(1.0 - f, color), scale(f, fc));  // color * (1 - f) + fog * f
+        fog_f = f;
+    }
+    if (F.feat && sr.level == 0) {
+        int fpix = feature_pixel(F, sr.path);
+        if (fpix >= 0) {
+            float* fp = F.feat + (size_t)fpix * kFeatCount;
+            fp[kFeatAlbedo + 0] = (float)mc.x;
+            fp[kFeatAlbedo + 1] = (float)mc.y;
+            fp[kFeatAlbedo + 2] = (float)mc.z;
+            fp[kFeatNormal + 0] = (float)n.x;
+            fp[kFeatNormal + 1] = (float)n.y;
+            fp[kFeatNormal + 2] = (float)n.z;
+            fp[kFeatDepth] = (float)length(sub(p, cam));
+            // visibility of the area-light samples (all shadow slots when there is no area light)
+            int nv = 0, nt = 0, pv = 0, pt = 0;
+            for (int li = 0, jj = 0; li < S.n_lights; ++li) {
+                const int cnt = light_count(S, S.lights[li]);
+                int v = 0;
+                for (int q = 0; q < cnt; ++q) v += vis[jj + q];
+                if (S.lights[li].is_area) {
+                    nv += v;
+                    nt += cnt;
+                } else {
+                    pv += v;
+                    pt += cnt;
+                }
+                jj += cnt;
+            }
+            if (nt == 0) {
+                nv = pv;
+                nt = pt;
+            }
+            float mean = nt > 0 ? (float)nv / (float)nt : 1.0f;
+            fp[kFeatVisMean] = mean;
+            fp[kFeatVisVar] = nt > 0 ? mean * (1.0f - mean) / (float)nt : 0.0f;
+            fp[kFeatFog] = (float)fog_f;
+        }
     }
     double* Lr = F.lvl + ((size_t)sr.level * F.n_paths + sr.path) * 4;
     Lr[0] = color.x;
@@ -1217,6 +1290,7 @@ RT_HD d3 path_color(const FrameView& F, int path) {
 // (src/renderer.rs:552, :692-698, :881-886).
 RT_HD void pixel_item(const FrameView& F, int pix) {
     int x = pix % F.W, y = pix / F.W;
+    if (!pixel_in_mask(F, x, y)) return;
     d3 c;
     if (F.max_depth <= 0) {
         c = mk(0, 0, 0);
@@ -1326,6 +1400,7 @@ RT_HD double dilate_item(const FrameView& F, const double* mask, int x, int y) {
 // blend_colors (src/outline.rs:271-278) then create_image_from_data quantisation
 // (src/renderer.rs:899-912)
 RT_HD void finalize_item(const FrameView& F, const double* mask, int pix) {
+    if (!pixel_in_mask(F, pix % F.W, pix / F.W)) return;  // keep what the output holds
     double* c = F.pix + (size_t)pix * 3;
     double r = c[0], g = c[1], b = c[2];
     if (mask && mask[pix] > 0.0) {
@@ -1333,6 +1408,17 @@ RT_HD void finalize_item(const FrameView& F, const double* mask, int pix) {
         r = r * (1.0 - w) + F.ol_color[0] * w;
         g = g * (1.0 - w) + F.ol_color[1] * w;
         b = b * (1.0 - w) + F.ol_color[2] * w;
+    }
+    if (F.rad) {
+        F.rad[(size_t)pix * 3 + 0] = (float)r;
+        F.rad[(size_t)pix * 3 + 1] = (float)g;
+        F.rad[(size_t)pix * 3 + 2] = (float)b;
+    }
+    if (F.feat) {
+        float* fp = F.feat + (size_t)pix * kFeatCount;
+        fp[kFeatRad + 0] = (float)r;
+        fp[kFeatRad + 1] = (float)g;
+        fp[kFeatRad + 2] = (float)b;
     }
     F.rgb[(size_t)pix * 3 + 0] = to_u8_sat(clamp_r(r, 0.0, 1.0) * 255.0);
     F.rgb[(size_t)pix * 3 + 1] = to_u8_sat(clamp_r(g, 0.0, 1.0) * 255.0);

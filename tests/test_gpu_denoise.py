@@ -1,0 +1,156 @@
+"""GPU tests of the shadow denoiser: its BF16 MFMA forward pass and gradients against the
+numpy oracle (oracle/denoise_ref.py, bf16 rounding emulated at the device's storage
+points), training that lowers the loss, inference and the denoised-render pipeline with
+its tile fallback."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import denoise_ref as D
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+# Forward activations: relative L2 error per layer. Gradients: relative L2 error and
+# cosine per layer. The device accumulates in f32 where the oracle uses f64, so a stored
+# bf16 value occasionally rounds the other way; those flips compound through the layers.
+ACT_REL = 2e-2
+GRAD_REL = 5e-2
+GRAD_COS = 0.998
+
+
+@pytest.fixture(scope="module")
+def DN():
+    import torch
+
+    assert torch.cuda.is_available(), "GPU tests need a ROCm device"
+    from rtrace_amd import denoise
+
+    return denoise
+
+
+def _batch(seed, B, H, W):
+    rng = np.random.default_rng(seed)
+    feat = rng.random((B, H, W, D.N_FEAT)).astype(np.float32)
+    feat[..., 6:9] = feat[..., 6:9] * 2 - 1
+    feat[..., 9] *= 20
+    feat[..., 11] *= 0.25
+    target = np.clip(feat[..., 0:3] + 0.1 * rng.standard_normal((B, H, W, 3)), 0, 1).astype(np.float32)
+    return feat, target
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30))
+
+
+def test_parameter_layout_matches_oracle(DN):
+    assert DN.param_count() == D.N_PARAMS
+    for l in range(DN.N_LAYERS):
+        cin, cp, wo, bo = DN.layer_info(l)
+        assert (cin, cp) == (D.LAYERS[l][4], D.coutp(l)) and (wo, bo) == D.OFFS[l]
+
+
+@pytest.mark.parametrize("B,H,W", [(1, 16, 32), (2, 20, 44)])
+def test_forward_matches_oracle(DN, B, H, W):
+    import torch
+    dn = DN.Denoiser(seed=11)
+    p = dn.get_params()
+    p[:] += np.random.default_rng(1).standard_normal(p.size).astype(np.float32) * 1e-3  # nonzero biases
+    dn.set_params(p)
+    feat, target = _batch(2, B, H, W)
+    ft, tt = torch.from_numpy(feat).cuda(), torch.from_numpy(target).cuda()
+    dn.gradients(ft, tt)  # runs the forward pass
+    net = D.UNet(p, bf16_emulation=True)
+    net.forward(feat)
+    Hp, Wp = (H + 7) // 8 * 8, (W + 7) // 8 * 8
+    for l, L in enumerate(D.LAYERS):
+        lv, cout = L[0], L[5]
+        shape = (B, Hp >> lv, Wp >> lv, 3 if l == D.OUT else cout)
+        got = dn.activation(l, shape)
+        want = net.acts[l]
+        r = _rel(got, want)
+        print(f"layer {l}: rel {r:.2e}")
+        assert r < ACT_REL, (l, r)
+
+
+def test_gradients_match_oracle(DN):
+    import torch
+    dn = DN.Denoiser(seed=12)
+    p = dn.get_params()
+    feat, target = _batch(3, 2, 24, 40)
+    loss, g = dn.gradients(torch.from_numpy(feat).cuda(), torch.from_numpy(target).cuda())
+    rloss, rg = D.UNet(p, bf16_emulation=True).loss_and_grads(feat, target)
+    assert abs(loss - rloss) <= 1e-3 * rloss
+    for l in range(len(D.LAYERS)):
+        wo, bo = D.OFFS[l]
+        for name, sl in (("w", slice(wo, bo)), ("b", slice(bo, bo + D.coutp(l)))):
+            a, b = g[sl].astype(np.float64), rg[sl]
+            if np.linalg.norm(b) == 0:
+                assert np.linalg.norm(a) == 0
+                continue
+            r = _rel(a, b)
+            cos = float(a @ b / (np.linalg.norm(a) * np.linalg.norm(b)))
+            print(f"layer {l} {name}: rel {r:.2e} cos {cos:.6f}")
+            assert r < GRAD_REL and cos > GRAD_COS, (l, name, r, cos)
+
+
+def test_training_lowers_the_loss(DN):
+    import torch
+    dn = DN.Denoiser(seed=13)
+    feat, target = _batch(4, 4, 32, 32)
+    ft, tt = torch.from_numpy(feat).cuda(), torch.from_numpy(target).cuda()
+    losses = [dn.train_step(ft, tt, 1e-3) for _ in range(60)]
+    assert np.isfinite(losses).all()
+    assert losses[-1] < 0.5 * losses[0], losses[::10]
+
+
+def test_denoise_is_noisy_plus_network_output(DN):
+    import torch
+    dn = DN.Denoiser(seed=14)
+    feat, target = _batch(5, 1, 36, 52)
+    ft = torch.from_numpy(feat[0]).cuda()
+    out = torch.zeros((36, 52, 3), dtype=torch.float32, device="cuda")
+    u8 = torch.zeros((36, 52, 3), dtype=torch.uint8, device="cuda")
+    dn.denoise(ft, out, u8)
+    want = D.UNet(dn.get_params(), True).output(feat)[0]
+    assert _rel(out.cpu().numpy(), want) < 1e-2
+    o = out.cpu().numpy()
+    assert np.array_equal(u8.cpu().numpy(), (np.clip(o, 0, 1) * np.float32(255)).astype(np.uint8))
+
+
+@pytest.fixture(scope="module")
+def square_scene(tmp_path_factory):
+    from rtrace_amd import scenes
+    return scenes.config2(str(tmp_path_factory.mktemp("dn")), subdiv=4, light="square", light_size=2.0,
+                          light_samples=4)
+
+
+def test_render_denoised_tile_fallback(DN, square_scene):
+    """threshold < 0 re-renders every tile: the frame is the many-sample render, exactly;
+    threshold > 1 re-renders none: the frame is the network's output on the few-sample
+    features."""
+    import torch
+    import rtrace_amd as R
+    W, H = 160, 90
+    scene = R.Scene.from_json_file(square_scene)
+    r = R.Renderer(W, H)
+    r.max_depth = 4
+    ref = torch.zeros(H * W * 3, dtype=torch.uint8, device="cuda")
+    r.light_samples = 64
+    r.render_device(scene, ref)
+    dn = DN.Denoiser(seed=15)
+    r.light_samples = 4
+    out = torch.zeros_like(ref)
+    st = dn.render(r, scene, out, fallback_samples=64, threshold=-1.0, tile=16)
+    assert st.tiles_fallback == st.tiles == 10 * 6
+    assert torch.equal(out, ref)
+    out2 = torch.zeros_like(ref)
+    st2 = dn.render(r, scene, out2, fallback_samples=64, threshold=2.0, tile=16)
+    assert st2.tiles_fallback == 0
+    feat = torch.zeros((H, W, R.FEAT_COUNT), dtype=torch.float32, device="cuda")
+    low = torch.zeros_like(ref)
+    r.render_device_aux(scene, low, features=feat)
+    want = torch.zeros_like(ref)
+    dn.denoise(feat, None, want)
+    assert torch.equal(out2, want)

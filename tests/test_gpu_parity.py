@@ -304,3 +304,63 @@ def test_object_bvh_exact_ties(R, tmp_path, with_cube):
         assert mism == 0 and int(d.max()) == 0
         for k in ("rays_primary", "rays_reflect", "rays_shadow"):
             assert getattr(st, k) == ost[k], k
+
+
+@pytest.mark.parametrize("aa,spp", [("quincunx", 1), ("stochastic", 2), ("no-jitter", 1)])
+def test_features_and_radiance_vs_oracle(R, config_dir, aa, spp):
+    """The per-pixel feature channels and pre-quantisation radiance are bit-identical to
+    the oracle's (the same f64 values rounded to f32)."""
+    import torch
+    from rtrace_amd import scenes
+    path = scenes.config2(os.path.dirname(config_dir["config2_s4"]), subdiv=4, light="square", light_size=2.0,
+                          light_samples=4)
+    W, H = 128, 72
+    opts = O.make_opts(W, H, aa=aa, samples=spp, max_depth=4)
+    r = R.Renderer(W, H)
+    r.max_depth, r.samples, r.anti_aliasing_mode = 4, spp, opts.aa_mode
+    out = torch.zeros(H * W * 3, dtype=torch.uint8, device="cuda")
+    feat = torch.full((H, W, R.FEAT_COUNT), 7.0, dtype=torch.float32, device="cuda")
+    rad = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    r.render_device_aux(R.Scene.from_json_file(path), out, features=feat, radiance=rad)
+    ref, rfeat, rrad, _ = O.render_aux(path, opts)
+    assert np.array_equal(out.cpu().numpy().reshape(H, W, 3), ref)
+    assert np.array_equal(rad.cpu().numpy(), rrad)
+    f = feat.cpu().numpy()
+    for c in range(R.FEAT_COUNT):
+        assert np.array_equal(f[..., c], rfeat[..., c]), f"feature channel {c}"
+    assert 0 < np.mean((f[..., 10] > 0) & (f[..., 10] < 1)) < 0.5  # a penumbra exists
+
+
+@pytest.mark.parametrize("aa", ["quincunx", "stochastic"])
+def test_tile_masked_rerender_composites_exactly(R, config_dir, aa):
+    """Re-rendering a set of tiles at 64 light samples over a 4-sample frame leaves the
+    other tiles untouched and makes the masked tiles equal to a full 64-sample frame."""
+    import torch
+    from rtrace_amd import scenes
+    path = scenes.config2(os.path.dirname(config_dir["config2_s4"]), subdiv=4, light="square", light_size=2.0,
+                          light_samples=4)
+    W, H, T = 150, 90, 16
+    scene = R.Scene.from_json_file(path)
+    r = R.Renderer(W, H)
+    r.max_depth, r.samples = 4, 2 if aa == "stochastic" else 1
+    r.anti_aliasing_mode = O.AA_BY_NAME[aa]
+    full = torch.zeros(H * W * 3, dtype=torch.uint8, device="cuda")
+    r.light_samples = 64
+    st_full = r.render_device(scene, full)
+    low = torch.zeros_like(full)
+    r.light_samples = 4
+    r.render_device(scene, low)
+    tx, ty = (W + T - 1) // T, (H + T - 1) // T
+    rng = np.random.default_rng(5)
+    mask = (rng.random((ty, tx)) < 0.3).astype(np.uint8)
+    mask[0, 0] = mask[-1, -1] = 1
+    out = low.clone()
+    r.light_samples = 64
+    st = r.render_device_aux(scene, out, tile_mask=torch.from_numpy(mask).cuda(), mask_tile=T)
+    o = out.cpu().numpy().reshape(H, W, 3)
+    f = full.cpu().numpy().reshape(H, W, 3)
+    lo = low.cpu().numpy().reshape(H, W, 3)
+    pm = np.kron(mask, np.ones((T, T), np.uint8))[:H, :W].astype(bool)
+    assert np.array_equal(o[pm], f[pm])
+    assert np.array_equal(o[~pm], lo[~pm])
+    assert st.rays_primary < st_full.rays_primary

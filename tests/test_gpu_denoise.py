@@ -98,11 +98,14 @@ def test_gradients_match_oracle(DN):
 def test_training_lowers_the_loss(DN):
     import torch
     dn = DN.Denoiser(seed=13)
-    feat, target = _batch(4, 4, 32, 32)
+    feat, _ = _batch(4, 4, 32, 32)
+    # a correction the features determine (the same network in torch float64 with Adam
+    # drops this loss about 5x in 100 steps)
+    target = (feat[..., 0:3] + 0.3 * (feat[..., 3:6] - 0.5) * feat[..., 10:11]).astype(np.float32)
     ft, tt = torch.from_numpy(feat).cuda(), torch.from_numpy(target).cuda()
-    losses = [dn.train_step(ft, tt, 1e-3) for _ in range(60)]
+    losses = [dn.train_step(ft, tt, 2e-3) for _ in range(100)]
     assert np.isfinite(losses).all()
-    assert losses[-1] < 0.5 * losses[0], losses[::10]
+    assert losses[-1] < 0.35 * losses[0], losses[::10]
 
 
 def test_denoise_is_noisy_plus_network_output(DN):

@@ -136,3 +136,59 @@ class Denoiser:
         _check(_bind().rt_render_denoised(_context(), scene._h, ctypes.byref(o), self._h, fallback_samples, threshold,
                                           tile, _ptr(out_u8), ctypes.byref(st)))
         return st
+
+
+def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.02, tile=16, subdiv=6, repeats=3,
+             work_dir=None, log=None):
+    """Config-5 frames: per seed, the plain render with `high` light samples (the judge),
+    the plain render with `low`, and the denoised render (low + network + tile fallback at
+    `high`). Returns per-image PSNR / SSIM against the judge and the frame times (ms, median
+    of `repeats` timed runs after one untimed)."""
+    import tempfile
+    import time
+
+    import torch
+
+    import rtrace_amd as R
+    from . import scenes
+    from .metrics import psnr, ssim
+
+    work_dir = work_dir or tempfile.mkdtemp(prefix="rt_dn_eval_")
+    r = R.Renderer(width, height)
+    r.max_depth = 4
+    out = torch.empty(height * width * 3, dtype=torch.uint8, device="cuda")
+    rows = []
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(repeats):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            res = fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts)), res
+
+    for s in seeds:
+        scene = R.Scene.from_json_file(scenes.config5(work_dir, s, subdiv=subdiv))
+        r.light_samples = high
+        ms_plain, st_plain = timed(lambda: r.render_device(scene, out))
+        ref = out.cpu().numpy().reshape(height, width, 3).copy()
+        r.light_samples = low
+        ms_low, _ = timed(lambda: r.render_device(scene, out))
+        noisy = out.cpu().numpy().reshape(height, width, 3).copy()
+        ms_dn, st_dn = timed(lambda: dn.render(r, scene, out, fallback_samples=high, threshold=threshold, tile=tile))
+        den = out.cpu().numpy().reshape(height, width, 3).copy()
+        row = {"seed": s, "ms_plain_high": round(ms_plain, 3), "ms_plain_low": round(ms_low, 3),
+               "ms_denoised": round(ms_dn, 3), "speedup": round(ms_plain / ms_dn, 2),
+               "psnr_low": round(psnr(noisy, ref), 2), "psnr_denoised": round(psnr(den, ref), 2),
+               "ssim_low": round(ssim(noisy, ref), 5), "ssim_denoised": round(ssim(den, ref), 5),
+               "tiles_fallback": st_dn.tiles_fallback, "tiles": st_dn.tiles,
+               "stage_ms": {"render": round(st_dn.ms_render, 3), "denoise": round(st_dn.ms_denoise, 3),
+                            "fallback": round(st_dn.ms_fallback, 3)},
+               "rays_plain_high": int(st_plain.rays_total)}
+        rows.append(row)
+        if log:
+            log(str(row))
+    return rows

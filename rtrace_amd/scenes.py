@@ -10,6 +10,8 @@ path consume identical vertices.
             perspective 50 deg, disk area light (16 samples), fog 0.05, depth 4.
   config3   1000 reflective spheres on a ring over a grid plane, 4 point lights,
             stochastic 4 spp, depth 10, outline on.
+  config5   seeded config-2-style scenes with a 2x2 square light: the denoiser's
+            training data (low / high light-sample pairs) and benchmark frames.
 """
 import json
 import math
@@ -62,9 +64,13 @@ def write_stl(path, tris):
         f.write(rec.tobytes())
 
 
-def icosphere(subdiv, radius, noise, seed):
-    """Icosahedron subdivided `subdiv` times (20 * 4^subdiv faces), vertices pushed to
-    radius * (1 + U(-noise, noise)) per vertex (shared vertices get one draw)."""
+_UNIT_ICO = {}
+
+
+def _unit_icosphere(subdiv):
+    """Unit-sphere vertices and faces of the icosahedron subdivided `subdiv` times."""
+    if subdiv in _UNIT_ICO:
+        return _UNIT_ICO[subdiv]
     t = (1.0 + math.sqrt(5.0)) / 2.0
     verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
              (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
@@ -88,12 +94,17 @@ def icosphere(subdiv, radius, noise, seed):
             ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
             nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
         faces = nf
+    _UNIT_ICO[subdiv] = (np.array(verts), np.array(faces, dtype=np.int64))
+    return _UNIT_ICO[subdiv]
+
+
+def icosphere(subdiv, radius, noise, seed):
+    """Icosahedron subdivided `subdiv` times (20 * 4^subdiv faces), vertices pushed to
+    radius * (1 + U(-noise, noise)) per vertex (shared vertices get one draw)."""
+    V, F = _unit_icosphere(subdiv)
     rng = SplitMix64(seed)
-    V = np.array(verts)
     scale = np.array([radius * (1.0 + rng.uniform(-noise, noise)) for _ in range(len(V))])
-    V = V * scale[:, None]
-    F = np.array(faces, dtype=np.int64)
-    return V[F]
+    return (V * scale[:, None])[F]
 
 
 def _camera_persp(pos, target, fov, aspect=(16, 9)):
@@ -196,6 +207,52 @@ def config3(out_dir, n_spheres=1000):
     return _write(out_dir, f"config3_n{n_spheres}", sc)
 
 
+def config5(out_dir, seed, subdiv=None):
+    """Config-2-style scene with a 2x2 square light (the denoiser's data and benchmark):
+    a noisy icosphere (subdivision 3-5 by seed unless given), 3 transformed cubes, 0-3
+    spheres, a grid plane, random light position, fog on half of the seeds, camera on a
+    random side. Render with quincunx, depth 4; the light's sample count is set per render."""
+    os.makedirs(out_dir, exist_ok=True)
+    rng = SplitMix64(0x5EED0000 + seed)
+    sd = 3 + seed % 3 if subdiv is None else subdiv
+    radius = rng.uniform(1.2, 1.8)
+    mesh_name = f"config5_{seed}_s{sd}.stl"
+    write_stl(os.path.join(out_dir, mesh_name), icosphere(sd, radius, rng.uniform(0.02, 0.08), seed=seed + 17))
+    rc = lambda lo=0.3: hex_color((rng.uniform(lo, 1), rng.uniform(lo, 1), rng.uniform(lo, 1)))  # noqa: E731
+    objects = [{"kind": "mesh", "filename": mesh_name,
+                "material": material(rc(), 0.1, rng.uniform(0.5, 0.8), rng.uniform(0.1, 0.5),
+                                     int(rng.uniform(8, 64)), reflectivity=rng.uniform(0.0, 0.3))}]
+    for i in range(3):
+        ang = 2 * math.pi * i / 3 + rng.uniform(-0.5, 0.5)
+        r = rng.uniform(2.4, 3.6)
+        tf = [f"rotate({rng.uniform(0, 90):.4f}, {rng.uniform(0, 90):.4f}, {rng.uniform(0, 90):.4f})",
+              f"scale({rng.uniform(0.6, 1.4):.4f}, {rng.uniform(0.6, 1.4):.4f}, {rng.uniform(0.6, 1.4):.4f})",
+              f"translate({r * math.cos(ang):.4f}, {r * math.sin(ang):.4f}, {rng.uniform(-1.0, 0.4):.4f})"]
+        objects.append({"kind": "cube", "center": [0, 0, 0], "size": [1, 1, 1], "transform": tf,
+                        "material": material(rc(), 0.1, 0.7, 0.3, 32, reflectivity=rng.uniform(0.0, 0.4))})
+    for _ in range(int(rng.uniform(0, 4))):
+        ang, r = rng.uniform(0, 2 * math.pi), rng.uniform(1.8, 4.0)
+        objects.append({"kind": "sphere", "center": [r * math.cos(ang), r * math.sin(ang), rng.uniform(-1.3, 0.5)],
+                        "radius": rng.uniform(0.2, 0.6), "material": material(rc(), 0.1, 0.7, 0.4, 32)})
+    plane_z = rng.uniform(-2.0, -1.5)
+    objects.append({"kind": "plane", "point": [0, 0, plane_z], "normal": [0, 0, 1],
+                    "material": material(rc(0.6), 0.1, 0.7, 0.1, 8, reflectivity=rng.uniform(0.0, 0.3),
+                                         texture={"type": "grid", "line_color": "#404040",
+                                                  "line_width": rng.uniform(0.02, 0.06),
+                                                  "cell_size": rng.uniform(0.6, 1.5)})})
+    az, el, dist = rng.uniform(0, 2 * math.pi), rng.uniform(0.6, 1.2), rng.uniform(6.0, 9.0)
+    light = {"position": [dist * math.cos(el) * math.cos(az), dist * math.cos(el) * math.sin(az), dist * math.sin(el)],
+             "color": "#FFFFFF", "intensity": rng.uniform(0.8, 1.2), "shape": "square", "size": 2.0, "samples": 16}
+    cam_az, cam_d = rng.uniform(0, 2 * math.pi), rng.uniform(7.0, 9.0)
+    cam = _camera_persp([cam_d * math.cos(cam_az), cam_d * math.sin(cam_az), rng.uniform(2.5, 5.0)], [0, 0, -0.3],
+                        50)
+    fog = None
+    if rng.uniform(0, 1) < 0.5:
+        fog = {"color": rc(0.5), "density": rng.uniform(0.02, 0.08), "start": 2.0, "end": 20.0}
+    sc = {"camera": cam, "objects": objects, "lights": [light], "scene_settings": _settings(rc(0.0), fog=fog)}
+    return _write(out_dir, f"config5_{seed}_s{sd}", sc)
+
+
 def _write(out_dir, name, scene):
     os.makedirs(out_dir, exist_ok=True)
     path = os.path.join(out_dir, name + ".json")
@@ -212,4 +269,5 @@ RENDER = {
     "config2_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
     "config2_square_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
     "config3": dict(width=1920, height=1080, aa="stochastic", samples=4, max_depth=10),
+    "config5": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
 }

@@ -1,0 +1,114 @@
+"""Train the shadow denoiser on the GPU from seeded random scenes.
+
+    python -m rtrace_amd.train_denoiser --scenes 2000 --steps 4000 --out rtrace_amd/data/denoiser.npy
+
+Data: config-5 scenes (rtrace_amd.scenes.config5, seeds 0..N-1) rendered by the plain
+method twice at a training resolution: `--low` light samples with the per-pixel features
+(the network's input) and `--high` light samples (the target radiance). All pairs stay
+resident in device memory. Steps draw random crops; the loss and the Adam update run in the
+library's own kernels (rt_denoiser_train_step). Progress goes to stdout.
+"""
+import argparse
+import math
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+
+def render_pairs(seeds, W, H, low, high, work_dir, log):
+    import torch
+
+    import rtrace_amd as R
+    from rtrace_amd import scenes
+    n = len(seeds)
+    feats = torch.empty((n, H, W, R.FEAT_COUNT), dtype=torch.float32, device="cuda")
+    targets = torch.empty((n, H, W, 3), dtype=torch.float32, device="cuda")
+    out = torch.empty(H * W * 3, dtype=torch.uint8, device="cuda")
+    r = R.Renderer(W, H)
+    r.max_depth = 4
+    t0 = time.time()
+    for i, s in enumerate(seeds):
+        path = scenes.config5(work_dir, s)
+        scene = R.Scene.from_json_file(path)
+        r.light_samples = low
+        r.render_device_aux(scene, out, features=feats[i])
+        r.light_samples = high
+        r.render_device_aux(scene, out, radiance=targets[i])
+        del scene
+        os.remove(path)
+        for f in os.listdir(work_dir):
+            if f.startswith(f"config5_{s}_"):
+                os.remove(os.path.join(work_dir, f))
+        if (i + 1) % 100 == 0 or i + 1 == n:
+            log(f"rendered {i + 1}/{n} pairs ({time.time() - t0:.1f} s)")
+    return feats, targets
+
+
+def crops(feats, targets, batch, size, gen):
+    import torch
+    n, H, W, _ = feats.shape
+    idx = torch.randint(0, n, (batch,), generator=gen)
+    ys = torch.randint(0, H - size + 1, (batch,), generator=gen)
+    xs = torch.randint(0, W - size + 1, (batch,), generator=gen)
+    f = torch.stack([feats[i, y:y + size, x:x + size] for i, y, x in zip(idx.tolist(), ys.tolist(), xs.tolist())])
+    t = torch.stack([targets[i, y:y + size, x:x + size] for i, y, x in zip(idx.tolist(), ys.tolist(), xs.tolist())])
+    return f.contiguous(), t.contiguous()
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--scenes", type=int, default=2000)
+    ap.add_argument("--size", default="320x180", help="training render resolution WxH")
+    ap.add_argument("--low", type=int, default=4, help="light samples of the network input")
+    ap.add_argument("--high", type=int, default=256, help="light samples of the target")
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--crop", type=int, default=128)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--time-limit", type=float, default=0, help="stop training after this many seconds")
+    args = ap.parse_args(argv)
+
+    import torch
+
+    from rtrace_amd.denoise import Denoiser
+
+    def log(msg):
+        print(msg, flush=True)
+
+    W, H = (int(v) for v in args.size.split("x"))
+    work = tempfile.mkdtemp(prefix="rt_train_")
+    t0 = time.time()
+    feats, targets = render_pairs(list(range(args.scenes)), W, H, args.low, args.high, work, log)
+    log(f"data: {args.scenes} pairs {W}x{H}, {args.low} vs {args.high} light samples, {time.time() - t0:.1f} s")
+    dn = Denoiser(seed=args.seed)
+    if args.init:
+        dn.set_params(np.load(args.init, allow_pickle=False))
+    gen = torch.Generator().manual_seed(args.seed)
+    t1 = time.time()
+    ema = None
+    for step in range(args.steps):
+        lr = args.lr * 0.5 * (1 + math.cos(math.pi * step / args.steps)) + 1e-6
+        f, t = crops(feats, targets, args.batch, args.crop, gen)
+        loss = dn.train_step(f, t, lr)
+        if not math.isfinite(loss):
+            log(f"step {step}: loss is {loss}, stopping")
+            return 1
+        ema = loss if ema is None else 0.98 * ema + 0.02 * loss
+        if step % 100 == 0 or step + 1 == args.steps:
+            log(f"step {step} loss {loss:.3e} ema {ema:.3e} lr {lr:.2e} ({time.time() - t1:.1f} s)")
+        if args.time_limit and time.time() - t1 > args.time_limit:
+            log(f"time limit after {step + 1} steps")
+            break
+    dn.save(args.out)
+    log(f"saved {args.out}")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,4 +1,5 @@
-"""Benchmark: Mrays/s of the per-pixel trace path at 1080p on one MI355X.
+"""Benchmark: Mrays/s of the per-pixel trace path at 1080p on one MI355X, and the
+denoised-vs-plain frame time of configuration 5.
 
 Workload (default `config2_1080p`): the procedural mesh scene of configuration 2
 (81,920-triangle noisy icosphere with a host-built k-d tree, 3 transformed cubes, a
@@ -8,6 +9,9 @@ full frame (camera rays, closest-hit, shading, shadow rays, reflections, resolve
 quantise) rendered into a device buffer; the scene is resident in HBM before timing.
 
 Rays = primary + reflection + shadow rays, counted exactly by the pipeline.
+The "denoise" object: configuration-5 frames at 1080p rendered plain with 256 light
+samples (the judge) and denoised (4 samples + the trained U-Net + tile fallback), with
+PSNR / SSIM per image and the denoiser's BF16 MFMA roofline.
 Prints one JSON line.
 """
 import argparse
@@ -85,6 +89,41 @@ def cpu_baseline(path, cfg, budget_s=20.0):
 
 COPY_BYTES = 1 << 30
 
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+# Denoiser multiply-accumulates per full-resolution pixel: sum over layers of
+# 9 * cin * cout / 4^level (13 real input features, 3 outputs; layer table in
+# include/rtrace_denoise.h)
+DN_LAYERS = [(13, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 128, 2), (128, 128, 2), (128, 128, 3),
+             (128, 128, 3), (256, 128, 2), (128, 128, 2), (192, 64, 1), (64, 64, 1), (96, 32, 0), (32, 32, 0),
+             (32, 3, 0)]
+DN_MAC_PER_PX = sum(9 * ci * co / 4 ** lv for ci, co, lv in DN_LAYERS)
+
+
+def denoise_section(seeds, threshold, low=4, high=256):
+    """Config 5 at 1080p: the plain render with `high` light samples (the judge) against
+    the denoised render (`low` samples + network + tile fallback), per image."""
+    import rtrace_amd as R  # noqa: F401 - loads the library
+    from rtrace_amd import denoise
+    dn = denoise.Denoiser.load()
+    rows = denoise.evaluate(dn, seeds, 1920, 1080, low=low, high=high, threshold=threshold, repeats=3)
+    ms_dn_net = sorted(r["stage_ms"]["denoise"] for r in rows)[len(rows) // 2]
+    flops = 2.0 * DN_MAC_PER_PX * 1920 * 1080
+    ach = flops / (ms_dn_net * 1e-3) / 1e12
+    plain = sum(r["ms_plain_high"] for r in rows) / len(rows)
+    den = sum(r["ms_denoised"] for r in rows) / len(rows)
+    return {
+        "workload": "config5 (config-2-style scenes, 2x2 square light), 1920x1080 quincunx, depth 4",
+        "low_samples": low, "high_samples": high, "threshold": threshold, "tile": 16,
+        "ms_per_frame_plain": round(plain, 3), "ms_per_frame_denoised": round(den, 3),
+        "speedup": round(plain / den, 2),
+        "psnr_min": min(r["psnr_denoised"] for r in rows), "psnr_low_min": min(r["psnr_low"] for r in rows),
+        "per_image": rows,
+        "roofline": {"bound": "mfma", "kernel": "denoiser (prep, 15 conv3x3, pools, compose, tile test)",
+                     "achieved": round(ach, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / BF16_PEAK_TFLOPS, 4), "flop_per_frame": flops,
+                     "ms_per_frame": ms_dn_net},
+    }
+
 
 def traffic_child(workload):
     """Profiled child: one frame of the workload, then a plain copy of COPY_BYTES (the
@@ -149,6 +188,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-denoise", action="store_true", help="skip the config-5 denoised-vs-plain section")
+    ap.add_argument("--dn-seeds", default="100001,100002", help="config-5 scene seeds of the denoise section")
+    ap.add_argument("--dn-threshold", type=float, default=0.06, help="tile fallback threshold")
     args = ap.parse_args()
     if args.traffic_child:
         traffic_child(args.workload)
@@ -230,6 +272,8 @@ def main():
                 f"(KiB counters scaled by a {COPY_BYTES >> 20} MiB copy)")
         else:
             line["roofline"]["traffic_note"] = err
+    if not args.no_denoise:
+        line["denoise"] = denoise_section([int(v) for v in args.dn_seeds.split(",")], args.dn_threshold)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(path, cfg)
     print(json.dumps(line))

@@ -128,7 +128,7 @@ class Denoiser:
         _check(_bind().rt_denoise(self._h, _ptr(feat), H, W, _ptr(out_f32), _ptr(out_u8), ctypes.byref(ms)))
         return ms.value
 
-    def render(self, renderer, scene, out_u8, fallback_samples=256, threshold=0.02, tile=16):
+    def render(self, renderer, scene, out_u8, fallback_samples=256, threshold=0.06, tile=16):
         """Denoised frame of `scene` with `renderer`'s settings (its light_samples is the low
         count) into out_u8 (device, H*W*3); returns DenoiseStats."""
         o = renderer._opts()
@@ -138,7 +138,7 @@ class Denoiser:
         return st
 
 
-def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.02, tile=16, subdiv=6, repeats=3,
+def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06, tile=16, subdiv=6, repeats=3,
              work_dir=None, log=None):
     """Config-5 frames: per seed, the plain render with `high` light samples (the judge),
     the plain render with `low`, and the denoised render (low + network + tile fallback at

@@ -242,6 +242,62 @@ int rt_render_device_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opt
  * object index (-1 miss), triangle index (-1 non-mesh), t. n = capacity. */
 int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, size_t n);
 
+/* ---- automatic cameras ---- */
+/* AutoCamera::generate_cameras (src/auto_camera.rs:9-130): left, front, top (ortho) and a
+ * 50-degree perspective view of the scene's finite bounds (Scene::compute_finite_bounds,
+ * src/scene.rs:538-660; planes excluded) with a 15% margin, written to out[0..3] in that
+ * order. RT_E_SCENE "Scene has no finite objects to compute bounds" when nothing is finite. */
+int rt_scene_auto_cameras(rt_scene* scene, rt_camera_desc out[4]);
+/* The same from a Scene JSON file, on the host only (no device needed; errors through
+ * rt_last_error(NULL)), as the reference's auto_camera CLI (cli/src/auto_camera_cli.rs). */
+int rt_auto_cameras_file(const char* scene_path, rt_camera_desc out[4]);
+
+/* ---- batched rendering (extension; no reference counterpart) ----
+ * Many frames (scenes x camera views x resolutions) rendered by one sequence of persistent
+ * kernels, captured once into a hipGraph and replayed: each stage kernel takes chunks of
+ * work from all jobs through one atomic counter, so jobs of uneven cost balance on the
+ * device. Each job's image is the one rt_render_device gives for the same scene, camera
+ * and options. */
+typedef struct {
+    rt_scene* scene;
+    const rt_camera_desc* camera; /* NULL = the scene's camera */
+    rt_render_opts opts;          /* size, anti-aliasing, samples, depth (>= 1), seed, light_samples;
+                                     no outline pass, no count_work / profile */
+    void* d_rgb_out;              /* device, width*height*3 u8 */
+} rt_batch_job;
+
+typedef struct {
+    int32_t tile;           /* raygen tile edge in pixels: 4, 8 or 16 */
+    int32_t chunk;          /* items a workgroup takes at a time, a multiple of 128 */
+    int32_t blocks_per_cu;  /* persistent workgroups per compute unit */
+    int32_t use_graph;      /* 1 = capture the launch sequence into a hipGraph and replay it */
+    double max_gib;         /* refuse batches whose buffers need more device memory (GiB) */
+} rt_batch_opts;
+
+typedef struct {
+    double ms;              /* device time of the last run (hipEvents around the graph launch) */
+    double ms_build;        /* host time of create: layout, allocation, upload, capture */
+    int32_t n_jobs;
+    int32_t graph_nodes;    /* kernels in the captured graph (0 without a graph) */
+    int64_t bytes;          /* device memory of the frame buffers */
+    uint64_t pixels;
+    uint64_t rays_primary, rays_reflect, rays_shadow;
+} rt_batch_stats;
+
+typedef struct rt_batch rt_batch;
+
+void rt_batch_opts_default(rt_batch_opts* bo);
+/* Lays out and allocates every job's buffers and captures the launch graph. */
+int rt_batch_create(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, const rt_batch_opts* bo,
+                    rt_batch** out);
+/* Renders every job into its d_rgb_out; blocks until done. */
+int rt_batch_run(rt_batch* b, rt_batch_stats* stats);
+void rt_batch_destroy(rt_batch* b);
+/* Chooses tile, chunk and blocks_per_cu for these jobs by timing candidate settings
+ * (coordinate descent from *bo, which also supplies use_graph and max_gib); writes the
+ * fastest to *bo and its time to *best_ms. */
+int rt_batch_autotune(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, rt_batch_opts* bo, double* best_ms);
+
 /* ---- ray queries (the `World::hit` seam, src/ray.rs:699-711) ---- */
 /* Closest hit of n rays (origins/dirs: n*3 f64; dirs are normalized as Ray::new does). */
 int rt_intersect(rt_ctx* ctx, rt_scene* scene, const double* origins, const double* dirs, size_t n,

@@ -1893,6 +1893,116 @@ void set_err(char* err, int errlen, const std::string& m) {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// AutoCamera (src/auto_camera.rs:9-130) over Scene::compute_finite_bounds
+// (src/scene.rs:538-660), straight from the scene JSON.
+// ---------------------------------------------------------------------------
+struct CamOut {
+    bool persp;
+    V3 pos, target, up;
+    double width, height, fov;
+};
+void auto_cameras(const std::string& scene_path, CamOut out[4]) {
+    std::ifstream f(scene_path, std::ios::binary);
+    if (!f) throw Err(-3, std::string("No such file or directory (os error 2): ") + scene_path);
+    std::string text((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    JP jp{text.data(), text.data() + text.size()};
+    J root = jp.parse();
+    const J& objs = req(root, "objects");
+    std::string base = dir_of(scene_path);
+    bool any = false;
+    V3 mn{}, mx{};
+    auto merge = [&](V3 a, V3 b) {
+        if (!any) {
+            mn = a;
+            mx = b;
+            any = true;
+            return;
+        }
+        mn = v3(rmin(mn.x, a.x), rmin(mn.y, a.y), rmin(mn.z, a.z));
+        mx = v3(rmax(mx.x, b.x), rmax(mx.y, b.y), rmax(mx.z, b.z));
+    };
+    for (const J& o : objs.a) {
+        std::string kind = strv(req(o, "kind"), "kind");
+        const J* tj = o.get("transform");
+        std::vector<std::string> tl;
+        bool has_t = tj && tj->t == J::Arr;
+        if (has_t)
+            for (auto& s_ : tj->a) tl.push_back(strv(s_, "transform"));
+        M4 M;
+        bool tok = has_t && parse_transforms(tl, M);
+        if (kind == "sphere") {
+            double c[3];
+            arr3(req(o, "center"), c, "center");
+            V3 cp = v3(c[0], c[1], c[2]);
+            double r = num(req(o, "radius"), "radius");
+            if (tok) {
+                cp = m4_point(M, cp);
+                double sx = norm(v3(M.at(0, 0), M.at(1, 0), M.at(2, 0)));
+                double sy = norm(v3(M.at(0, 1), M.at(1, 1), M.at(2, 1)));
+                double sz = norm(v3(M.at(0, 2), M.at(1, 2), M.at(2, 2)));
+                r *= rmax(rmax(sx, sy), sz);
+            }
+            merge(cp - v3(r, r, r), cp + v3(r, r, r));
+        } else if (kind == "cube") {
+            double c[3], sz[3];
+            arr3(req(o, "center"), c, "center");
+            arr3(req(o, "size"), sz, "size");
+            V3 cp = v3(c[0], c[1], c[2]), h = v3(sz[0], sz[1], sz[2]) / 2.0;
+            M4 inv;
+            if (tok && !m4_inverse(M, inv)) inv = m4_identity();
+            if (!tok || m4_is_identity(inv)) {
+                merge(cp - h, cp + h);
+            } else {
+                V3 a = v3(INFINITY, INFINITY, INFINITY), b = v3(-INFINITY, -INFINITY, -INFINITY);
+                for (int k = 0; k < 8; ++k) {
+                    V3 corner = v3((k & 4) ? h.x : -h.x, (k & 2) ? h.y : -h.y, (k & 1) ? h.z : -h.z);
+                    V3 w = cp + m4_point(M, corner);
+                    a = v3(rmin(a.x, w.x), rmin(a.y, w.y), rmin(a.z, w.z));
+                    b = v3(rmax(b.x, w.x), rmax(b.y, w.y), rmax(b.z, w.z));
+                }
+                merge(a, b);
+            }
+        } else if (kind == "mesh") {
+            std::string fn = strv(req(o, "filename"), "filename");
+            std::string path = (fn.size() && fn[0] == '/') ? fn : base + "/" + fn;
+            std::vector<Tri> tris = load_stl(path);
+            V3 a = tris[0].v[0], b = tris[0].v[0];
+            for (const Tri& t : tris)
+                for (const V3& v : t.v) {
+                    a = v3(rmin(a.x, v.x), rmin(a.y, v.y), rmin(a.z, v.z));
+                    b = v3(rmax(b.x, v.x), rmax(b.y, v.y), rmax(b.z, v.z));
+                }
+            if (tok) {
+                V3 cs[8] = {v3(a.x, a.y, a.z), v3(a.x, a.y, b.z), v3(a.x, b.y, a.z), v3(a.x, b.y, b.z),
+                            v3(b.x, a.y, a.z), v3(b.x, a.y, b.z), v3(b.x, b.y, a.z), v3(b.x, b.y, b.z)};
+                V3 na = m4_point(M, cs[0]), nb = na;
+                for (int k = 1; k < 8; ++k) {
+                    V3 w = m4_point(M, cs[k]);
+                    na = v3(rmin(na.x, w.x), rmin(na.y, w.y), rmin(na.z, w.z));
+                    nb = v3(rmax(nb.x, w.x), rmax(nb.y, w.y), rmax(nb.z, w.z));
+                }
+                a = na;
+                b = nb;
+            }
+            merge(a, b);
+        }
+    }
+    if (!any) throw Err(-2, "Scene has no finite objects to compute bounds");
+    V3 center = v3((mn.x + mx.x) / 2.0, (mn.y + mx.y) / 2.0, (mn.z + mx.z) / 2.0);
+    V3 size = mx - mn;
+    double vw = rmax(rmax(size.x, size.y), size.z) * 1.15;
+    double d = vw * 2.0;
+    out[0] = {false, v3(center.x, center.y - d, center.z), center, v3(0, 0, 1), vw, vw, 0};
+    out[1] = {false, v3(center.x - d, center.y, center.z), center, v3(0, 0, 1), vw, vw, 0};
+    out[2] = {false, v3(center.x, center.y, center.z + d), center, v3(0, 1, 0), vw, vw, 0};
+    double dist = vw / std::tan(50.0 * (M_PI / 180.0) / 2.0);
+    double hd = dist * std::cos(35.0 * (M_PI / 180.0));
+    V3 pp = v3(center.x + hd * std::cos(45.0 * (M_PI / 180.0)), center.y + hd * std::sin(45.0 * (M_PI / 180.0)),
+               center.z + dist * std::sin(35.0 * (M_PI / 180.0)));
+    out[3] = {true, pp, center, v3(0, 0, 1), 1.0, 1.0, 50.0};
+}
+
 extern "C" {
 
 int oracle_render_aux(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_out, float* features,
@@ -1912,6 +2022,28 @@ int oracle_render(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_
                   int32_t* hit_tri, double* hit_t, oracle_stats* stats, char* err, int errlen) {
     try {
         return do_render(scene_path, opts, rgb_out, hit_obj, hit_tri, hit_t, stats);
+    } catch (const Err& e) {
+        set_err(err, errlen, e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_err(err, errlen, e.what());
+        return -99;
+    }
+}
+
+int oracle_auto_cameras(const char* scene_path, double* out, char* err, int errlen) {
+    try {
+        CamOut c[4];
+        auto_cameras(scene_path, c);
+        for (int k = 0; k < 4; ++k) {
+            double* o = out + 14 * k;
+            o[0] = c[k].persp ? 1 : 0;
+            o[1] = c[k].pos.x; o[2] = c[k].pos.y; o[3] = c[k].pos.z;
+            o[4] = c[k].target.x; o[5] = c[k].target.y; o[6] = c[k].target.z;
+            o[7] = c[k].up.x; o[8] = c[k].up.y; o[9] = c[k].up.z;
+            o[10] = c[k].width; o[11] = c[k].height; o[12] = c[k].fov; o[13] = 0;
+        }
+        return 0;
     } catch (const Err& e) {
         set_err(err, errlen, e.what());
         return e.code;

@@ -69,6 +69,11 @@ int oracle_render(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_
 int oracle_render_aux(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_out, float* features,
                       float* radiance, oracle_stats* stats, char* err, int errlen);
 
+/* AutoCamera::generate_cameras (src/auto_camera.rs:9-130) of a scene file: 4 cameras
+ * (left, front, top, perspective) x 14 doubles: kind (0 ortho, 1 perspective), position,
+ * target, up, width, height, fov (0 for ortho), 0. */
+int oracle_auto_cameras(const char* scene_path, double* out, char* err, int errlen);
+
 /* Unit-level hooks used by the tests (each restates one reference function). */
 void oracle_chacha_block(const uint32_t in[16], int double_rounds, uint32_t out[16]);
 void oracle_seed_key(uint64_t seed, uint32_t key_out[8]);

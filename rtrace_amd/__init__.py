@@ -49,12 +49,53 @@ class _Opts(ctypes.Structure):
     ]
 
 
+class CameraDesc(ctypes.Structure):
+    """rt_camera_desc: a Camera config (src/scene.rs:34-46)."""
+    _fields_ = [("kind", ctypes.c_int32), ("position", ctypes.c_double * 3), ("target", ctypes.c_double * 3),
+                ("up", ctypes.c_double * 3), ("width", ctypes.c_double), ("height", ctypes.c_double),
+                ("has_fov", ctypes.c_int32), ("fov", ctypes.c_double), ("has_grid", ctypes.c_int32),
+                ("grid_pitch", ctypes.c_double), ("grid_thickness", ctypes.c_double), ("grid_color", ctypes.c_double * 3)]
+
+    def to_json(self):
+        """The serde form of Camera (src/scene.rs:34-46), as AutoCameraResult::to_cameras_json writes it."""
+        return {"kind": "perspective" if self.kind == 1 else "ortho", "position": list(self.position),
+                "target": list(self.target), "up": list(self.up), "width": self.width, "height": self.height,
+                "fov": self.fov if self.has_fov else None, "grid_pitch": None, "grid_color": None,
+                "grid_thickness": None}
+
+
 class _Aux(ctypes.Structure):
     _fields_ = [("features", ctypes.c_void_p), ("radiance", ctypes.c_void_p), ("tile_mask", ctypes.c_void_p),
                 ("mask_tile", ctypes.c_int32)]
 
 
 FEAT_COUNT = 13  # rt_render_aux feature channels (include/rtrace_hip.h)
+
+
+class _BatchJob(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_void_p), ("camera", ctypes.c_void_p), ("opts", _Opts), ("d_rgb_out", ctypes.c_void_p)]
+
+
+class BatchOpts(ctypes.Structure):
+    """rt_batch_opts: raygen tile, chunk (items per take), persistent workgroups per CU, graph, memory cap."""
+    _fields_ = [("tile", ctypes.c_int32), ("chunk", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("use_graph", ctypes.c_int32), ("max_gib", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double), ("ms_build", ctypes.c_double), ("n_jobs", ctypes.c_int32),
+                ("graph_nodes", ctypes.c_int32), ("bytes", ctypes.c_int64), ("pixels", ctypes.c_uint64),
+                ("rays_primary", ctypes.c_uint64), ("rays_reflect", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @property
+    def rays_total(self):
+        return self.rays_primary + self.rays_reflect + self.rays_shadow
 
 STAGES = ["raygen", "closest", "shade", "shadow", "finish", "post"]
 KERNEL_CLOSEST, KERNEL_SHADOW = 0, 1
@@ -117,6 +158,14 @@ def lib():
         L.rt_last_primary_hits.argtypes = [P, P, P, P, ctypes.c_size_t]
         L.rt_intersect.argtypes = [P, P, P, P, ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                    P, P, P]
+        L.rt_scene_auto_cameras.argtypes = [P, P]
+        L.rt_auto_cameras_file.argtypes = [ctypes.c_char_p, P]
+        L.rt_batch_opts_default.argtypes = [ctypes.POINTER(BatchOpts)]
+        L.rt_batch_create.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(BatchOpts), ctypes.POINTER(P)]
+        L.rt_batch_run.argtypes = [P, ctypes.POINTER(BatchStats)]
+        L.rt_batch_run.restype = ctypes.c_int
+        L.rt_batch_destroy.argtypes = [P]
+        L.rt_batch_autotune.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(BatchOpts), ctypes.POINTER(ctypes.c_double)]
         _lib = L
     return _lib
 
@@ -140,14 +189,15 @@ def _check(rc):
 class Scene:
     """A scene uploaded to the device (Scene + World assembly, src/scene.rs / src/renderer.rs:137-317)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, path=None):
         self._h = handle
+        self._path = path  # the JSON file it came from, if any
 
     @classmethod
     def from_json_file(cls, path):
         h = ctypes.c_void_p()
         _check(lib().rt_scene_load_json_file(_context(), os.fspath(path).encode(), ctypes.byref(h)))
-        return cls(h)
+        return cls(h, os.fspath(path))
 
     @classmethod
     def from_json_str(cls, text, base_dir=None):
@@ -317,3 +367,93 @@ def cli_size(camera_width, camera_height, diagonal):
         return int(math.copysign(f + (1.0 if abs(x) - f >= 0.5 else 0.0), x))
 
     return rnd(w), rnd(h)
+
+
+# ---------------------------------------------------------------- auto cameras
+CAMERA_NAMES = ("left", "front", "top", "perspective")
+
+
+class AutoCameraResult:
+    """The four cameras of AutoCamera::generate_cameras (src/auto_camera.rs:9-130)."""
+
+    def __init__(self, cams):
+        self.left, self.front, self.top, self.perspective = cams
+
+    def cameras(self):
+        return [self.left, self.front, self.top, self.perspective]
+
+    def to_cameras_json(self):
+        return {n: c.to_json() for n, c in zip(CAMERA_NAMES, self.cameras())}
+
+
+class AutoCamera:
+    @staticmethod
+    def generate_cameras(scene):
+        """`scene`: a loaded Scene, or a Scene JSON path (host only, no device needed)."""
+        out = (CameraDesc * 4)()
+        if isinstance(scene, (str, os.PathLike)):
+            rc = lib().rt_auto_cameras_file(os.fspath(scene).encode(), out)
+            if rc != 0:
+                raise RtraceError(rc, lib().rt_last_error(None).decode())
+        else:
+            _check(lib().rt_scene_auto_cameras(scene._h, out))
+        return AutoCameraResult([out[i] for i in range(4)])
+
+
+# ---------------------------------------------------------------- batches
+class Batch:
+    """Many frames in one captured launch of persistent kernels (rt_batch_*).
+
+    jobs: list of (scene, camera or None, renderer, out) where `renderer` supplies the
+    Renderer fields of that frame and `out` is a device uint8 tensor of H*W*3."""
+
+    def __init__(self, jobs, opts=None):
+        self._jobs, self._keep = _make_jobs(jobs)
+        o = opts if opts is not None else default_batch_opts()
+        self.opts = o
+        self._h = ctypes.c_void_p()
+        _check(lib().rt_batch_create(_context(), self._jobs, len(jobs), ctypes.byref(o), ctypes.byref(self._h)))
+
+    def run(self):
+        st = BatchStats()
+        _check(lib().rt_batch_run(self._h, ctypes.byref(st)))
+        return st
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().rt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def default_batch_opts():
+    o = BatchOpts()
+    lib().rt_batch_opts_default(ctypes.byref(o))
+    return o
+
+
+def _make_jobs(jobs):
+    arr = (_BatchJob * len(jobs))()
+    keep = []
+    for i, (scene, cam, renderer, out) in enumerate(jobs):
+        arr[i].scene = scene._h
+        if cam is not None:
+            c = CameraDesc()
+            ctypes.memmove(ctypes.byref(c), ctypes.byref(cam), ctypes.sizeof(CameraDesc))
+            keep.append(c)
+            arr[i].camera = ctypes.addressof(c)
+        arr[i].opts = renderer._opts()
+        arr[i].d_rgb_out = out.data_ptr()
+    return arr, keep
+
+
+def autotune_batch(jobs, opts=None):
+    """rt_batch_autotune: the fastest (tile, chunk, blocks_per_cu) for these jobs and its time (ms)."""
+    arr, keep = _make_jobs(jobs)
+    o = opts if opts is not None else default_batch_opts()
+    ms = ctypes.c_double()
+    _check(lib().rt_batch_autotune(_context(), arr, len(jobs), ctypes.byref(o), ctypes.byref(ms)))
+    return o, ms.value
+

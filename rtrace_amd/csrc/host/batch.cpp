@@ -1,0 +1,294 @@
+// batch.cpp — batched rendering: many (scene, camera, resolution) frames in one captured
+// launch sequence of persistent kernels (kernels/batch.hip), and its autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../kernels/batch.hpp"
+#include "context.hpp"
+#include "frame.hpp"
+
+struct rt_batch {
+    rt_ctx* ctx = nullptr;
+    int n_jobs = 0;
+    rt_batch_opts bo{};
+    rt::DevBuf arena, views, frames, plan;
+    std::vector<rt::FrameView> hframes;  // [2 * n_jobs]
+    std::vector<rt::SceneView> hviews;
+    std::vector<rt::FrameLayout> layouts;
+    rt::BatchArgs args{};
+    rt::BatchLaunch launch{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    size_t graph_nodes = 0;
+    double ms_build = 0;
+    uint64_t pixels = 0;
+    ~rt_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace rt {
+namespace {
+
+int compute_units(int device) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
+void check_opts(const rt_batch_opts& bo) {
+    if (bo.tile != 4 && bo.tile != 8 && bo.tile != 16) throw Error(RT_E_INVALID_ARG, "tile must be 4, 8 or 16");
+    if (bo.chunk < kBatchBlock || bo.chunk % kBatchBlock != 0 || bo.chunk > (1 << 20))
+        throw Error(RT_E_INVALID_ARG, "chunk must be a positive multiple of 128");
+    if (bo.blocks_per_cu < 1 || bo.blocks_per_cu > 64) throw Error(RT_E_INVALID_ARG, "blocks_per_cu out of range");
+}
+
+std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, const rt_batch_opts& bo) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (n <= 0 || !jobs) throw Error(RT_E_INVALID_ARG, "a batch needs at least one job");
+    check_opts(bo);
+    auto b = std::make_unique<rt_batch>();
+    b->ctx = ctx;
+    b->n_jobs = n;
+    b->bo = bo;
+    b->hviews.resize(n);
+    b->hframes.resize(2 * (size_t)n);
+    b->layouts.resize(n);
+    std::vector<size_t> offs(n);
+    size_t total = 0;
+    int max_depth = 0;
+    for (int j = 0; j < n; ++j) {
+        const rt_batch_job& jb = jobs[j];
+        if (!jb.scene || !jb.scene->dev || !jb.d_rgb_out) throw Error(RT_E_INVALID_ARG, "job without scene or output");
+        if (jb.scene->ctx != ctx) throw Error(RT_E_INVALID_ARG, "job scene belongs to another context");
+        const rt_render_opts& o = jb.opts;
+        if (o.max_depth < 1) throw Error(RT_E_INVALID_ARG, "batched jobs need max_depth >= 1");
+        if (o.outline_enabled && o.aa_mode != RT_AA_QUINCUNX)
+            throw Error(RT_E_INVALID_ARG, "batched jobs do not take the outline pass");
+        if (o.count_work || o.profile) throw Error(RT_E_INVALID_ARG, "count_work and profile are single-frame options");
+        if (o.light_samples < 0 || o.light_samples > 65536) throw Error(RT_E_INVALID_ARG, "light_samples out of range");
+        if (o.width <= 0 || o.height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
+        const SceneDevice& D = *jb.scene->dev;
+        CameraDev cam = make_camera(jb.camera ? *jb.camera : D.host.camera, (double)o.width / (double)o.height);
+        b->hviews[j] = make_view(D, cam, o.light_samples);
+        FrameView& F = b->hframes[2 * j];
+        F = FrameView{};
+        b->layouts[j] = frame_fill(F, b->hviews[j], &o, bo.tile);
+        RayRec *q0 = nullptr, *q1 = nullptr;
+        offs[j] = total;
+        total += frame_carve(F, b->hviews[j], b->layouts[j], nullptr, &q0, &q1, false);
+        max_depth = std::max(max_depth, b->layouts[j].depth);
+        b->pixels += (uint64_t)o.width * o.height;
+    }
+    if ((double)total > bo.max_gib * 1073741824.0)
+        throw Error(RT_E_OOM, "batch needs " + std::to_string(total >> 20) + " MiB of frame buffers, over max_gib");
+    b->arena.reserve(total);
+    for (int j = 0; j < n; ++j) {
+        FrameView& F = b->hframes[2 * j];
+        RayRec *q0 = nullptr, *q1 = nullptr;
+        frame_carve(F, b->hviews[j], b->layouts[j], b->arena.as<char>() + offs[j], &q0, &q1, false);
+        F.rgb = static_cast<uint8_t*>(jobs[j].d_rgb_out);
+        F.rays = q0;
+        F.rays_next = q1;
+        FrameView& G = b->hframes[2 * j + 1];
+        G = F;
+        G.rays = q1;
+        G.rays_next = q0;
+    }
+    hipStream_t st = ctx_stream(ctx);
+    b->views.reserve(n * sizeof(SceneView));
+    b->frames.reserve(2 * (size_t)n * sizeof(FrameView));
+    b->plan.reserve((n + 2) * sizeof(int32_t));
+    HIP_TRY(hipMemcpyAsync(b->views.p, b->hviews.data(), n * sizeof(SceneView), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->frames.p, b->hframes.data(), 2 * (size_t)n * sizeof(FrameView), hipMemcpyHostToDevice,
+                           st));
+    b->args.views = b->views.as<SceneView>();
+    b->args.frames = b->frames.as<FrameView>();
+    b->args.base = b->plan.as<int32_t>();
+    b->args.counter = b->plan.as<int32_t>() + n + 1;
+    b->args.n_jobs = n;
+    b->args.chunk = bo.chunk;
+    b->launch.grid = compute_units(ctx_device(ctx)) * bo.blocks_per_cu;
+    b->launch.max_depth = max_depth;
+    HIP_TRY(hipStreamSynchronize(st));
+    if (bo.use_graph) {
+        HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t le = launch_batch(b->args, b->launch, st);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(st, &g);
+        b->graph = g;
+        HIP_TRY(le);
+        HIP_TRY(ce);
+        HIP_TRY(hipGraphGetNodes(b->graph, nullptr, &b->graph_nodes));
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    b->ms_build = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return b;
+}
+
+void run(rt_batch* b, rt_batch_stats* stats) {
+    rt_ctx* ctx = b->ctx;
+    hipStream_t st = ctx_stream(ctx);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    struct Ev {
+        hipEvent_t a, b;
+        ~Ev() {
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+        }
+    } ev{e0, e1};
+    HIP_TRY(hipEventRecord(e0, st));
+    if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, st));
+    else HIP_TRY(launch_batch(b->args, b->launch, st));
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    if (!stats) return;
+    memset(stats, 0, sizeof(*stats));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    stats->ms = ms;
+    stats->ms_build = b->ms_build;
+    stats->n_jobs = b->n_jobs;
+    stats->graph_nodes = (int32_t)b->graph_nodes;
+    stats->bytes = (int64_t)b->arena.bytes;
+    stats->pixels = b->pixels;
+    for (int j = 0; j < b->n_jobs; ++j) {
+        const FrameView& F = b->hframes[2 * j];
+        const FrameLayout& L = b->layouts[j];
+        std::vector<int32_t> counts(L.depth + 1), hits(L.depth);
+        HIP_TRY(hipMemcpy(counts.data(), F.counts, (L.depth + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(hits.data(), F.hit_count, L.depth * sizeof(int32_t), hipMemcpyDeviceToHost));
+        stats->rays_primary += (uint64_t)L.n_paths;
+        for (int l = 1; l < L.depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
+        const uint64_t k = b->hviews[j].n_lights > 0 ? (uint64_t)b->hviews[j].shadow_per_hit : 0;
+        for (int l = 0; l < L.depth; ++l) stats->rays_shadow += (uint64_t)hits[l] * k;
+    }
+}
+
+template <class F>
+int guard(rt_ctx* ctx, F fn) {
+    try {
+        fn();
+        return RT_OK;
+    } catch (const Error& e) {
+        return ctx_fail(ctx, e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return ctx_fail(ctx, RT_E_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return ctx_fail(ctx, RT_E_INTERNAL, e.what());
+    }
+}
+
+}  // namespace
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" {
+
+void rt_batch_opts_default(rt_batch_opts* bo) {
+    if (!bo) return;
+    bo->tile = 8;
+    bo->chunk = 1024;
+    bo->blocks_per_cu = 8;
+    bo->use_graph = 1;
+    bo->max_gib = 200.0;
+}
+
+int rt_batch_create(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, const rt_batch_opts* bo, rt_batch** out) {
+    if (!ctx || !out) return ctx_fail(ctx, RT_E_INVALID_ARG, "null argument");
+    *out = nullptr;
+    rt_batch_opts o;
+    rt_batch_opts_default(&o);
+    if (bo) o = *bo;
+    return guard(ctx, [&] { *out = build(ctx, jobs, n_jobs, o).release(); });
+}
+
+int rt_batch_run(rt_batch* b, rt_batch_stats* stats) {
+    if (!b) return ctx_fail(nullptr, RT_E_INVALID_ARG, "null batch");
+    return guard(b->ctx, [&] { run(b, stats); });
+}
+
+void rt_batch_destroy(rt_batch* b) {
+    if (!b) return;
+    (void)hipStreamSynchronize(ctx_stream(b->ctx));
+    delete b;
+}
+
+int rt_batch_autotune(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, rt_batch_opts* bo, double* best_ms) {
+    if (!ctx || !bo) return ctx_fail(ctx, RT_E_INVALID_ARG, "null argument");
+    return guard(ctx, [&] {
+        rt_batch_opts cur = *bo;
+        check_opts(cur);
+        // one timed replay after one untimed, best of two
+        auto measure = [&](const rt_batch_opts& o) {
+            std::unique_ptr<rt_batch> b = build(ctx, jobs, n_jobs, o);
+            rt_batch_stats s{};
+            run(b.get(), nullptr);
+            double best = 1e300;
+            for (int r = 0; r < 2; ++r) {
+                run(b.get(), &s);
+                best = std::min(best, s.ms);
+            }
+            return best;
+        };
+        double best = measure(cur);
+        const int chunks[] = {256, 512, 1024, 2048, 4096};
+        const int blocks[] = {2, 4, 8, 16};
+        const int tiles[] = {4, 8, 16};
+        for (int c : chunks) {
+            if (c == cur.chunk) continue;
+            rt_batch_opts t = cur;
+            t.chunk = c;
+            double ms = measure(t);
+            if (ms < best) {
+                best = ms;
+                cur = t;
+            }
+        }
+        for (int k : blocks) {
+            if (k == cur.blocks_per_cu) continue;
+            rt_batch_opts t = cur;
+            t.blocks_per_cu = k;
+            double ms = measure(t);
+            if (ms < best) {
+                best = ms;
+                cur = t;
+            }
+        }
+        for (int tl : tiles) {
+            if (tl == cur.tile) continue;
+            rt_batch_opts t = cur;
+            t.tile = tl;
+            double ms = measure(t);
+            if (ms < best) {
+                best = ms;
+                cur = t;
+            }
+        }
+        *bo = cur;
+        if (best_ms) *best_ms = best;
+    });
+}
+
+int rt_scene_auto_cameras(rt_scene* scene, rt_camera_desc out[4]) {
+    if (!scene || !scene->dev || !out) return ctx_fail(scene ? scene->ctx : nullptr, RT_E_INVALID_ARG, "null argument");
+    return guard(scene->ctx, [&] { auto_cameras(scene->dev->host, out); });
+}
+
+int rt_auto_cameras_file(const char* scene_path, rt_camera_desc out[4]) {
+    if (!scene_path || !out) return ctx_fail(nullptr, RT_E_INVALID_ARG, "null argument");
+    return guard(nullptr, [&] { auto_cameras(load_scene_file(scene_path), out); });
+}
+
+}  // extern "C"

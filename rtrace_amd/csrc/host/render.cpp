@@ -15,6 +15,7 @@
 #include "../common/layout.hpp"
 #include "../common/vec.hpp"
 #include "../kernels/launch.hpp"
+#include "frame.hpp"
 #include "kdbuild.hpp"
 #include "scene_host.hpp"
 
@@ -23,45 +24,13 @@ namespace {
 
 thread_local std::string g_tls_error;
 
-#define HIP_TRY(expr)                                                                             \
-    do {                                                                                          \
-        hipError_t e_ = (expr);                                                                   \
-        if (e_ != hipSuccess)                                                                     \
-            throw Error(e_ == hipErrorOutOfMemory ? RT_E_OOM : RT_E_DEVICE,                       \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                       \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    void reserve(size_t n) {
-        if (n <= bytes && p) return;
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-            bytes = 0;
-        }
-        size_t want = n ? n : 16;
-        HIP_TRY(hipMalloc(&p, want));
-        bytes = want;
-    }
-    template <class T>
-    T* as() const {
-        return static_cast<T*>(p);
-    }
-};
-
 template <class T>
 void upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
     b.reserve(v.size() * sizeof(T));
     if (!v.empty()) HIP_TRY(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
 }
+
+}  // namespace
 
 // Columns of the jittered grid for n square-light samples: the largest divisor of n that
 // is at most sqrt(n) (16 -> 4x4, 8 -> 2x4, 7 -> 1x7).
@@ -70,6 +39,8 @@ int grid_cols(int n) {
     while (c > 1 && n % c != 0) --c;
     return c < 1 ? 1 : c;
 }
+
+namespace {
 
 float round_down(double x) {
     float f = (float)x;
@@ -183,39 +154,7 @@ bool is_identity(const double* m) {
 
 }  // namespace
 
-struct SceneDevice {
-    HostScene host;
-    int device = 0;
-    DevBuf objs, meshes, mats, lights, obvh, obvh_items, others;
-    bool has_obvh = false, obvh_closest = false;
-    int n_others = 0;
-    std::vector<std::unique_ptr<DevBuf>> mesh_bufs;
-    int n_obj = 0, n_lights = 0;
-    double ms_upload = 0;
-    std::vector<FlatKd> kd;  // host copies (stats)
-};
-
-struct FrameBuffers {
-    DevBuf path_seed, lvl, prim_obj, prim_tri, prim_t, ol_dn, q0, q1, counts, hits, hitq, hit_count, tieq, tie_count,
-        shade, spt, vis, work, pix, rgb, px_dn, edge, edge2;
-};
-
 }  // namespace rt
-
-struct rt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    std::string err;
-    rt::FrameBuffers fb;
-    rt::StageTimer timer;
-    int last_paths = 0;
-};
-
-struct rt_scene {
-    std::unique_ptr<rt::SceneDevice> dev;
-    rt_ctx* ctx = nullptr;
-};
 
 namespace rt {
 namespace {
@@ -363,6 +302,8 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
 }
 
 // Camera::from_config (src/camera.rs:23-152) for this image's aspect ratio.
+}  // namespace
+
 CameraDev make_camera(const rt_camera_desc& c, double aspect) {
     d3 origin = mk(c.position[0], c.position[1], c.position[2]);
     d3 target = mk(c.target[0], c.target[1], c.target[2]);
@@ -456,6 +397,95 @@ SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_sample
     return S;
 }
 
+constexpr int kRaygenTile = 8;  // 8x8 pixel tile = one 64-lane wavefront
+
+// Scalar fields of the frame view for options `o` with raygen tiles of `tile` pixels;
+// validates the options and returns the queue sizes.
+FrameLayout frame_fill(FrameView& F, const SceneView& S, const rt_render_opts* o, int tile) {
+    if (o->samples == 0) throw Error(RT_E_INVALID_ARG, "Samples must be greater than 0");
+    if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
+    if (o->aa_mode < 0 || o->aa_mode > 2) throw Error(RT_E_INVALID_ARG, "unknown anti-aliasing mode");
+    if (tile != 4 && tile != 8 && tile != 16) throw Error(RT_E_INVALID_ARG, "raygen tile must be 4, 8 or 16");
+    const int W = o->width, H = o->height;
+    const bool quin = o->aa_mode == RT_AA_QUINCUNX;
+    F.W = W;
+    F.H = H;
+    F.N = quin ? 1 : o->samples;
+    F.aa = o->aa_mode;
+    F.use_kd = o->use_kdtree;
+    F.outline = (!quin && o->outline_enabled) ? 1 : 0;
+    F.grid_miss = quin ? 0 : 1;  // quincunx calls ray_color without the camera
+    F.count_work = o->count_work;
+    F.seed = o->has_seed ? o->seed : 0;
+    F.tile = tile;
+    F.tiles_x = (W + tile - 1) / tile;
+    const long long tiled = (long long)F.tiles_x * ((H + tile - 1) / tile) * tile * tile;
+    FrameLayout L;
+    if (quin) {
+        const long long tiled_c = (long long)((W + tile) / tile) * ((H + tile) / tile) * tile * tile;
+        L.n_paths = (long long)W * H + (long long)(W + 1) * (H + 1);
+        L.n_q0 = tiled + tiled_c;
+    } else {
+        L.n_paths = (long long)W * H * F.N;
+        L.n_q0 = tiled * F.N;
+    }
+    if (L.n_q0 > (1ll << 30)) throw Error(RT_E_INVALID_ARG, "image too large for one launch");
+    F.n_paths = (int)L.n_paths;
+    F.n_q0 = (int)L.n_q0;
+    L.depth = std::max(o->max_depth, 0);
+    F.max_depth = L.depth;
+    L.cap = (size_t)L.n_q0;
+    L.K = (size_t)S.shadow_per_hit;
+    if (L.cap * L.K >= (1ull << 31)) throw Error(RT_E_INVALID_ARG, "too many shadow samples per frame for one launch");
+    F.ol_wd = o->outline_depth_weight;
+    F.ol_wn = o->outline_normal_weight;
+    F.ol_thr = o->outline_threshold;
+    F.ol_thick = o->outline_thickness;
+    memcpy(F.ol_color, o->outline_color, sizeof(F.ol_color));
+    F.ol_8 = o->outline_use_8_neighbors;
+    return L;
+}
+
+// Places the frame's buffers in one allocation starting at `base` (256-byte aligned
+// pieces); with base == nullptr only measures. Returns the bytes needed.
+size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char* base, RayRec** q0, RayRec** q1,
+                   bool own_rgb) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> char* {
+        char* p = base ? base + off : nullptr;
+        off += (std::max(bytes, (size_t)16) + 255) & ~(size_t)255;
+        return p;
+    };
+    const size_t np = (size_t)L.n_paths, cap = L.cap, px = (size_t)F.W * F.H;
+    const size_t depth = (size_t)L.depth;
+    F.path_seed = (uint64_t*)take(np * sizeof(uint64_t));
+    F.lvl = (double*)take(std::max(depth, (size_t)1) * np * 4 * sizeof(double));
+    F.prim_obj = (int32_t*)take(np * sizeof(int32_t));
+    F.prim_tri = (int32_t*)take(np * sizeof(int32_t));
+    F.prim_t = (double*)take(np * sizeof(double));
+    F.ol_dn = (double*)take(F.outline ? np * 4 * sizeof(double) : 16);
+    *q0 = (RayRec*)take(cap * sizeof(RayRec));
+    *q1 = (RayRec*)take(cap * sizeof(RayRec));
+    F.counts = (int32_t*)take((depth + 2) * sizeof(int32_t));
+    F.hits = (HitRec*)take(cap * sizeof(HitRec));
+    F.hitq = (int32_t*)take(cap * sizeof(int32_t));
+    F.hit_count = (int32_t*)take((depth + 1) * sizeof(int32_t));
+    F.tieq = (int32_t*)take(cap * sizeof(int32_t));
+    F.tie_count = (int32_t*)take((depth + 1) * sizeof(int32_t));
+    F.shade = (ShadeRec*)take(cap * sizeof(ShadeRec));
+    F.spt = (double*)take(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
+    F.vis = (uint8_t*)take(cap * L.K);
+    F.work = (unsigned long long*)take(2 * kWorkCount * sizeof(unsigned long long));
+    F.pix = (double*)take(px * 3 * sizeof(double));
+    F.rgb = own_rgb ? (uint8_t*)take(px * 3) : nullptr;
+    F.px_dn = (double*)take(F.outline ? px * 4 * sizeof(double) : 16);
+    F.edge = (double*)take(F.outline ? px * sizeof(double) : 16);
+    F.edge2 = (double*)take(F.outline ? px * sizeof(double) : 16);
+    return off;
+}
+
+namespace {
+
 // Camera rays traced: every path, or with a tile mask the masked pixels' rays (quincunx:
 // their centres and every corner that touches one).
 uint64_t primary_rays(const FrameView& F, long long n_paths) {
@@ -482,96 +512,21 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
     SceneDevice& D = *scene->dev;
     const int W = o->width, H = o->height;
-    const bool quin = o->aa_mode == RT_AA_QUINCUNX;
     if (o->aa_mode < 0 || o->aa_mode > 2) throw Error(RT_E_INVALID_ARG, "unknown anti-aliasing mode");
     CameraDev cam = make_camera(D.host.camera, (double)W / (double)H);
     if (o->light_samples < 0 || o->light_samples > 65536) throw Error(RT_E_INVALID_ARG, "light_samples out of range");
     SceneView S = make_view(D, cam, o->light_samples);
-    FrameBuffers& B = ctx->fb;
     hipStream_t st = ctx->stream;
 
     FrameView F{};
-    F.W = W;
-    F.H = H;
-    F.N = quin ? 1 : o->samples;
-    F.aa = o->aa_mode;
-    F.max_depth = o->max_depth;
-    F.use_kd = o->use_kdtree;
-    F.outline = (!quin && o->outline_enabled) ? 1 : 0;
-    F.grid_miss = quin ? 0 : 1;  // quincunx calls ray_color without the camera
-    F.count_work = o->count_work;
-    F.seed = o->has_seed ? o->seed : 0;
-    F.tiles_x = (W + kTile - 1) / kTile;
-    long long tiled = (long long)F.tiles_x * ((H + kTile - 1) / kTile) * kTile * kTile;
-    long long n_paths, n_q0;
-    if (quin) {
-        long long tiled_c = (long long)((W + 1 + kTile - 1) / kTile) * ((H + 1 + kTile - 1) / kTile) * kTile * kTile;
-        n_paths = (long long)W * H + (long long)(W + 1) * (H + 1);
-        n_q0 = tiled + tiled_c;
-    } else {
-        n_paths = (long long)W * H * F.N;
-        n_q0 = tiled * F.N;
-    }
-    if (n_q0 > (1ll << 30)) throw Error(RT_E_INVALID_ARG, "image too large for one launch");
-    F.n_paths = (int)n_paths;
-    F.n_q0 = (int)n_q0;
-    const int depth = std::max(o->max_depth, 0);
-    const size_t cap = (size_t)n_q0;
-    const size_t K = (size_t)S.shadow_per_hit;
-
-    B.path_seed.reserve(n_paths * sizeof(uint64_t));
-    B.lvl.reserve((size_t)std::max(depth, 1) * n_paths * 4 * sizeof(double));
-    B.prim_obj.reserve(n_paths * sizeof(int32_t));
-    B.prim_tri.reserve(n_paths * sizeof(int32_t));
-    B.prim_t.reserve(n_paths * sizeof(double));
-    B.ol_dn.reserve(F.outline ? n_paths * 4 * sizeof(double) : 16);
-    B.q0.reserve(cap * sizeof(RayRec));
-    B.q1.reserve(cap * sizeof(RayRec));
-    B.counts.reserve((depth + 2) * sizeof(int32_t));
-    B.hits.reserve(cap * sizeof(HitRec));
-    B.hitq.reserve(cap * sizeof(int32_t));
-    B.hit_count.reserve((depth + 1) * sizeof(int32_t));
-    B.tieq.reserve(cap * sizeof(int32_t));
-    B.tie_count.reserve((depth + 1) * sizeof(int32_t));
-    B.shade.reserve(cap * sizeof(ShadeRec));
-    if (cap * K >= (1ull << 31)) throw Error(RT_E_INVALID_ARG, "too many shadow samples per frame for one launch");
-    B.spt.reserve(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
-    B.vis.reserve(cap * K);
-    B.work.reserve(2 * kWorkCount * sizeof(unsigned long long));
-    B.pix.reserve((size_t)W * H * 3 * sizeof(double));
-    B.rgb.reserve((size_t)W * H * 3);
-    B.px_dn.reserve(F.outline ? (size_t)W * H * 4 * sizeof(double) : 16);
-    B.edge.reserve(F.outline ? (size_t)W * H * sizeof(double) : 16);
-    B.edge2.reserve(F.outline ? (size_t)W * H * sizeof(double) : 16);
-
-    F.path_seed = B.path_seed.as<uint64_t>();
-    F.lvl = B.lvl.as<double>();
-    F.prim_obj = B.prim_obj.as<int32_t>();
-    F.prim_tri = B.prim_tri.as<int32_t>();
-    F.prim_t = B.prim_t.as<double>();
-    F.ol_dn = B.ol_dn.as<double>();
-    F.counts = B.counts.as<int32_t>();
-    F.hits = B.hits.as<HitRec>();
-    F.hitq = B.hitq.as<int32_t>();
-    F.hit_count = B.hit_count.as<int32_t>();
-    F.tieq = B.tieq.as<int32_t>();
-    F.tie_count = B.tie_count.as<int32_t>();
-    F.shade = B.shade.as<ShadeRec>();
-    F.spt = B.spt.as<double>();
-    F.vis = B.vis.as<uint8_t>();
-    F.work = B.work.as<unsigned long long>();
-    F.pix = B.pix.as<double>();
-    F.rgb = dev_rgb ? static_cast<uint8_t*>(dev_rgb) : B.rgb.as<uint8_t>();
-    F.px_dn = B.px_dn.as<double>();
-    F.edge = B.edge.as<double>();
-    F.edge2 = B.edge2.as<double>();
-    F.ol_wd = o->outline_depth_weight;
-    F.ol_wn = o->outline_normal_weight;
-    F.ol_thr = o->outline_threshold;
-    F.ol_thick = o->outline_thickness;
-    memcpy(F.ol_color, o->outline_color, sizeof(F.ol_color));
-    F.ol_8 = o->outline_use_8_neighbors;
-    F.max_depth = depth;
+    const FrameLayout L = frame_fill(F, S, o, kRaygenTile);
+    const int depth = L.depth;
+    const long long n_paths = L.n_paths;
+    RayRec *q0 = nullptr, *q1 = nullptr;
+    const size_t bytes = frame_carve(F, S, L, nullptr, &q0, &q1, dev_rgb == nullptr);
+    ctx->fb.arena.reserve(bytes);
+    frame_carve(F, S, L, ctx->fb.arena.as<char>(), &q0, &q1, dev_rgb == nullptr);
+    if (dev_rgb) F.rgb = static_cast<uint8_t*>(dev_rgb);
     if (aux) {
         F.feat = aux->features;
         F.rad = aux->radiance;
@@ -593,7 +548,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     }
     ctx->timer.reset();
     HIP_TRY(hipEventRecord(ctx->ev0, st));
-    HIP_TRY(launch_frame(S, F, B.q0.as<RayRec>(), B.q1.as<RayRec>(), st, o->profile ? &ctx->timer : nullptr));
+    HIP_TRY(launch_frame(S, F, q0, q1, st, o->profile ? &ctx->timer : nullptr));
     HIP_TRY(hipEventRecord(ctx->ev1, st));
     if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> counts(depth + 2, 0), hit_count(depth + 1, 0);
@@ -604,6 +559,9 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     HIP_TRY(hipMemcpyAsync(work, F.work, sizeof(work), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     ctx->last_paths = (int)n_paths;
+    ctx->last_prim_obj = F.prim_obj;
+    ctx->last_prim_tri = F.prim_tri;
+    ctx->last_prim_t = F.prim_t;
     if (stats) {
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -806,9 +764,9 @@ int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, siz
     if (!ctx) return fail(nullptr, RT_E_INVALID_ARG, "null context");
     return guarded(ctx, [&] {
         size_t m = std::min(n, (size_t)ctx->last_paths);
-        if (obj) HIP_TRY(hipMemcpy(obj, ctx->fb.prim_obj.p, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-        if (tri) HIP_TRY(hipMemcpy(tri, ctx->fb.prim_tri.p, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-        if (t) HIP_TRY(hipMemcpy(t, ctx->fb.prim_t.p, m * sizeof(double), hipMemcpyDeviceToHost));
+        if (obj) HIP_TRY(hipMemcpy(obj, ctx->last_prim_obj, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (tri) HIP_TRY(hipMemcpy(tri, ctx->last_prim_tri, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (t) HIP_TRY(hipMemcpy(t, ctx->last_prim_t, m * sizeof(double), hipMemcpyDeviceToHost));
     });
 }
 

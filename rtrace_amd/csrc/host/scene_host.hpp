@@ -16,6 +16,11 @@ struct Error : std::runtime_error {
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 
+struct ObjBounds {
+    bool finite = false;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+};
+
 struct HostScene {
     rt_camera_desc camera{};
     std::vector<rt_object_desc> objects;
@@ -33,12 +38,19 @@ struct HostScene {
     bool outline_color_ok = true;
     rt_render_opts outline{};
 
+    // Scene::compute_finite_bounds per object (src/scene.rs:538-660): planes are not finite;
+    // a transformed mesh contributes its untransformed box's 8 corners, transformed
+    std::vector<ObjBounds> bounds;
+
     rt_scene_desc desc(std::vector<rt_mesh_desc>& mesh_storage) const;
 };
 
 HostScene load_scene_json(const std::string& text, const std::string& base_dir);
 HostScene load_scene_file(const std::string& path);
 HostScene from_desc(const rt_scene_desc& d);
+// AutoCamera::generate_cameras (src/auto_camera.rs:9-130) from HostScene::bounds:
+// left, front, top, perspective.
+void auto_cameras(const HostScene& S, rt_camera_desc out[4]);
 
 // STL loader (src/mesh.rs:751-1003): returns 9 f64 per triangle.
 std::vector<double> load_stl_file(const std::string& path);

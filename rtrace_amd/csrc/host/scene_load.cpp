@@ -382,6 +382,68 @@ std::vector<double> load_stl_file(const std::string& path) {
     return out;
 }
 
+// Cube::bounds (src/ray.rs:228-262): the plain box when untransformed, else the box of
+// center + M (corner, 1) over the 8 corners.
+void cube_bounds(const rt_object_desc& d, ObjBounds& ob) {
+    bool ident = true;
+    for (int i = 0; i < 16; ++i)
+        if (d.world_to_local[i] != ((i % 5 == 0) ? 1.0 : 0.0)) ident = false;
+    ob.finite = true;
+    if (ident) {
+        for (int i = 0; i < 3; ++i) {
+            ob.lo[i] = d.center[i] - d.half_size[i];
+            ob.hi[i] = d.center[i] + d.half_size[i];
+        }
+        return;
+    }
+    for (int i = 0; i < 3; ++i) {
+        ob.lo[i] = INFINITY;
+        ob.hi[i] = -INFINITY;
+    }
+    const double* h = d.half_size;
+    const double cs[8][3] = {{-h[0], -h[1], -h[2]}, {-h[0], -h[1], h[2]}, {-h[0], h[1], -h[2]}, {-h[0], h[1], h[2]},
+                             {h[0], -h[1], -h[2]},  {h[0], -h[1], h[2]},  {h[0], h[1], -h[2]},  {h[0], h[1], h[2]}};
+    for (const auto& c : cs) {
+        d3 w = m4_apply(d.local_to_world, mk(c[0], c[1], c[2]), 1.0);
+        const double wc[3] = {d.center[0] + w.x, d.center[1] + w.y, d.center[2] + w.z};
+        for (int i = 0; i < 3; ++i) {
+            ob.lo[i] = fmin_r(ob.lo[i], wc[i]);
+            ob.hi[i] = fmax_r(ob.hi[i], wc[i]);
+        }
+    }
+}
+
+// Mesh::bounds (src/mesh.rs:26-36) of the loaded vertices; with a transform, the box of
+// its 8 corners (in the order of src/scene.rs:612-621) through M (src/scene.rs:599-648).
+void mesh_bounds(const std::vector<double>& v, const double* M, ObjBounds& ob) {
+    if (v.empty()) return;  // an empty STL is rejected by the loader before this
+    for (int i = 0; i < 3; ++i) ob.lo[i] = ob.hi[i] = v[i];
+    for (size_t k = 3; k < v.size(); k += 3)
+        for (int i = 0; i < 3; ++i) {
+            ob.lo[i] = fmin_r(ob.lo[i], v[k + i]);
+            ob.hi[i] = fmax_r(ob.hi[i], v[k + i]);
+        }
+    ob.finite = true;
+    if (!M) return;
+    const double* a = ob.lo;
+    const double* b = ob.hi;
+    const double cs[8][3] = {{a[0], a[1], a[2]}, {a[0], a[1], b[2]}, {a[0], b[1], a[2]}, {a[0], b[1], b[2]},
+                             {b[0], a[1], a[2]}, {b[0], a[1], b[2]}, {b[0], b[1], a[2]}, {b[0], b[1], b[2]}};
+    double lo[3], hi[3];
+    for (int k = 0; k < 8; ++k) {
+        d3 w = m4_apply(M, mk(cs[k][0], cs[k][1], cs[k][2]), 1.0);
+        const double wc[3] = {w.x, w.y, w.z};
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = k == 0 ? wc[i] : fmin_r(lo[i], wc[i]);
+            hi[i] = k == 0 ? wc[i] : fmax_r(hi[i], wc[i]);
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        ob.lo[i] = lo[i];
+        ob.hi[i] = hi[i];
+    }
+}
+
 HostScene load_scene_json(const std::string& text, const std::string& base_dir) {
     Value root;
     try {
@@ -437,6 +499,7 @@ HostScene load_scene_json(const std::string& text, const std::string& base_dir) 
             have_m = parse_transform_list(list, M);  // invalid strings are ignored (src/renderer.rs:153-155)
         }
         rt_object_desc d{};
+        ObjBounds ob;
         d.material = (int32_t)idx;
         d.mesh = -1;
         ident4(d.world_to_local);
@@ -451,6 +514,11 @@ HostScene load_scene_json(const std::string& text, const std::string& base_dir) 
                 double s0 = length(mk(M[0], M[1], M[2])), s1 = length(mk(M[4], M[5], M[6])), s2 = length(mk(M[8], M[9], M[10]));
                 d.radius *= fmax_r(fmax_r(s0, s1), s2);
             }
+            for (int i = 0; i < 3; ++i) {  // center -/+ (r, r, r)
+                ob.lo[i] = d.center[i] - d.radius;
+                ob.hi[i] = d.center[i] + d.radius;
+            }
+            ob.finite = true;
         } else if (k == "plane") {
             d.kind = RT_OBJ_PLANE;
             as_vec3(field(o, "point"), "point", d.point);
@@ -479,11 +547,13 @@ HostScene load_scene_json(const std::string& text, const std::string& base_dir) 
                 if (!invert4(M, d.world_to_local)) ident4(d.world_to_local);
                 memcpy(d.local_to_world, M, sizeof(M));
             }
+            cube_bounds(d, ob);
         } else if (k == "mesh") {
             d.kind = RT_OBJ_MESH;
             std::string fn = as_str(field(o, "filename"), "filename");
             std::string path = (!fn.empty() && fn[0] == '/') ? fn : base_dir + "/" + fn;
             std::vector<double> v = load_stl_file(path);
+            mesh_bounds(v, have_m ? M : nullptr, ob);
             if (have_m) {
                 for (size_t i = 0; i < v.size(); i += 3) {
                     d3 p = m4_apply(M, mk(v[i], v[i + 1], v[i + 2]), 1.0);
@@ -496,6 +566,7 @@ HostScene load_scene_json(const std::string& text, const std::string& base_dir) 
             throw Error(RT_E_SCENE, "unknown variant `" + k + "`, expected one of `sphere`, `plane`, `cube`, `mesh`");
         }
         S.objects.push_back(d);
+        S.bounds.push_back(ob);
         mats.push_back(std::move(mat));
     }
     for (auto& m : mats) S.materials.push_back(lower_material(m, true));
@@ -627,6 +698,21 @@ HostScene from_desc(const rt_scene_desc& d) {
             throw Error(RT_E_INVALID_ARG, "mesh without vertices");
         S.meshes.emplace_back(d.meshes[i].vertices, d.meshes[i].vertices + (size_t)d.meshes[i].n_tris * 9);
     }
+    for (const auto& o : S.objects) {
+        ObjBounds ob;
+        if (o.kind == RT_OBJ_SPHERE) {
+            for (int i = 0; i < 3; ++i) {
+                ob.lo[i] = o.center[i] - o.radius;
+                ob.hi[i] = o.center[i] + o.radius;
+            }
+            ob.finite = true;
+        } else if (o.kind == RT_OBJ_CUBE) {
+            cube_bounds(o, ob);
+        } else if (o.kind == RT_OBJ_MESH && o.mesh >= 0 && o.mesh < d.n_meshes) {
+            mesh_bounds(S.meshes[o.mesh], nullptr, ob);  // vertices arrive transformed
+        }
+        S.bounds.push_back(ob);
+    }
     for (auto& o : S.objects) {
         if (o.material < 0 || o.material >= d.n_materials) throw Error(RT_E_INVALID_ARG, "object material out of range");
         if (o.kind == RT_OBJ_MESH && (o.mesh < 0 || o.mesh >= d.n_meshes)) throw Error(RT_E_INVALID_ARG, "object mesh out of range");
@@ -644,6 +730,60 @@ HostScene from_desc(const rt_scene_desc& d) {
     S.fog_end = d.fog_end;
     memcpy(S.background, d.background, sizeof(S.background));
     return S;
+}
+
+void auto_cameras(const HostScene& S, rt_camera_desc out[4]) {
+    bool any = false;
+    double lo[3], hi[3];
+    for (const ObjBounds& b : S.bounds) {
+        if (!b.finite) continue;
+        for (int i = 0; i < 3; ++i) {
+            lo[i] = any ? fmin_r(lo[i], b.lo[i]) : b.lo[i];
+            hi[i] = any ? fmax_r(hi[i], b.hi[i]) : b.hi[i];
+        }
+        any = true;
+    }
+    if (!any) throw Error(RT_E_SCENE, "Scene has no finite objects to compute bounds");
+    const double c[3] = {(lo[0] + hi[0]) / 2.0, (lo[1] + hi[1]) / 2.0, (lo[2] + hi[2]) / 2.0};
+    const double size[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    const double margin = 1.15;
+    const double vw = fmax_r(fmax_r(size[0], size[1]), size[2]) * margin;  // square viewports
+    const double dist = vw * 2.0;
+    auto ortho = [&](rt_camera_desc& cam, double px, double py, double pz, double ux, double uy, double uz) {
+        cam = rt_camera_desc{};
+        cam.kind = 0;
+        cam.position[0] = px;
+        cam.position[1] = py;
+        cam.position[2] = pz;
+        for (int i = 0; i < 3; ++i) cam.target[i] = c[i];
+        cam.up[0] = ux;
+        cam.up[1] = uy;
+        cam.up[2] = uz;
+        cam.width = vw;
+        cam.height = vw;
+    };
+    ortho(out[0], c[0], c[1] - dist, c[2], 0, 0, 1);  // left: from -y
+    ortho(out[1], c[0] - dist, c[1], c[2], 0, 0, 1);  // front: from -x
+    ortho(out[2], c[0], c[1], c[2] + dist, 0, 1, 0);  // top: from +z, y forward
+    // perspective (src/auto_camera.rs:97-130): 50 degree fov, 35 degrees down, 45 degrees in x-y
+    const double fov = 50.0, down = 35.0;
+    const double max_dim = fmax_r(fmax_r(size[0], size[1]), size[2]) * margin;
+    const double fov_rad = fov * (3.141592653589793 / 180.0);
+    const double d_fov = max_dim / std::tan(fov_rad / 2.0);
+    const double xy = 45.0 * (3.141592653589793 / 180.0), dr = down * (3.141592653589793 / 180.0);
+    const double horiz = d_fov * std::cos(dr);
+    rt_camera_desc& p = out[3];
+    p = rt_camera_desc{};
+    p.kind = 1;
+    p.position[0] = c[0] + horiz * std::cos(xy);
+    p.position[1] = c[1] + horiz * std::sin(xy);
+    p.position[2] = c[2] + d_fov * std::sin(dr);
+    for (int i = 0; i < 3; ++i) p.target[i] = c[i];
+    p.up[2] = 1.0;
+    p.width = 1.0;
+    p.height = 1.0;
+    p.has_fov = 1;
+    p.fov = fov;
 }
 
 }  // namespace rt

@@ -1,0 +1,209 @@
+// batch.hip — persistent kernels that run the trace pipeline of many frames (scenes,
+// camera views, resolutions) together.
+//
+// Every stage of the single-frame pipeline (raygen, closest, resolve, shade, shadow,
+// finish, pixel, finalize) becomes one launch for the whole batch. A plan kernel reads
+// each job's device-side item count for the stage and lays the jobs' chunks end to end;
+// a persistent grid then takes chunks from one atomic counter. A chunk never spans two
+// jobs, so the scene and frame views a workgroup reads are uniform (scalar loads), and
+// jobs of very different cost share the machine without a host round trip. The item
+// functions are the single-frame ones (trace_core.hpp), so a batched frame is the same
+// image as a frame rendered alone.
+#include <hip/hip_runtime.h>
+
+#include "batch.hpp"
+#include "trace_core.hpp"
+#include "wave_ops.hpp"
+
+namespace rt {
+namespace {
+
+enum Stage : int {
+    kStRaygen = 0,
+    kStClosest,
+    kStResolve,
+    kStShadePrep,
+    kStShadow,
+    kStFinish,
+    kStPixel,
+    kStFinalize
+};
+
+__device__ __forceinline__ long long stage_items(int stage, const SceneView& S, const FrameView& F, int level) {
+    switch (stage) {
+        case kStRaygen: return F.n_q0;
+        case kStPixel:
+        case kStFinalize: return (long long)F.W * F.H;
+        default: break;
+    }
+    if (level >= F.max_depth) return 0;  // this job has no such level
+    switch (stage) {
+        case kStClosest: return F.counts[level];
+        case kStResolve: return F.tie_count[level];
+        case kStShadePrep:
+        case kStFinish: return F.hit_count[level];
+        case kStShadow: return S.n_lights > 0 ? (long long)F.hit_count[level] * S.shadow_per_hit : 0;
+        default: return 0;
+    }
+}
+
+// One workgroup: chunk counts of every job, exclusive scan into base[], total in
+// base[n_jobs], counter reset.
+__global__ __launch_bounds__(1024) void k_plan(BatchArgs a, int stage, int level) {
+    __shared__ int32_t s_part[1024];
+    const int per = (a.n_jobs + 1023) / 1024;
+    const int j0 = (int)threadIdx.x * per, j1 = min(j0 + per, a.n_jobs);
+    int32_t local = 0;
+    for (int j = j0; j < j1; ++j) {
+        const long long n = stage_items(stage, a.views[j], a.frames[2 * j], level);
+        local += (int32_t)((n + a.chunk - 1) / a.chunk);
+    }
+    s_part[threadIdx.x] = local;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int32_t v = (int)threadIdx.x >= off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    int32_t run = s_part[threadIdx.x] - local;
+    for (int j = j0; j < j1; ++j) {
+        a.base[j] = run;
+        const long long n = stage_items(stage, a.views[j], a.frames[2 * j], level);
+        run += (int32_t)((n + a.chunk - 1) / a.chunk);
+    }
+    if (threadIdx.x == 1023) a.base[a.n_jobs] = s_part[1023];
+    if (threadIdx.x == 0) *a.counter = 0;
+}
+
+__global__ void k_batch_init(BatchArgs a) {
+    const FrameView& F = a.frames[2 * blockIdx.x];
+    for (int i = threadIdx.x; i <= F.max_depth; i += blockDim.x) {
+        F.counts[i] = i == 0 ? F.n_q0 : 0;
+        if (i < F.max_depth) {
+            F.hit_count[i] = 0;
+            F.tie_count[i] = 0;
+        }
+    }
+}
+
+// The scene and frame views reach the stage kernels as separate read-only, non-aliased
+// kernel arguments so that their wave-uniform fields load through the scalar cache.
+#define BATCH_PARAMS                                                                                        \
+    const SceneView *__restrict__ views, const FrameView *__restrict__ frames, const int32_t *__restrict__ base, \
+        int32_t *counter, int n_jobs, int chunk, int level
+
+// Takes the next chunk for this workgroup: false when none is left, else its job and the
+// job's first item of the chunk (both wave-uniform).
+__device__ __forceinline__ bool next_chunk(const int32_t* __restrict__ base, int32_t* counter, int n_jobs, int chunk,
+                                           int32_t* s_chunk, int& job, long long& first) {
+    if (threadIdx.x == 0) *s_chunk = atomicAdd(counter, 1);
+    __syncthreads();
+    const int c = __builtin_amdgcn_readfirstlane(*s_chunk);
+    __syncthreads();
+    if (c >= base[n_jobs]) return false;
+    int lo = 0, hi = n_jobs;  // last job whose first chunk is <= c
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (base[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    job = lo;
+    first = (long long)(c - base[lo]) * chunk;
+    return true;
+}
+
+template <int kStage>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_trace(BATCH_PARAMS) {
+    __shared__ uint32_t s_node[kMaxKdDepth * kBatchBlock];
+    __shared__ float s_t0[kMaxKdDepth * kBatchBlock];
+    __shared__ int32_t s_chunk;
+    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBatchBlock};
+    int j;
+    long long i0;
+    while (next_chunk(base, counter, n_jobs, chunk, &s_chunk, j, i0)) {
+        const SceneView& S = views[j];
+        const FrameView& F = frames[2 * j + (level & 1)];
+        const long long n = stage_items(kStage, S, F, level);
+        for (int k = 0; k < chunk; k += kBatchBlock) {
+            const long long i = i0 + k + threadIdx.x;
+            const bool live = i < n;
+            if (kStage == kStClosest) {
+                Work w;
+                bool hit = false, tie = false;
+                if (live) hit = closest_item<false>(S, F, (int)i, level, stk, w, &tie);
+                const int pos = wave_append(&F.hit_count[level], hit);
+                if (hit) F.hitq[pos] = (int)i;
+                const int tpos = wave_append(&F.tie_count[level], tie);
+                if (tie) F.tieq[tpos] = (int)i;
+            } else if (kStage == kStResolve) {
+                if (live) resolve_item(S, F, F.tieq[i], level, stk);
+            } else {  // kStShadow
+                Work w;
+                if (live) shadow_item<false>(S, F, (int)i, stk, w);
+            }
+        }
+    }
+}
+
+template <int kStage>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_plain(BATCH_PARAMS) {
+    __shared__ int32_t s_chunk;
+    int j;
+    long long i0;
+    while (next_chunk(base, counter, n_jobs, chunk, &s_chunk, j, i0)) {
+        const SceneView& S = views[j];
+        const FrameView& F = frames[2 * j + (level & 1)];
+        const long long n = stage_items(kStage, S, F, level);
+        for (int k = 0; k < chunk; k += kBatchBlock) {
+            const long long i = i0 + k + threadIdx.x;
+            const bool live = i < n;
+            if (kStage == kStRaygen) {
+                if (live) raygen_item(S, F, (int)i);
+            } else if (kStage == kStShadePrep) {
+                RayRec refl;
+                bool spawn = false;
+                if (live) spawn = shade_prepare_item(S, F, (int)i, level, &refl);
+                const int pos = wave_append(&F.counts[level + 1], spawn);
+                if (spawn) F.rays_next[pos] = refl;
+            } else if (kStage == kStFinish) {
+                if (live) shade_finish_item(S, F, (int)i);
+            } else if (kStage == kStPixel) {
+                if (live) pixel_item(F, (int)i);
+            } else {  // kStFinalize
+                if (live) finalize_item(F, nullptr, (int)i);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t st) {
+    const dim3 blk(kBatchBlock), grid(l.grid);
+#define BATCH_ARGS(lvl) a.views, a.frames, a.base, a.counter, a.n_jobs, a.chunk, (lvl)
+    auto plan = [&](int stage, int level) { hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, a, stage, level); };
+    hipLaunchKernelGGL(k_batch_init, dim3(a.n_jobs), dim3(64), 0, st, a);
+    plan(kStRaygen, 0);
+    hipLaunchKernelGGL(k_batch_plain<kStRaygen>, grid, blk, 0, st, BATCH_ARGS(0));
+    for (int level = 0; level < l.max_depth; ++level) {
+        plan(kStClosest, level);
+        hipLaunchKernelGGL(k_batch_trace<kStClosest>, grid, blk, 0, st, BATCH_ARGS(level));
+        plan(kStResolve, level);
+        hipLaunchKernelGGL(k_batch_trace<kStResolve>, dim3(64), blk, 0, st, BATCH_ARGS(level));
+        plan(kStShadePrep, level);
+        hipLaunchKernelGGL(k_batch_plain<kStShadePrep>, grid, blk, 0, st, BATCH_ARGS(level));
+        plan(kStShadow, level);
+        hipLaunchKernelGGL(k_batch_trace<kStShadow>, grid, blk, 0, st, BATCH_ARGS(level));
+        plan(kStFinish, level);
+        hipLaunchKernelGGL(k_batch_plain<kStFinish>, grid, blk, 0, st, BATCH_ARGS(level));
+    }
+    plan(kStPixel, 0);
+    hipLaunchKernelGGL(k_batch_plain<kStPixel>, grid, blk, 0, st, BATCH_ARGS(0));
+    plan(kStFinalize, 0);
+    hipLaunchKernelGGL(k_batch_plain<kStFinalize>, grid, blk, 0, st, BATCH_ARGS(0));
+#undef BATCH_ARGS
+    return hipGetLastError();
+}
+
+}  // namespace rt

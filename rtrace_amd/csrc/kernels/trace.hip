@@ -12,6 +12,7 @@
 
 #include "launch.hpp"
 #include "trace_core.hpp"
+#include "wave_ops.hpp"
 
 namespace rt {
 
@@ -26,26 +27,6 @@ __device__ __forceinline__ int xcd_block() {
     int nb = gridDim.x, b = blockIdx.x;
     if (nb % kXcd) return b;
     return (b % kXcd) * (nb / kXcd) + b / kXcd;
-}
-
-// One atomic per wavefront; returns this lane's slot (or -1 when !pred).
-__device__ __forceinline__ int wave_append(int32_t* counter, bool pred) {
-    unsigned long long mask = __ballot(pred);
-    if (mask == 0) return -1;
-    int lane = __lane_id();
-    int leader = __ffsll((long long)mask) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(counter, (int)__popcll(mask));
-    base = __shfl(base, leader);
-    int rank = (int)__popcll(mask & ((1ull << lane) - 1ull));
-    return pred ? base + rank : -1;
-}
-
-__device__ __forceinline__ void flush_work(const FrameView& F, const Work& w, int kernel) {
-    if (!F.count_work) return;
-#pragma unroll
-    for (int k = 0; k < kWorkCount; ++k)
-        if (w.v[k]) atomicAdd(&F.work[kernel * kWorkCount + k], w.v[k]);
 }
 
 __global__ void k_init_counts(FrameView F) {

@@ -78,7 +78,8 @@ struct FrameView {
     int32_t grid_miss;    // misses may show the ortho grid (camera passed to ray_color)
     int32_t count_work;
     int32_t n_paths;
-    int32_t tiles_x;      // raygen tiling
+    int32_t tile;         // raygen tile edge (pixels)
+    int32_t tiles_x;      // raygen tiles per row
     int32_t n_q0;         // level-0 queue length including holes
     uint64_t seed;        // Renderer.seed.unwrap_or(0)
     uint64_t* path_seed;  // [n_paths]
@@ -735,24 +736,24 @@ RT_HD bool grid_color(const CameraDev& c, const RayD& r, d3& out) {
 }
 
 // ------------------------------------------------------------ raygen
-constexpr int kTile = 8;  // 8x8 pixel tile = one 64-lane wavefront
 
 // Level-0 queue slot q -> (path, ray). Slots enumerate 8x8 tiles (holes outside the
 // image get path = -1); quincunx puts centres first, then the (W+1)x(H+1) corners.
 RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
+    const int T = F.tile, T2 = T * T;  // slots run tile by tile, T x T pixels each
     RayRec rec;
     rec.level = 0;
     rec.path = -1;
     const CameraDev& cam = S.cam;
     if (F.aa == 1) {
         // render_quincunx (src/renderer.rs:752-897)
-        int tiles_c = F.tiles_x * ((F.H + kTile - 1) / kTile) * kTile * kTile;
+        int tiles_c = F.tiles_x * ((F.H + T - 1) / T) * T2;
         bool corner = q >= tiles_c;
         int qq = corner ? q - tiles_c : q;
         int gw = corner ? F.W + 1 : F.W, gh = corner ? F.H + 1 : F.H;
-        int tx = (gw + kTile - 1) / kTile;
-        int tile = qq / (kTile * kTile), in = qq % (kTile * kTile);
-        int x = (tile % tx) * kTile + in % kTile, y = (tile / tx) * kTile + in / kTile;
+        int tx = (gw + T - 1) / T;
+        int tile = qq / (T2), in = qq % (T2);
+        int x = (tile % tx) * T + in % T, y = (tile / tx) * T + in / T;
         bool want = x < gw && y < gh;
         if (want && F.tmask)
             want = corner ? (pixel_in_mask(F, x - 1, y - 1) || pixel_in_mask(F, x, y - 1) ||
@@ -788,10 +789,10 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
     } else {
         // render_standard / render_standard_with_outline (src/renderer.rs:445-749);
         // slot = sample * (tiled pixels) + tiled pixel
-        int tiled = F.tiles_x * ((F.H + kTile - 1) / kTile) * kTile * kTile;
+        int tiled = F.tiles_x * ((F.H + T - 1) / T) * T2;
         int smp = q / tiled, qq = q % tiled;
-        int tile = qq / (kTile * kTile), in = qq % (kTile * kTile);
-        int x = (tile % F.tiles_x) * kTile + in % kTile, y = (tile / F.tiles_x) * kTile + in / kTile;
+        int tile = qq / (T2), in = qq % (T2);
+        int x = (tile % F.tiles_x) * T + in % T, y = (tile / F.tiles_x) * T + in / T;
         if (x < F.W && y < F.H && smp < F.N && pixel_in_mask(F, x, y)) {
             double pu = (double)x / (double)(F.W - 1);
             double pv = (double)(F.H - 1 - y) / (double)(F.H - 1);

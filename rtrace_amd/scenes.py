@@ -10,6 +10,8 @@ path consume identical vertices.
             perspective 50 deg, disk area light (16 samples), fog 0.05, depth 4.
   config3   1000 reflective spheres on a ring over a grid plane, 4 point lights,
             stochastic 4 spp, depth 10, outline on.
+  config4   64 seeded scenes (displaced icospheres / extruded polygons, 1k-500k
+            triangles, 256x256 to 1920x1080) for the batched launch, 4 auto-camera views each.
   config5   seeded config-2-style scenes with a 2x2 square light: the denoiser's
             training data (low / high light-sample pairs) and benchmark frames.
 """
@@ -251,6 +253,81 @@ def config5(out_dir, seed, subdiv=None):
         fog = {"color": rc(0.5), "density": rng.uniform(0.02, 0.08), "start": 2.0, "end": 20.0}
     sc = {"camera": cam, "objects": objects, "lights": [light], "scene_settings": _settings(rc(0.0), fog=fog)}
     return _write(out_dir, f"config5_{seed}_s{sd}", sc)
+
+
+def extruded_polygon(target_tris, radius, height, seed):
+    """A random star-shaped polygon (angles jittered inside their sectors, radii in
+    [0.55, 1] x radius) extruded along z into `segments` bands, with fan-triangulated caps:
+    2 * n * segments + 2 * n triangles, n and segments chosen to land near target_tris."""
+    rng = SplitMix64(seed)
+    seg = max(1, int(math.sqrt(target_tris / 16.0)))
+    n = max(3, int(round(target_tris / (2.0 * (seg + 1)))))
+    k = np.arange(n)
+    ang = 2 * math.pi * (k + np.array([rng.uniform(-0.4, 0.4) for _ in range(n)])) / n
+    r = radius * np.array([0.55 + 0.45 * rng.uniform(0, 1) for _ in range(n)])
+    px, py = r * np.cos(ang), r * np.sin(ang)
+    zs = np.linspace(-height / 2, height / 2, seg + 1)
+    i0, i1 = k, (k + 1) % n
+    z0, z1 = zs[:-1, None], zs[1:, None]
+    a = np.stack(np.broadcast_arrays(px[i0], py[i0], z0), -1)  # (seg, n, 3)
+    b = np.stack(np.broadcast_arrays(px[i1], py[i1], z0), -1)
+    c = np.stack(np.broadcast_arrays(px[i1], py[i1], z1), -1)
+    d = np.stack(np.broadcast_arrays(px[i0], py[i0], z1), -1)
+    sides = np.concatenate([np.stack([a, b, c], -2), np.stack([a, c, d], -2)], 0).reshape(-1, 3, 3)
+    caps = []
+    for z in (-height / 2, height / 2):
+        ctr = np.broadcast_to(np.array([0.0, 0.0, z]), (n, 3))
+        caps.append(np.stack([ctr, np.stack([px[i0], py[i0], np.full(n, z)], -1),
+                              np.stack([px[i1], py[i1], np.full(n, z)], -1)], 1))
+    return np.concatenate([sides] + caps, 0)
+
+
+def config4(out_dir, n_scenes=64, seed=4):
+    """The batched workload: `n_scenes` seeded scenes, each a displaced icosphere or an
+    extruded random polygon whose triangle count is log-uniform in [1k, 500k] (icospheres
+    take the nearest subdivision level), seen with a transform, over a grid plane with
+    1-3 spheres and a disk light of 8 shadow samples; each with its own resolution,
+    log-uniform in pixel count between 256x256 and 1920x1080 (aspect 1:1, 4:3 or 16:9).
+    Returns [(scene path, width, height, triangles)]; render each from its 4 auto-camera
+    views with quincunx, depth 4."""
+    os.makedirs(out_dir, exist_ok=True)
+    rng = SplitMix64(seed)
+    out = []
+    for i in range(n_scenes):
+        target = math.exp(rng.uniform(math.log(1000.0), math.log(500000.0)))
+        mesh_name = f"config4_{seed}_{i}.stl"
+        if rng.uniform(0, 1) < 0.5:
+            sd = int(min(7, max(3, round(math.log(target / 20.0, 4)))))
+            tris = icosphere(sd, 1.0, 0.12, seed=1000 * seed + i)
+        else:
+            tris = extruded_polygon(target, 1.0, rng.uniform(0.4, 1.6), seed=1000 * seed + i)
+        write_stl(os.path.join(out_dir, mesh_name), tris)
+        tf = [f"scale({rng.uniform(0.8, 1.6):.4f}, {rng.uniform(0.8, 1.6):.4f}, {rng.uniform(0.8, 1.6):.4f})",
+              f"rotate({rng.uniform(0, 180):.4f}, {rng.uniform(0, 180):.4f}, {rng.uniform(0, 180):.4f})",
+              f"translate(0, 0, {rng.uniform(-0.2, 0.4):.4f})"]
+        rc = lambda lo=0.3: hex_color((rng.uniform(lo, 1), rng.uniform(lo, 1), rng.uniform(lo, 1)))  # noqa: E731
+        objects = [{"kind": "mesh", "filename": mesh_name, "transform": tf,
+                    "material": material(rc(), 0.1, 0.7, 0.4, 32, reflectivity=rng.uniform(0.0, 0.3))}]
+        for _ in range(1 + int(rng.uniform(0, 3))):
+            ang, rr = rng.uniform(0, 2 * math.pi), rng.uniform(2.0, 3.5)
+            objects.append({"kind": "sphere", "center": [rr * math.cos(ang), rr * math.sin(ang), -1.0],
+                            "radius": rng.uniform(0.3, 0.6), "material": material(rc(), 0.1, 0.7, 0.5, 48)})
+        objects.append({"kind": "plane", "point": [0, 0, -1.6], "normal": [0, 0, 1],
+                        "material": material("#C8C8C0", 0.1, 0.7, 0.1, 8, reflectivity=0.2,
+                                             texture={"type": "grid", "line_color": "#404040", "line_width": 0.04,
+                                                      "cell_size": 1.0})})
+        az, el = rng.uniform(0, 2 * math.pi), rng.uniform(0.7, 1.2)
+        lights = [{"position": [7 * math.cos(el) * math.cos(az), 7 * math.cos(el) * math.sin(az), 7 * math.sin(el)],
+                   "color": "#FFFFFF", "intensity": 1.0, "diameter": rng.uniform(0.8, 2.0), "samples": 8}]
+        sc = {"camera": _camera_persp([0, -7, 3], [0, 0, 0], 50), "objects": objects, "lights": lights,
+              "scene_settings": _settings(rc(0.0))}
+        px = math.exp(rng.uniform(math.log(256 * 256), math.log(1920 * 1080)))
+        aspect = (1.0, 4.0 / 3.0, 16.0 / 9.0)[int(rng.uniform(0, 3))]
+        w = int(round(math.sqrt(px * aspect)))
+        w = max(256, min(1920, w))
+        h = max(256, min(1080, int(round(w / aspect))))
+        out.append((_write(out_dir, f"config4_{seed}_{i}", sc), w, h, len(tris)))
+    return out
 
 
 def _write(out_dir, name, scene):

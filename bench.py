@@ -179,6 +179,81 @@ def measure_traffic(workload, kernel):
     return res, None
 
 
+def bench_config4(args):
+    """Configuration 4: 64 seeded scenes x 4 auto-camera views at their own resolutions,
+    rendered as one batch (persistent kernels, one captured graph) with autotuned tile,
+    chunk and occupancy. A step is one replay of the whole batch. Also times the same jobs
+    rendered one by one (rt_render_device), the unbatched path."""
+    import torch
+
+    import rtrace_amd as R
+    from rtrace_amd import scenes
+    out_dir = os.path.join(tempfile.gettempdir(), "rtrace_amd_bench_c4")
+    t0 = time.perf_counter()
+    work = scenes.config4(out_dir, n_scenes=args.c4_scenes)
+    jobs, tris = [], []
+    for path, w, h, n_tri in work:
+        scene = R.Scene.from_json_file(path)
+        for cam in R.AutoCamera.generate_cameras(scene).cameras():
+            r = R.Renderer(w, h)
+            r.max_depth = 4
+            jobs.append((scene, cam, r, torch.empty(h * w * 3, dtype=torch.uint8, device="cuda")))
+            tris.append(n_tri)
+    t_load = time.perf_counter() - t0
+    tune_jobs = jobs[::max(1, len(jobs) // 32)]
+    opts, tune_ms = R.autotune_batch(tune_jobs)
+    b = R.Batch(jobs, opts)
+    for _ in range(args.warmup):
+        b.run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_dev = []
+    for _ in range(args.steps):
+        st = b.run()
+        ms_dev.append(st.ms)
+    torch.cuda.synchronize()
+    ms_step = (time.perf_counter() - t1) * 1e3 / args.steps
+    # the same frames one at a time
+    seq_ms = None
+    if not args.no_sequential:
+        cam_scenes = {}
+        singles = []
+        for scene, cam, r, out in jobs:
+            key = (scene._path, tuple(cam.position))
+            if key not in cam_scenes:
+                sc = json.load(open(scene._path))
+                sc["camera"] = {k: v for k, v in cam.to_json().items() if v is not None}
+                cam_scenes[key] = R.Scene.from_json_str(json.dumps(sc), os.path.dirname(scene._path))
+            singles.append((cam_scenes[key], r, out))
+        for s_, r, out in singles:  # warm
+            r.render_device(s_, out)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for s_, r, out in singles:
+            r.render_device(s_, out)
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t2) * 1e3
+    rays = st.rays_total
+    line = {
+        "metric": "Mrays/s, batched (config 4)", "value": round(rays / (ms_step * 1e-3) / 1e6, 2), "unit": "Mrays/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+f64",
+        "data": "synthetic (seeded procedural scenes)",
+        "config": {"workload": f"config4: {len(work)} scenes x 4 auto-camera views, quincunx, depth 4, disk light "
+                               f"8 samples", "jobs": len(jobs), "pixels": st.pixels,
+                   "triangles_min": min(tris), "triangles_max": max(tris), "rays_per_step": rays,
+                   "primary": st.rays_primary, "reflect": st.rays_reflect, "shadow": st.rays_shadow},
+        "batch": {"opts": opts.as_dict(), "tune_ms_subset": round(tune_ms, 3), "tune_jobs": len(tune_jobs),
+                  "graph_nodes": st.graph_nodes, "graph_launches_per_step": 1, "device_ms": round(min(ms_dev), 3),
+                  "frame_buffer_gib": round(st.bytes / 2**30, 2), "build_ms": round(st.ms_build, 1),
+                  "scene_load_s": round(t_load, 1)},
+    }
+    if seq_ms is not None:
+        line["sequential"] = {"ms": round(seq_ms, 3), "mrays_s": round(rays / (seq_ms * 1e-3) / 1e6, 2),
+                              "batch_speedup": round(seq_ms / ms_step, 3)}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +266,8 @@ def main():
     ap.add_argument("--no-denoise", action="store_true", help="skip the config-5 denoised-vs-plain section")
     ap.add_argument("--dn-seeds", default="100001,100002", help="config-5 scene seeds of the denoise section")
     ap.add_argument("--dn-threshold", type=float, default=0.06, help="tile fallback threshold")
+    ap.add_argument("--c4-scenes", type=int, default=64, help="config4: number of scenes")
+    ap.add_argument("--no-sequential", action="store_true", help="config4: skip the one-by-one comparison")
     args = ap.parse_args()
     if args.traffic_child:
         traffic_child(args.workload)
@@ -199,6 +276,10 @@ def main():
         print(f"bench.py measures one GPU: the trace path has no multi-GPU mode in this build (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "config4":
+        bench_config4(args)
+        return
 
     import torch
 

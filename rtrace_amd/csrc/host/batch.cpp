@@ -77,8 +77,9 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
         if (o.light_samples < 0 || o.light_samples > 65536) throw Error(RT_E_INVALID_ARG, "light_samples out of range");
         if (o.width <= 0 || o.height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
         const SceneDevice& D = *jb.scene->dev;
-        CameraDev cam = make_camera(jb.camera ? *jb.camera : D.host.camera, (double)o.width / (double)o.height);
-        b->hviews[j] = make_view(D, cam, o.light_samples);
+        const rt_camera_desc& cd = jb.camera ? *jb.camera : D.host.camera;
+        CameraDev cam = make_camera(cd, (double)o.width / (double)o.height);
+        b->hviews[j] = make_view(D, cam, o.light_samples, cd.position);
         FrameView& F = b->hframes[2 * j];
         F = FrameView{};
         b->layouts[j] = frame_fill(F, b->hviews[j], &o, bo.tile);

@@ -76,7 +76,8 @@ struct FrameLayout {
 };
 
 CameraDev make_camera(const rt_camera_desc& c, double aspect);
-SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_samples);
+// cam_pos: position of the camera `cam` was built from (nullptr = the scene's camera)
+SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_samples, const double* cam_pos = nullptr);
 int grid_cols(int n);
 // Scalar fields of the frame view for options `o` with raygen tiles of `tile` pixels;
 // validates the options and returns the queue sizes.

@@ -353,7 +353,7 @@ CameraDev make_camera(const rt_camera_desc& c, double aspect) {
     return cam;
 }
 
-SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_samples) {
+SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_samples, const double* cam_pos) {
     const HostScene& H = D.host;
     SceneView S{};
     S.objs = D.objs.as<ObjDev>();
@@ -379,7 +379,7 @@ SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_sample
     S.light_cols = grid_cols(light_samples > 0 ? light_samples : 1);
     S.has_fog = H.has_fog;
     S.cam = cam;
-    memcpy(S.cam_pos, H.camera.position, sizeof(S.cam_pos));
+    memcpy(S.cam_pos, cam_pos ? cam_pos : H.camera.position, sizeof(S.cam_pos));  // the rendering camera's
     memcpy(S.amb, H.ambient_color, sizeof(S.amb));
     S.amb_i = H.ambient_intensity;
     memcpy(S.fog_color, H.fog_color, sizeof(S.fog_color));

@@ -143,6 +143,12 @@ struct RayD {
 #else
 #define RT_COLD inline
 #endif
+// Hot traversal steps are inlined into their kernels so per-ray state stays in registers.
+#if defined(__HIPCC__)
+#define RT_HOT __host__ __device__ inline __attribute__((always_inline))
+#else
+#define RT_HOT inline
+#endif
 
 // ---------------------------------------------------------------- exact f64 tests
 // Sphere::hit (src/ray.rs:89-120); t_max inclusive.

@@ -203,7 +203,7 @@ RT_HD MeshRay make_world_ray(const RayD& r) {
 }
 
 // Conservative slab test of a node's tight box, clipped to [c0, c1]; false = empty.
-RT_HD bool box_clip(const MeshRay& q, const float* bmin, const float* bmax, float& c0, float& c1) {
+RT_HOT bool box_clip(const MeshRay& q, const float* bmin, const float* bmax, float& c0, float& c1) {
     float ax = (bmin[0] - q.ox) * q.ix, bx = (bmax[0] - q.ox) * q.ix;
     float ay = (bmin[1] - q.oy) * q.iy, by = (bmax[1] - q.oy) * q.iy;
     float az = (bmin[2] - q.oz) * q.iz, bz = (bmax[2] - q.oz) * q.iz;
@@ -246,14 +246,15 @@ RT_HD bool line_hits_box64(const RayD& r, const double* b) {
 // t_split >= 0, leaves gated by their line-vs-box test) without any pruning and return
 // the first triangle whose exact t equals `best`. Runs only for tied rays.
 RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, double best, StackRef stk) {
+    const double rox = r.o.x, roy = r.o.y, roz = r.o.z, rdx = r.d.x, rdy = r.d.y, rdz = r.d.z;
     uint32_t node = M.root_kd;
     int sp = 0;
     for (;;) {
         const KdNode& nd = M.nodes[node];
         uint32_t ax = nd.b & 3u;
         if (ax != kAxisLeaf) {
-            double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
-            double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
+            double oa = ax == 0 ? rox : (ax == 1 ? roy : roz);
+            double da = ax == 0 ? rdx : (ax == 1 ? rdy : rdz);
             bool left_near = oa <= nd.split;
             uint32_t nearc = left_near ? node + 1 : nd.a;
             uint32_t farc = left_near ? nd.a : node + 1;
@@ -277,8 +278,7 @@ RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, dou
     }
 }
 
-// kAny = shadow (any hit with tmin < t < tmax); otherwise closest hit improving `best`
-// strictly (t < best, first visited wins ties, src/ray.rs:620-649).
+// Per-ray state of a k-d walk over one mesh.
 //
 // Visit order and child selection are the reference's (src/mesh.rs:399-465), decided in
 // f64: origin side first (origin <= split -> left), far side only when t_split >= 0
@@ -289,164 +289,207 @@ RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, dou
 // box clips it again. Skipped subtrees hold no triangle the reference could accept
 // ahead of the current best, so the candidate set and its visit order are the
 // reference's.
+struct MeshWalk {
+    MeshRay q;
+    float R;        // max |v - centre|_1 over the mesh
+    float tmin_lo;  // lower bound of tmin in f32
+    float best_hi;  // upper bound of best in f32
+    float c0, c1;   // interval of the current node
+    uint32_t node;
+    int sp;
+    bool have_node, par0, par1, par2;
+};
+constexpr uint32_t kNoLeaf = 0xffffffffu;
+
+RT_HOT void walk_init(MeshWalk& s, const MeshDev& M, const RayD& r, bool use_kd, double tmin, double best) {
+    s.q = make_mesh_ray(M, r);
+    s.R = M.radius;
+    s.par0 = fabs(r.d.x) < 1e-9;
+    s.par1 = fabs(r.d.y) < 1e-9;
+    s.par2 = fabs(r.d.z) < 1e-9;
+    s.tmin_lo = (float)tmin * (1.0f - kTolC);
+    s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
+    // the mesh AABB check (src/ray.rs:586-610) is the root's box test
+    s.c0 = s.tmin_lo * 0.5f;
+    s.c1 = s.best_hi;
+    s.node = use_kd ? M.root_kd : M.root_bf;
+    s.have_node = true;
+    s.sp = 0;
+}
+
+// Walks inner nodes until it holds the next leaf whose interval can still beat `best`
+// (returned, with the interval start in leaf_c0) or has no node left (kNoLeaf).
+template <bool count>
+RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, StackRef stk, Work& w, float& leaf_c0) {
+    // selects on values, not on member addresses, keep the ray in registers
+    const double rox = r.o.x, roy = r.o.y, roz = r.o.z, rdx = r.d.x, rdy = r.d.y, rdz = r.d.z;
+    uint32_t leaf = kNoLeaf;
+    while (s.have_node && leaf == kNoLeaf) {
+        bool next = false;  // `node` already holds the next node to visit
+        if (s.c0 <= s.best_hi) {
+            const float4* np = reinterpret_cast<const float4*>(M.nodes + s.node);
+            float4 n0 = np[0];
+            float4 n1 = np[1];
+            if (count) w.v[kWorkNodes]++;
+            const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
+            uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
+            uint32_t ax = nb & 3u;
+            if (box_clip(s.q, bmin, bmax, s.c0, s.c1) && s.c0 <= s.best_hi) {
+                if (ax == kAxisLeaf) {
+                    leaf = s.node;
+                    leaf_c0 = s.c0;
+                } else {
+                    double split = M.nodes[s.node].split;
+                    double oa = ax == 0 ? rox : (ax == 1 ? roy : roz);
+                    double da = ax == 0 ? rdx : (ax == 1 ? rdy : rdz);
+                    float ia = ax == 0 ? s.q.ix : (ax == 1 ? s.q.iy : s.q.iz);
+                    bool par = ax == 0 ? s.par0 : (ax == 1 ? s.par1 : s.par2);
+                    bool left_near = oa <= split;
+                    uint32_t nearc = left_near ? s.node + 1 : na;
+                    uint32_t farc = left_near ? na : s.node + 1;
+                    bool far_ok = !par && ((oa == split) || ((oa < split) == (da > 0.0)));
+                    next = true;
+                    if (!far_ok) {
+                        s.node = nearc;
+                    } else if (nb & kFlagMisplaced) {
+                        stk.node[s.sp * stk.stride] = farc;
+                        stk.t0[s.sp * stk.stride] = s.c0;
+                        ++s.sp;
+                        s.node = nearc;
+                    } else {
+                        float ts = (float)(split - oa) * ia;
+                        float tol = kTolC * fabsf(ts) + 1e-30f;
+                        bool dn = s.c0 <= ts + tol;
+                        bool dfar = ts - tol <= s.c1;
+                        if (dn && dfar) {
+                            stk.node[s.sp * stk.stride] = farc;
+                            stk.t0[s.sp * stk.stride] = fmaxf(s.c0, ts - tol);
+                            ++s.sp;
+                            s.node = nearc;
+                            s.c1 = fminf(s.c1, ts + tol);
+                        } else if (dn) {
+                            s.node = nearc;
+                            s.c1 = fminf(s.c1, ts + tol);
+                        } else if (dfar) {
+                            s.node = farc;
+                            s.c0 = fmaxf(s.c0, ts - tol);
+                        } else {
+                            next = false;
+                        }
+                    }
+                }
+            }
+        }
+        if (!next) {
+            // pop the next pending subtree that can still beat `best`
+            s.have_node = false;
+            while (s.sp > 0) {
+                --s.sp;
+                s.node = stk.node[s.sp * stk.stride];
+                s.c0 = stk.t0[s.sp * stk.stride];
+                if (s.c0 <= s.best_hi) {
+                    s.have_node = true;
+                    break;
+                }
+            }
+            s.c1 = s.best_hi;  // the popped node's own box clips the far end again
+        }
+    }
+    return leaf;
+}
+
+// Tests a leaf's triangles: FP32 Moller-Trumbore with error bounds; survivors re-tested
+// in f64 (intersect_triangle, src/ray.rs:527-583). kAny: true on a hit with
+// tmin < t < best. Otherwise improves best / best_tri strictly (first visited wins ties)
+// and flags `tie` when another triangle hits at exactly `best`.
+template <bool kAny, bool count>
+RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t leaf, double tmin, double& best,
+                     int32_t& best_tri, bool& found, bool& tie, Work& w) {
+    const float4* recs = reinterpret_cast<const float4*>(M.recs);
+    const MeshRay& q = s.q;
+    const KdNode& ln = M.nodes[leaf];
+    uint32_t first = ln.a, cnt = ln.b >> 2;
+    for (uint32_t k = 0; k < cnt; ++k) {
+        float4 r0 = recs[(size_t)(first + k) * 3 + 0];
+        float4 r1 = recs[(size_t)(first + k) * 3 + 1];
+        float4 r2 = recs[(size_t)(first + k) * 3 + 2];
+        if (count) w.v[kWorkTris]++;
+        float e1x = r0.w, e1y = r1.x, e1z = r1.y;
+        float e2x = r1.z, e2y = r1.w, e2z = r2.x;
+        float E = r2.z;  // max(|e1|_1, |e2|_1)
+        float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
+        float a = e1x * hx + e1y * hy + e1z * hz;
+        float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
+        float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + s.R;
+        float fa = fabsf(a);
+        bool sure_in = false;
+        bool maybe = true;
+        if (fa > 4e-8f + kTolC * E * E) {
+            float f = 1.0f / a;
+            float g = E * fabsf(f);
+            float du = kTolC * (S + E) * g + 1e-7f;
+            float u = f * (sx * hx + sy * hy + sz * hz);
+            if (u < -du || u > 1.0f + du) {
+                maybe = false;
+            } else {
+                float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+                float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
+                if (v < -du || u + v > 1.0f + 2.0f * du) {
+                    maybe = false;
+                } else {
+                    float t = f * (e2x * qx + e2y * qy + e2z * qz);
+                    float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
+                    if (t + dt <= s.tmin_lo || t - dt >= s.best_hi) {
+                        maybe = false;
+                    } else if (kAny) {
+                        sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
+                                  t - dt > (float)tmin * (1.0f + kTolC) && t + dt < (float)best * (1.0f - kTolC);
+                    }
+                }
+            }
+        }
+        if (!maybe) continue;
+        if (kAny && sure_in) return true;
+        uint32_t ti = f32_to_bits(r2.y);
+        double tt, uu, vv;
+        d3 nn;
+        if (count) w.v[kWorkRetests]++;
+        if (kAny) {
+            if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) return true;
+        } else if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
+            if (tt < best) {
+                best = tt;
+                best_tri = (int32_t)ti;
+                s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
+                found = true;
+                tie = false;
+            } else if (tt == best && found && (int32_t)ti != best_tri) {
+                tie = true;
+            }
+        }
+    }
+    return false;
+}
+
+// kAny = shadow (any hit with tmin < t < tmax); otherwise closest hit improving `best`
+// strictly (t < best, first visited wins ties, src/ray.rs:620-649).
+//
+// "while-while" traversal: each lane walks inner nodes until it holds its next leaf,
+// then the wavefront tests leaves together, so lanes at inner nodes do not wait through
+// other lanes' triangle loops. Leaves are still tested in this lane's visit order.
 template <bool kAny, bool count>
 RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
                       StackRef stk, Work& w) {
-    const MeshRay q = make_mesh_ray(M, r);
-    const float R = M.radius;  // max |v - centre|_1 over the mesh
-    const bool par0 = fabs(r.d.x) < 1e-9, par1 = fabs(r.d.y) < 1e-9, par2 = fabs(r.d.z) < 1e-9;
-    const float4* recs = reinterpret_cast<const float4*>(M.recs);
-    const float tmin_lo = (float)tmin * (1.0f - kTolC);
-    float best_hi = (float)best * (1.0f + kTolC) + kTolC;  // upper bound of best in f32
-    // the mesh AABB check (src/ray.rs:586-610) is the root's box test
-    float c0 = tmin_lo * 0.5f, c1 = best_hi;
-    uint32_t node = use_kd ? M.root_kd : M.root_bf;
-    bool have_node = true;
-    int sp = 0;
+    MeshWalk s;
+    walk_init(s, M, r, use_kd, tmin, best);
     bool found = false;
     bool tie = false;  // another triangle of this mesh hit at exactly `best`
-    // "while-while" traversal: each lane walks inner nodes until it holds its next leaf,
-    // then the wavefront tests leaves together, so lanes at inner nodes do not wait
-    // through other lanes' triangle loops. Leaves are still tested in this lane's
-    // visit order.
     for (;;) {
-        uint32_t leaf = 0xffffffffu;
         float leaf_c0 = 0.0f;
-        while (have_node && leaf == 0xffffffffu) {
-            bool next = false;  // `node` already holds the next node to visit
-            if (c0 <= best_hi) {
-                const float4* np = reinterpret_cast<const float4*>(M.nodes + node);
-                float4 n0 = np[0];
-                float4 n1 = np[1];
-                if (count) w.v[kWorkNodes]++;
-                const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
-                uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
-                uint32_t ax = nb & 3u;
-                if (box_clip(q, bmin, bmax, c0, c1) && c0 <= best_hi) {
-                    if (ax == kAxisLeaf) {
-                        leaf = node;
-                        leaf_c0 = c0;
-                    } else {
-                        double split = M.nodes[node].split;
-                        double oa = ax == 0 ? r.o.x : (ax == 1 ? r.o.y : r.o.z);
-                        double da = ax == 0 ? r.d.x : (ax == 1 ? r.d.y : r.d.z);
-                        float ia = ax == 0 ? q.ix : (ax == 1 ? q.iy : q.iz);
-                        bool par = ax == 0 ? par0 : (ax == 1 ? par1 : par2);
-                        bool left_near = oa <= split;
-                        uint32_t nearc = left_near ? node + 1 : na;
-                        uint32_t farc = left_near ? na : node + 1;
-                        bool far_ok = !par && ((oa == split) || ((oa < split) == (da > 0.0)));
-                        next = true;
-                        if (!far_ok) {
-                            node = nearc;
-                        } else if (nb & kFlagMisplaced) {
-                            stk.node[sp * stk.stride] = farc;
-                            stk.t0[sp * stk.stride] = c0;
-                            ++sp;
-                            node = nearc;
-                        } else {
-                            float ts = (float)(split - oa) * ia;
-                            float tol = kTolC * fabsf(ts) + 1e-30f;
-                            bool dn = c0 <= ts + tol;
-                            bool dfar = ts - tol <= c1;
-                            if (dn && dfar) {
-                                stk.node[sp * stk.stride] = farc;
-                                stk.t0[sp * stk.stride] = fmaxf(c0, ts - tol);
-                                ++sp;
-                                node = nearc;
-                                c1 = fminf(c1, ts + tol);
-                            } else if (dn) {
-                                node = nearc;
-                                c1 = fminf(c1, ts + tol);
-                            } else if (dfar) {
-                                node = farc;
-                                c0 = fmaxf(c0, ts - tol);
-                            } else {
-                                next = false;
-                            }
-                        }
-                    }
-                }
-            }
-            if (!next) {
-                // pop the next pending subtree that can still beat `best`
-                have_node = false;
-                while (sp > 0) {
-                    --sp;
-                    node = stk.node[sp * stk.stride];
-                    c0 = stk.t0[sp * stk.stride];
-                    if (c0 <= best_hi) {
-                        have_node = true;
-                        break;
-                    }
-                }
-                c1 = best_hi;  // the popped node's own box clips the far end again
-            }
-        }
-        if (leaf == 0xffffffffu) break;  // this lane has no node left
-        if (leaf_c0 > best_hi) continue;  // `best` improved since the leaf was found
-        // leaf: FP32 Moller-Trumbore with error bounds; survivors re-tested in f64
-        const KdNode& ln = M.nodes[leaf];
-        uint32_t first = ln.a, cnt = ln.b >> 2;
-        for (uint32_t k = 0; k < cnt; ++k) {
-            float4 r0 = recs[(size_t)(first + k) * 3 + 0];
-            float4 r1 = recs[(size_t)(first + k) * 3 + 1];
-            float4 r2 = recs[(size_t)(first + k) * 3 + 2];
-            if (count) w.v[kWorkTris]++;
-            float e1x = r0.w, e1y = r1.x, e1z = r1.y;
-            float e2x = r1.z, e2y = r1.w, e2z = r2.x;
-            float E = r2.z;  // max(|e1|_1, |e2|_1)
-            float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
-            float a = e1x * hx + e1y * hy + e1z * hz;
-            float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
-            float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
-            float fa = fabsf(a);
-            bool sure_in = false;
-            bool maybe = true;
-            if (fa > 4e-8f + kTolC * E * E) {
-                float f = 1.0f / a;
-                float g = E * fabsf(f);
-                float du = kTolC * (S + E) * g + 1e-7f;
-                float u = f * (sx * hx + sy * hy + sz * hz);
-                if (u < -du || u > 1.0f + du) {
-                    maybe = false;
-                } else {
-                    float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
-                    float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
-                    if (v < -du || u + v > 1.0f + 2.0f * du) {
-                        maybe = false;
-                    } else {
-                        float t = f * (e2x * qx + e2y * qy + e2z * qz);
-                        float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
-                        if (t + dt <= tmin_lo || t - dt >= best_hi) {
-                            maybe = false;
-                        } else if (kAny) {
-                            sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
-                                      t - dt > (float)tmin * (1.0f + kTolC) && t + dt < (float)best * (1.0f - kTolC);
-                        }
-                    }
-                }
-            }
-            if (!maybe) continue;
-            if (kAny && sure_in) return true;
-            uint32_t ti = f32_to_bits(r2.y);
-            double tt, uu, vv;
-            d3 nn;
-            if (count) w.v[kWorkRetests]++;
-            if (kAny) {
-                if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) return true;
-            } else if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
-                if (tt < best) {
-                    best = tt;
-                    best_tri = (int32_t)ti;
-                    best_hi = (float)best * (1.0f + kTolC) + kTolC;
-                    found = true;
-                    tie = false;
-                } else if (tt == best && found && (int32_t)ti != best_tri) {
-                    tie = true;
-                }
-            }
-        }
+        const uint32_t leaf = walk_to_leaf<count>(s, M, r, stk, w, leaf_c0);
+        if (leaf == kNoLeaf) break;  // this lane has no node left
+        if (leaf_c0 > s.best_hi) continue;  // `best` improved since the leaf was found
+        if (leaf_test<kAny, count>(s, M, r, leaf, tmin, best, best_tri, found, tie, w)) return true;
     }
     // a tie between triangles of this mesh: the reference-order walk (mesh_resolve_tie)
     // picks the winner in a separate pass; kTiePending marks the hit as pending
@@ -636,7 +679,7 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
 }
 
 // Shadow query: World::hit(shadow_ray, 0.001, dist).is_some() (src/lighting.rs:120,185).
-template <bool count>
+template <bool count, bool kMeshes = true>
 RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w) {
     RayF f = make_rayf(r);
     const MeshRay wq = make_world_ray(r);
@@ -663,7 +706,7 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
                 bool may = cube_may_hit(wq, ob);
                 if (count && may) w.v[kWorkCube]++;
                 if (may && cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
-            } else {
+            } else if (kMeshes) {
                 int32_t tri = -1;
                 double best = tmax;
                 if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
@@ -685,7 +728,7 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
             bool may = cube_may_hit(wq, ob);
             if (count && may) w.v[kWorkCube]++;
             if (may && cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
-        } else {
+        } else if (kMeshes) {
             int32_t tri = -1;
             double best = tmax;
             if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
@@ -1124,8 +1167,9 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
 }
 
 // ------------------------------------------------------------ shadow stage
-template <bool count>
-RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRef stk, Work& w) {
+// Shadow ray of task `task` (hit task / slots, slot task % slots): from p + 0.001 n
+// toward the light point, with the distance to it (src/lighting.rs:117-120, 178-185).
+RT_HD void shadow_ray(const SceneView& S, const FrameView& F, int task, RayD& r, double& dist) {
     int hq = task / S.shadow_per_hit;
     const ShadeRec& sr = F.shade[hq];
     d3 p = ld3(sr.p), n = ld3(sr.n);
@@ -1149,10 +1193,16 @@ RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRe
         }
     }
     d3 ld = unit(sub(sp, p));
-    double dist = length(sub(sp, p));
-    RayD r;
+    dist = length(sub(sp, p));
     r.o = add(p, scale(0.001, n));
     r.d = unit(ld);
+}
+
+template <bool count>
+RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRef stk, Work& w) {
+    RayD r;
+    double dist;
+    shadow_ray(S, F, task, r, dist);
     bool blocked = world_any<count>(S, r, 0.001, dist, F.use_kd != 0, stk, w);
     F.vis[task] = blocked ? 0 : 1;
 }

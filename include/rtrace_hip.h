@@ -242,6 +242,13 @@ int rt_render_device_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opt
  * object index (-1 miss), triangle index (-1 non-mesh), t. n = capacity. */
 int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, size_t n);
 
+/* ---- on-disk formats ---- */
+/* Mesh::from_stl_file (src/mesh.rs:751-1003) on the host: ASCII or binary detection, the
+ * reference's parse rules and error messages. Writes the triangle count to *n_tris and,
+ * with vertices != NULL, up to capacity_tris triangles as 9 f64 each. Errors through
+ * rt_last_error(NULL). */
+int rt_mesh_load_stl(const char* path, double* vertices, int64_t capacity_tris, int64_t* n_tris);
+
 /* ---- automatic cameras ---- */
 /* AutoCamera::generate_cameras (src/auto_camera.rs:9-130): left, front, top (ortho) and a
  * 50-degree perspective view of the scene's finite bounds (Scene::compute_finite_bounds,

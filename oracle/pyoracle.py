@@ -74,10 +74,27 @@ def lib():
         L.oracle_mesh_hit.restype = ctypes.c_int
         L.oracle_mesh_hit.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_load_stl.restype = ctypes.c_int
+        L.oracle_load_stl.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64,
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.c_char_p, ctypes.c_int]
         L.oracle_outline.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                      ctypes.POINTER(OracleOpts), ctypes.c_void_p]
         _lib = L
     return _lib
+
+
+def load_stl(path):
+    """Mesh::from_stl_file (src/mesh.rs:751-1003) -> (n, 3, 3) float64 triangles; raises
+    ValueError carrying the reference's error message."""
+    p = os.fspath(path).encode()
+    n = ctypes.c_int64(0)
+    err = ctypes.create_string_buffer(512)
+    if lib().oracle_load_stl(p, None, 0, ctypes.byref(n), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    out = np.zeros((n.value, 3, 3), np.float64)
+    if n.value and lib().oracle_load_stl(p, out.ctypes.data, n.value, ctypes.byref(n), err, 512) != 0:
+        raise ValueError(err.value.decode())
+    return out
 
 
 def rust_round(x):

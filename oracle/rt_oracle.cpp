@@ -424,11 +424,51 @@ bool rust_parse_f64(const std::string& s, double& out) {
     out = strtod(s.c_str(), nullptr);
     return true;
 }
+// char::is_whitespace: Unicode White_Space.
+bool is_white(uint32_t cp) {
+    return cp == 0x20 || (cp >= 0x09 && cp <= 0x0D) || cp == 0x85 || cp == 0xA0 || cp == 0x1680 ||
+           (cp >= 0x2000 && cp <= 0x200A) || cp == 0x2028 || cp == 0x2029 || cp == 0x202F || cp == 0x205F ||
+           cp == 0x3000;
+}
+// Decodes the code point at s[i] (lossy: an ill-formed byte reads as U+FFFD, length 1).
+uint32_t decode_at(const std::string& s, size_t i, size_t& len) {
+    const unsigned char c = (unsigned char)s[i];
+    len = 1;
+    if (c < 0x80) return c;
+    int extra = (c & 0xE0) == 0xC0 ? 1 : (c & 0xF0) == 0xE0 ? 2 : (c & 0xF8) == 0xF0 ? 3 : -1;
+    if (extra < 0 || i + extra >= s.size()) return 0xFFFD;
+    uint32_t cp = c & (0x7F >> (extra + 1));
+    for (int k = 1; k <= extra; ++k) {
+        const unsigned char d = (unsigned char)s[i + k];
+        if ((d & 0xC0) != 0x80) return 0xFFFD;
+        cp = (cp << 6) | (d & 0x3F);
+    }
+    const uint32_t min_cp = extra == 1 ? 0x80 : extra == 2 ? 0x800 : 0x10000;
+    if (cp < min_cp || cp > 0x10FFFF || (cp >= 0xD800 && cp <= 0xDFFF)) return 0xFFFD;
+    len = (size_t)extra + 1;
+    return cp;
+}
+// (start, length) of each code point of s.
+std::vector<std::pair<size_t, size_t>> code_points(const std::string& s) {
+    std::vector<std::pair<size_t, size_t>> r;
+    for (size_t i = 0, len; i < s.size(); i += len) {
+        decode_at(s, i, len);
+        r.push_back({i, len});
+    }
+    return r;
+}
+bool white_at(const std::string& s, const std::pair<size_t, size_t>& c) {
+    size_t len;
+    return is_white(decode_at(s, c.first, len));
+}
+// str::trim
 std::string trim(const std::string& s) {
-    size_t a = 0, b = s.size();
-    while (a < b && isspace((unsigned char)s[a])) ++a;
-    while (b > a && isspace((unsigned char)s[b - 1])) --b;
-    return s.substr(a, b - a);
+    auto cps = code_points(s);
+    size_t a = 0, b = cps.size();
+    while (a < b && white_at(s, cps[a])) ++a;
+    while (b > a && white_at(s, cps[b - 1])) --b;
+    if (a == b) return std::string();
+    return s.substr(cps[a].first, cps[b - 1].first + cps[b - 1].second - cps[a].first);
 }
 
 // Transform::from_str + to_matrix + parse_transforms (src/scene.rs:117-203).
@@ -565,17 +605,58 @@ std::vector<std::string> rust_lines(const std::string& content) {
     }
     return out;
 }
+// str::split_whitespace
 std::vector<std::string> split_ws(const std::string& s) {
     std::vector<std::string> r;
-    std::istringstream is(s);
-    std::string t;
-    while (is >> t) r.push_back(t);
+    std::string cur;
+    for (const auto& c : code_points(s)) {
+        if (white_at(s, c)) {
+            if (!cur.empty()) r.push_back(cur);
+            cur.clear();
+        } else {
+            cur += s.substr(c.first, c.second);
+        }
+    }
+    if (!cur.empty()) r.push_back(cur);
     return r;
 }
+// std::str::from_utf8 acceptance: shortest forms only, no surrogates, at most U+10FFFF.
+bool utf8_ok(const std::string& s) {
+    size_t i = 0;
+    const size_t n = s.size();
+    while (i < n) {
+        const unsigned char c = (unsigned char)s[i];
+        if (c < 0x80) {
+            ++i;
+            continue;
+        }
+        uint32_t cp;
+        int extra;
+        if ((c & 0xE0) == 0xC0) { cp = c & 0x1F; extra = 1; }
+        else if ((c & 0xF0) == 0xE0) { cp = c & 0x0F; extra = 2; }
+        else if ((c & 0xF8) == 0xF0) { cp = c & 0x07; extra = 3; }
+        else return false;
+        if (i + extra >= n) return false;  // truncated sequence
+        for (int k = 1; k <= extra; ++k) {
+            const unsigned char d = (unsigned char)s[i + k];
+            if ((d & 0xC0) != 0x80) return false;
+            cp = (cp << 6) | (d & 0x3F);
+        }
+        const uint32_t min_cp = extra == 1 ? 0x80 : extra == 2 ? 0x800 : 0x10000;
+        if (cp < min_cp || cp > 0x10FFFF || (cp >= 0xD800 && cp <= 0xDFFF)) return false;
+        i += extra + 1;
+    }
+    return true;
+}
+// is_ascii_stl (src/mesh.rs:787-809): BufRead::lines over the file; each line read must
+// be UTF-8 (an io error otherwise); the loop stops after reading an 11th line.
 bool is_ascii_stl(const std::string& bytes) {
     auto lines = rust_lines(bytes);
-    for (size_t i = 0; i < lines.size() && i < 10; ++i) {
-        std::string t = trim(lines[i]);
+    int count = 0;
+    for (const std::string& l : lines) {
+        if (!utf8_ok(l)) throw Err(-3, "stream did not contain valid UTF-8");
+        if (++count > 10) break;
+        std::string t = trim(l);
         if (t.compare(0, 12, "facet normal") == 0 || t == "outer loop" || t == "endloop") return true;
     }
     return false;
@@ -585,9 +666,10 @@ std::vector<Tri> load_stl(const std::string& path) {
     if (!f) throw Err(-3, "No such file or directory (os error 2): " + path);
     std::string bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
     if (bytes.size() < 80) throw Err(-3, "failed to fill whole buffer");
-    std::string header = trim(bytes.substr(0, 80));
+    std::string header = trim(bytes.substr(0, 80));  // from_utf8_lossy(header).trim_start()
     std::vector<Tri> tris;
     if (header.compare(0, 5, "solid") == 0 && is_ascii_stl(bytes)) {
+        if (!utf8_ok(bytes)) throw Err(-3, "stream did not contain valid UTF-8");  // read_to_string (:831-835)
         auto lines = rust_lines(bytes);
         size_t i = 0;
         while (i < lines.size()) {
@@ -595,6 +677,9 @@ std::vector<Tri> load_stl(const std::string& path) {
             if (line.compare(0, 12, "facet normal") == 0) {
                 auto parts = split_ws(line);
                 if (parts.size() != 5) throw Err(-3, "Invalid facet normal format");
+                double nrm[3];  // Triangle::normal, never read by the path (:855-858)
+                for (int k = 0; k < 3; ++k)
+                    if (!rust_parse_f64(parts[k + 2], nrm[k])) throw Err(-3, "invalid float literal");
                 ++i;
                 if (i >= lines.size() || trim(lines[i]) != "outer loop") throw Err(-3, "Expected 'outer loop' after facet normal");
                 Tri t;
@@ -2028,6 +2113,24 @@ int oracle_render(const char* scene_path, const oracle_opts* opts, uint8_t* rgb_
     } catch (const std::exception& e) {
         set_err(err, errlen, e.what());
         return -99;
+    }
+}
+
+int oracle_load_stl(const char* path, double* out, int64_t capacity_tris, int64_t* n_tris, char* err, int errlen) {
+    try {
+        std::vector<Tri> t = load_stl(path);
+        *n_tris = (int64_t)t.size();
+        if (out)
+            for (size_t k = 0; k < t.size() && (int64_t)k < capacity_tris; ++k)
+                for (int j = 0; j < 3; ++j) {
+                    out[k * 9 + j * 3 + 0] = t[k].v[j].x;
+                    out[k * 9 + j * 3 + 1] = t[k].v[j].y;
+                    out[k * 9 + j * 3 + 2] = t[k].v[j].z;
+                }
+        return 0;
+    } catch (const Err& e) {
+        set_err(err, errlen, e.what());
+        return e.code;
     }
 }
 

@@ -73,6 +73,9 @@ int oracle_render_aux(const char* scene_path, const oracle_opts* opts, uint8_t* 
  * (left, front, top, perspective) x 14 doubles: kind (0 ortho, 1 perspective), position,
  * target, up, width, height, fov (0 for ortho), 0. */
 int oracle_auto_cameras(const char* scene_path, double* out, char* err, int errlen);
+/* Mesh::from_stl_file (src/mesh.rs:751-1003): triangle count, and with out != NULL the
+ * f64 vertices (9 per triangle, up to capacity_tris). */
+int oracle_load_stl(const char* path, double* out, int64_t capacity_tris, int64_t* n_tris, char* err, int errlen);
 
 /* Unit-level hooks used by the tests (each restates one reference function). */
 void oracle_chacha_block(const uint32_t in[16], int double_rounds, uint32_t out[16]);

@@ -17,7 +17,7 @@ import os
 import numpy as np
 
 __all__ = ["AntiAliasingMode", "OutlineConfig", "Renderer", "Scene", "RenderStats", "RtraceError", "lib",
-           "intersect", "last_primary_hits", "cli_size"]
+           "intersect", "last_primary_hits", "cli_size", "load_stl"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtrace_hip.so")
@@ -160,6 +160,7 @@ def lib():
                                    P, P, P]
         L.rt_scene_auto_cameras.argtypes = [P, P]
         L.rt_auto_cameras_file.argtypes = [ctypes.c_char_p, P]
+        L.rt_mesh_load_stl.argtypes = [ctypes.c_char_p, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
         L.rt_batch_opts_default.argtypes = [ctypes.POINTER(BatchOpts)]
         L.rt_batch_create.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(BatchOpts), ctypes.POINTER(P)]
         L.rt_batch_run.argtypes = [P, ctypes.POINTER(BatchStats)]
@@ -355,6 +356,24 @@ def intersect(scene, origins, dirs, t_min=0.001, t_max=float("inf"), use_kdtree=
     return obj, tri, t
 
 
+def load_stl(path):
+    """Mesh::from_stl_file (src/mesh.rs:751-1003) on the host: the triangles as an
+    (n, 3, 3) float64 array (vertex-major, xyz). Raises RtraceError with the reference's
+    message for a malformed file."""
+    p = os.fspath(path).encode()
+    n = ctypes.c_int64(0)
+    _check_global(lib().rt_mesh_load_stl(p, None, 0, ctypes.byref(n)))
+    out = np.zeros((n.value, 3, 3), np.float64)
+    if n.value:
+        _check_global(lib().rt_mesh_load_stl(p, out.ctypes.data, n.value, ctypes.byref(n)))
+    return out
+
+
+def _check_global(rc):
+    if rc != 0:
+        raise RtraceError(rc, lib().rt_last_error(None).decode())
+
+
 def cli_size(camera_width, camera_height, diagonal):
     """cli/src/main.rs:79-89: H = round(D / sqrt(R^2 + 1)), W = round(R * H), round half away from zero."""
     import math
@@ -392,9 +411,7 @@ class AutoCamera:
         """`scene`: a loaded Scene, or a Scene JSON path (host only, no device needed)."""
         out = (CameraDesc * 4)()
         if isinstance(scene, (str, os.PathLike)):
-            rc = lib().rt_auto_cameras_file(os.fspath(scene).encode(), out)
-            if rc != 0:
-                raise RtraceError(rc, lib().rt_last_error(None).decode())
+            _check_global(lib().rt_auto_cameras_file(os.fspath(scene).encode(), out))
         else:
             _check(lib().rt_scene_auto_cameras(scene._h, out))
         return AutoCameraResult([out[i] for i in range(4)])

@@ -287,6 +287,15 @@ int rt_scene_auto_cameras(rt_scene* scene, rt_camera_desc out[4]) {
     return guard(scene->ctx, [&] { auto_cameras(scene->dev->host, out); });
 }
 
+int rt_mesh_load_stl(const char* path, double* vertices, int64_t capacity_tris, int64_t* n_tris) {
+    if (!path || !n_tris) return ctx_fail(nullptr, RT_E_INVALID_ARG, "null argument");
+    return guard(nullptr, [&] {
+        std::vector<double> v = load_stl_file(path);
+        *n_tris = (int64_t)(v.size() / 9);
+        if (vertices) std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), (size_t)capacity_tris * 9), vertices);
+    });
+}
+
 int rt_auto_cameras_file(const char* scene_path, rt_camera_desc out[4]) {
     if (!scene_path || !out) return ctx_fail(nullptr, RT_E_INVALID_ARG, "null argument");
     return guard(nullptr, [&] { auto_cameras(load_scene_file(scene_path), out); });

@@ -29,11 +29,59 @@ std::string read_file(const std::string& path) {
     return ss.str();
 }
 
+// Byte length of the code point at p when it has Unicode's White_Space property (what
+// Rust's str::trim and split_whitespace remove), else 0.
+size_t ws_len(const unsigned char* p, const unsigned char* e) {
+    const unsigned c = p[0];
+    const ptrdiff_t left = e - p;
+    if (c == ' ' || (c >= 0x09 && c <= 0x0D)) return 1;
+    if (c == 0xC2) return left >= 2 && (p[1] == 0x85 || p[1] == 0xA0) ? 2 : 0;  // U+0085, U+00A0
+    if (left < 3) return 0;
+    if (c == 0xE1) return p[1] == 0x9A && p[2] == 0x80 ? 3 : 0;  // U+1680
+    if (c == 0xE3) return p[1] == 0x80 && p[2] == 0x80 ? 3 : 0;  // U+3000
+    if (c != 0xE2) return 0;
+    if (p[1] == 0x81) return p[2] == 0x9F ? 3 : 0;  // U+205F
+    if (p[1] != 0x80) return 0;
+    const unsigned d = p[2];  // U+2000..U+200A, U+2028, U+2029, U+202F
+    return (d >= 0x80 && d <= 0x8A) || d == 0xA8 || d == 0xA9 || d == 0xAF ? 3 : 0;
+}
+
+// str::trim
 std::string strip(const std::string& s) {
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(s.data());
     size_t a = 0, b = s.size();
-    while (a < b && std::isspace((unsigned char)s[a])) ++a;
-    while (b > a && std::isspace((unsigned char)s[b - 1])) --b;
+    for (size_t k; a < b && (k = ws_len(p + a, p + b)) != 0;) a += k;
+    for (bool more = true; more && b > a;) {
+        more = false;
+        for (size_t k = 1; k <= 3 && k <= b - a; ++k)
+            if (ws_len(p + b - k, p + b) == k) {
+                b -= k;
+                more = true;
+                break;
+            }
+    }
     return s.substr(a, b - a);
+}
+
+// str::split_whitespace
+std::vector<std::string> split_ws(const std::string& s) {
+    const unsigned char* p = reinterpret_cast<const unsigned char*>(s.data());
+    const unsigned char* e = p + s.size();
+    std::vector<std::string> w;
+    size_t i = 0, start = std::string::npos;
+    while (i < s.size()) {
+        const size_t k = ws_len(p + i, e);
+        if (k == 0) {
+            if (start == std::string::npos) start = i;
+            ++i;
+            continue;
+        }
+        if (start != std::string::npos) w.push_back(s.substr(start, i - start));
+        start = std::string::npos;
+        i += k;
+    }
+    if (start != std::string::npos) w.push_back(s.substr(start));
+    return w;
 }
 
 // Rust's f64 FromStr grammar: [+-] (inf | infinity | nan | digits[.digits][e[+-]digits]).
@@ -307,22 +355,53 @@ bool invert4(const double m[16], double out[16]) {
     return true;
 }
 
+// Whether bytes [b, e) are well-formed UTF-8 (what Rust's String::from_utf8 accepts).
+bool valid_utf8(const unsigned char* b, const unsigned char* e) {
+    while (b < e) {
+        unsigned c = *b;
+        int n;
+        unsigned lo = 0x80, hi = 0xBF;
+        if (c < 0x80) { ++b; continue; }
+        if (c >= 0xC2 && c <= 0xDF) n = 1;
+        else if (c == 0xE0) { n = 2; lo = 0xA0; }
+        else if (c == 0xED) { n = 2; hi = 0x9F; }
+        else if (c >= 0xE1 && c <= 0xEF) n = 2;
+        else if (c == 0xF0) { n = 3; lo = 0x90; }
+        else if (c >= 0xF1 && c <= 0xF3) n = 3;
+        else if (c == 0xF4) { n = 3; hi = 0x8F; }
+        else return false;
+        if (e - b <= n) return false;
+        if (b[1] < lo || b[1] > hi) return false;
+        for (int k = 2; k <= n; ++k)
+            if (b[k] < 0x80 || b[k] > 0xBF) return false;
+        b += n + 1;
+    }
+    return true;
+}
+
 std::vector<double> load_stl_file(const std::string& path) {
     std::string bytes = read_file(path);
     if (bytes.size() < 80) throw Error(RT_E_IO, "failed to fill whole buffer");
     std::vector<double> out;
+    const unsigned char* data = reinterpret_cast<const unsigned char*>(bytes.data());
     // ASCII when the 80-byte header starts with "solid" (after leading whitespace) and
-    // one of the first 10 lines is a facet/loop marker (src/mesh.rs:754-828).
+    // one of the first 10 lines is a facet/loop marker (src/mesh.rs:754-828). The probe
+    // reads whole lines as UTF-8 strings (BufRead::lines, up to an 11th line before it
+    // stops), so a binary file whose header starts with "solid" usually fails there.
     bool ascii = false;
     {
-        std::string head = bytes.substr(0, 80);
+        // String::from_utf8_lossy(header).trim_start(): an ill-formed byte becomes U+FFFD,
+        // which is not whitespace, so trimming the raw bytes gives the same prefix
         size_t a = 0;
-        while (a < head.size() && std::isspace((unsigned char)head[a])) ++a;
-        if (head.compare(a, 5, "solid") == 0) {
+        for (size_t k; a < 80 && (k = ws_len(data + a, data + 80)) != 0;) a += k;
+        if (bytes.compare(a, 5, "solid") == 0 && a + 5 <= 80) {
             size_t pos = 0;
-            for (int line = 0; line < 10 && pos < bytes.size(); ++line) {
+            for (int line = 0; pos < bytes.size(); ++line) {
                 size_t nl = bytes.find('\n', pos);
-                std::string l = strip(bytes.substr(pos, nl == std::string::npos ? std::string::npos : nl - pos));
+                size_t end = nl == std::string::npos ? bytes.size() : nl;
+                if (!valid_utf8(data + pos, data + end)) throw Error(RT_E_IO, "stream did not contain valid UTF-8");
+                if (line == 10) break;
+                std::string l = strip(bytes.substr(pos, end - pos));
                 if (l.compare(0, 12, "facet normal") == 0 || l == "outer loop" || l == "endloop") { ascii = true; break; }
                 if (nl == std::string::npos) break;
                 pos = nl + 1;
@@ -330,6 +409,8 @@ std::vector<double> load_stl_file(const std::string& path) {
         }
     }
     if (ascii) {
+        // read_to_string: the whole file must be UTF-8
+        if (!valid_utf8(data, data + bytes.size())) throw Error(RT_E_IO, "stream did not contain valid UTF-8");
         std::vector<std::string> lines;
         size_t pos = 0;
         while (pos < bytes.size()) {
@@ -340,22 +421,19 @@ std::vector<double> load_stl_file(const std::string& path) {
             if (nl == std::string::npos) break;
             pos = nl + 1;
         }
-        auto words = [](const std::string& s) {
-            std::vector<std::string> w;
-            std::istringstream is(s);
-            std::string t;
-            while (is >> t) w.push_back(t);
-            return w;
-        };
         for (size_t i = 0; i < lines.size(); ++i) {
             if (lines[i].compare(0, 12, "facet normal") != 0) continue;
-            if (words(lines[i]).size() != 5) throw Error(RT_E_IO, "Invalid facet normal format");
+            const auto nw = split_ws(lines[i]);
+            if (nw.size() != 5) throw Error(RT_E_IO, "Invalid facet normal format");
+            double normal;  // parsed for its errors only: the path never reads it
+            for (int k = 2; k < 5; ++k)
+                if (!parse_rust_f64(nw[k], normal)) throw Error(RT_E_IO, "invalid float literal");
             if (++i >= lines.size() || lines[i] != "outer loop") throw Error(RT_E_IO, "Expected 'outer loop' after facet normal");
             double tri[9];
             for (int v = 0; v < 3; ++v) {
                 if (++i >= lines.size()) throw Error(RT_E_IO, "Unexpected end of file while reading vertex");
                 if (lines[i].compare(0, 6, "vertex") != 0) throw Error(RT_E_IO, "Expected vertex line");
-                auto w = words(lines[i]);
+                auto w = split_ws(lines[i]);
                 if (w.size() != 4) throw Error(RT_E_IO, "Invalid vertex format");
                 for (int k = 0; k < 3; ++k)
                     if (!parse_rust_f64(w[k + 1], tri[3 * v + k])) throw Error(RT_E_IO, "invalid float literal");

@@ -179,6 +179,11 @@ def measure_traffic(workload, kernel):
     return res, None
 
 
+def _progress(msg):
+    """A progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def bench_config4(args):
     """Configuration 4: 64 seeded scenes x 4 auto-camera views at their own resolutions,
     rendered as one batch (persistent kernels, one captured graph) with autotuned tile,
@@ -199,9 +204,12 @@ def bench_config4(args):
             r.max_depth = 4
             jobs.append((scene, cam, r, torch.empty(h * w * 3, dtype=torch.uint8, device="cuda")))
             tris.append(n_tri)
+        if len(tris) % 32 == 0:
+            _progress(f"loaded {len(tris) // 4} scenes ({time.perf_counter() - t0:.1f} s)")
     t_load = time.perf_counter() - t0
     tune_jobs = jobs[::max(1, len(jobs) // 32)]
     opts, tune_ms = R.autotune_batch(tune_jobs)
+    _progress(f"autotuned on {len(tune_jobs)} jobs: {opts.as_dict()}")
     b = R.Batch(jobs, opts)
     for _ in range(args.warmup):
         b.run()
@@ -213,6 +221,7 @@ def bench_config4(args):
         ms_dev.append(st.ms)
     torch.cuda.synchronize()
     ms_step = (time.perf_counter() - t1) * 1e3 / args.steps
+    _progress(f"batch: {ms_step:.1f} ms per step")
     # the same frames one at a time
     seq_ms = None
     if not args.no_sequential:
@@ -228,6 +237,7 @@ def bench_config4(args):
         for s_, r, out in singles:  # warm
             r.render_device(s_, out)
         torch.cuda.synchronize()
+        _progress(f"sequential warm-up done ({len(singles)} frames)")
         t2 = time.perf_counter()
         for s_, r, out in singles:
             r.render_device(s_, out)

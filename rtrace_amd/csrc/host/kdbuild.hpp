@@ -20,6 +20,8 @@ struct FlatKd {
     int max_leaf = 0;
 };
 
-void build_flat_kd(const std::vector<double>& verts, FlatKd& out);
+// parallel: build the subtrees below the top levels on their own threads (meshes of
+// 20k triangles or more); the result is the same as the serial build, array for array.
+void build_flat_kd(const std::vector<double>& verts, FlatKd& out, bool parallel = true);
 
 }  // namespace rt

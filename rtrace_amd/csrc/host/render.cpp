@@ -493,15 +493,33 @@ uint64_t primary_rays(const FrameView& F, long long n_paths) {
     const int tx = F.mtiles_x, ty = (F.H + F.mtile - 1) / F.mtile;
     std::vector<uint8_t> m((size_t)tx * ty);
     HIP_TRY(hipMemcpy(m.data(), F.tmask, m.size(), hipMemcpyDeviceToHost));
-    auto in = [&](int x, int y) {
-        return x >= 0 && y >= 0 && x < F.W && y < F.H && m[(size_t)(y / F.mtile) * tx + x / F.mtile] != 0;
+    // mask tile of pixel (x, y), or -1 outside the image or the mask
+    auto tile_of = [&](int x, int y) -> long long {
+        if (x < 0 || y < 0 || x >= F.W || y >= F.H) return -1;
+        const long long t = (long long)(y / F.mtile) * tx + x / F.mtile;
+        return m[t] ? t : -1;
     };
+    // Visits the flagged tiles only. A corner is counted by the tile of its first masked
+    // neighbour pixel (up-left, up-right, down-left, down-right), which is one of the
+    // flagged tiles whose closed pixel range holds the corner.
     uint64_t n = 0;
-    for (int y = 0; y < F.H; ++y)
-        for (int x = 0; x < F.W; ++x) n += in(x, y) ? (F.aa == 1 ? 1 : F.N) : 0;
-    if (F.aa == 1)
-        for (int y = 0; y <= F.H; ++y)
-            for (int x = 0; x <= F.W; ++x) n += (in(x - 1, y - 1) || in(x, y - 1) || in(x - 1, y) || in(x, y)) ? 1 : 0;
+    for (int j = 0; j < ty; ++j)
+        for (int i = 0; i < tx; ++i) {
+            const long long t = (long long)j * tx + i;
+            if (!m[t]) continue;
+            const int x0 = i * F.mtile, y0 = j * F.mtile;
+            const int x1 = std::min(F.W, x0 + F.mtile), y1 = std::min(F.H, y0 + F.mtile);
+            n += (uint64_t)(x1 - x0) * (y1 - y0) * (F.aa == 1 ? 1 : F.N);
+            if (F.aa != 1) continue;
+            for (int y = y0; y <= y1; ++y)
+                for (int x = x0; x <= x1; ++x) {
+                    long long owner = tile_of(x - 1, y - 1);
+                    if (owner < 0) owner = tile_of(x, y - 1);
+                    if (owner < 0) owner = tile_of(x - 1, y);
+                    if (owner < 0) owner = tile_of(x, y);
+                    n += owner == t ? 1 : 0;
+                }
+        }
     return n;
 }
 

@@ -364,3 +364,11 @@ def test_tile_masked_rerender_composites_exactly(R, config_dir, aa):
     assert np.array_equal(o[pm], f[pm])
     assert np.array_equal(o[~pm], lo[~pm])
     assert st.rays_primary < st_full.rays_primary
+    # camera rays of the masked pixels: quincunx centres plus every corner touching one
+    if aa == "quincunx":
+        pad = np.zeros((H + 2, W + 2), bool)
+        pad[1:-1, 1:-1] = pm
+        corners = pad[:-1, :-1] | pad[:-1, 1:] | pad[1:, :-1] | pad[1:, 1:]
+        assert st.rays_primary == pm.sum() + corners.sum()
+    else:
+        assert st.rays_primary == pm.sum() * r.samples

@@ -1,0 +1,23 @@
+"""Host k-d build (KdTree::new / build_recursive, src/mesh.rs:75-197): the concurrent
+subtree build produces the serial build's arrays exactly (nodes, triangle records, f64
+node bounds, roots, leaf statistics) on seeded random meshes, including a flat mesh with
+a degenerate triangle. The check program is compiled with g++ from the library's own
+builder source."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_parallel_kd_build_matches_serial(tmp_path):
+    exe = str(tmp_path / "kdbuild_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-pthread", "-o", exe,
+                    os.path.join(ROOT, "tests", "cpp", "kdbuild_check.cpp"),
+                    os.path.join(ROOT, "rtrace_amd", "csrc", "host", "kdbuild.cpp")], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "ok 6"

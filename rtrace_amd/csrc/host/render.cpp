@@ -436,7 +436,9 @@ FrameLayout frame_fill(FrameView& F, const SceneView& S, const rt_render_opts* o
     F.max_depth = L.depth;
     L.cap = (size_t)L.n_q0;
     L.K = (size_t)S.shadow_per_hit;
-    if (L.cap * L.K >= (1ull << 31)) throw Error(RT_E_INVALID_ARG, "too many shadow samples per frame for one launch");
+    // shadow items are int32 indices; the work counters overshoot by one take per wavefront
+    if (L.cap * L.K >= (1ull << 31) - (1ull << 24))
+        throw Error(RT_E_INVALID_ARG, "too many shadow samples per frame for one launch");
     F.ol_wd = o->outline_depth_weight;
     F.ol_wn = o->outline_normal_weight;
     F.ol_thr = o->outline_threshold;
@@ -472,6 +474,7 @@ size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char*
     F.hit_count = (int32_t*)take((depth + 1) * sizeof(int32_t));
     F.tieq = (int32_t*)take(cap * sizeof(int32_t));
     F.tie_count = (int32_t*)take((depth + 1) * sizeof(int32_t));
+    F.take = (int32_t*)take((2 * depth + 2) * 8 * 32 * sizeof(int32_t));
     F.shade = (ShadeRec*)take(cap * sizeof(ShadeRec));
     F.spt = (double*)take(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
     F.vis = (uint8_t*)take(cap * L.K);

@@ -10,6 +10,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "launch.hpp"
 #include "trace_core.hpp"
 #include "wave_ops.hpp"
@@ -18,8 +20,14 @@ namespace rt {
 
 namespace {
 
-constexpr int kBlock = 128;  // traversal kernels: 2 wavefronts, 16 KiB of LDS stack
+constexpr int kBlock = 128;       // shadow and resolve: 2 wavefronts, 16 KiB of LDS stack
+constexpr int kClosestBlock = 64;  // closest hit: 1 wavefront, 8 KiB of LDS stack
 constexpr int kXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robin
+// traversal work queues (WaveQueue): per launch up to kXcd slice counters, kQueuePad
+// int32 apart (one 128-byte line each)
+constexpr int kQueuePad = 32;
+constexpr int kClosestSlices = 1;
+constexpr int kShadowSlices = kXcd;
 
 // Workgroup id remapped so that each XCD (and its L2) walks one contiguous slice of
 // the item range instead of every 8th workgroup.
@@ -37,6 +45,7 @@ __global__ void k_init_counts(FrameView F) {
             F.tie_count[i] = 0;
         }
     }
+    for (int i = threadIdx.x; i < 2 * kXcd * kQueuePad * F.max_depth; i += blockDim.x) F.take[i] = 0;
     if (threadIdx.x < 2 * kWorkCount) F.work[threadIdx.x] = 0;
 }
 
@@ -45,27 +54,67 @@ __global__ __launch_bounds__(256) void k_raygen(SceneView S, FrameView F) {
     for (int q = xcd_block() * blockDim.x + threadIdx.x; q < F.n_q0; q += stride) raygen_item(S, F, q);
 }
 
-template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_closest(SceneView S, FrameView F, int level) {
-    __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
-    __shared__ float s_t0[kMaxKdDepth * kBlock];
-    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
-    Work w;
-    const int n = F.counts[level];
-    const int stride = gridDim.x * kBlock;
-    // every lane of a wavefront runs the same number of iterations so the ballot in
-    // wave_append sees the whole wavefront
-    const int base = xcd_block() * kBlock + (int)threadIdx.x;
-    const int iters = (n + stride - 1) / stride;
-    for (int it = 0; it < iters; ++it) {
-        int slot = base + it * stride;
-        bool hit = false, tie = false;
-        if (slot < n) hit = closest_item<kCount>(S, F, slot, level, stk, w, &tie);
-        int pos = wave_append(&F.hit_count[level], hit);
-        if (hit) F.hitq[pos] = slot;
-        int tpos = wave_append(&F.tie_count[level], tie);
-        if (tie) F.tieq[tpos] = slot;
+// Work queue of a traversal launch: items [0, n) in kSlices contiguous slices (8: one per
+// XCD, as workgroups go to the XCDs round-robin by block id, so each L2 serves one part
+// of the spatially ordered queue). A wavefront takes `step` items at a time from its own
+// slice and moves on to the others once it is drained. The step is sized for about 16
+// takes per wavefront (64 to 256 items): small enough to balance short launches, large
+// enough that the atomic round trip of a take is paid rarely on long ones. Each slice
+// counter has its own 128-byte line. Called where every lane of the wavefront is active.
+template <int kSlices>
+struct WaveQueue {
+    int32_t* ctr;  // [kSlices * kQueuePad] items taken from each slice
+    int n, slice, left, step;
+    __device__ WaveQueue(int32_t* c, int items) : ctr(c), n(items), slice((int)(blockIdx.x % kSlices)), left(kSlices) {
+        const long long waves = (long long)gridDim.x * (blockDim.x / 64);
+        step = (int)min(256ll, max(64ll, ((long long)n / (waves * 16)) & ~63ll));
     }
+    // [b, e): the next items of this wavefront; false when every slice is drained
+    __device__ bool next(int& b, int& e) {
+        while (left > 0) {
+            const int lo = (int)((long long)n * slice / kSlices);
+            const int hi = (int)((long long)n * (slice + 1) / kSlices);
+            int32_t t = 0;
+            if (__lane_id() == 0) t = atomicAdd(&ctr[slice * kQueuePad], step);
+            t = __builtin_amdgcn_readfirstlane(t);
+            if (t < hi - lo) {
+                b = lo + t;
+                e = min(hi, b + step);
+                return true;
+            }
+            slice = (slice + 1) % kSlices;
+            --left;
+        }
+        return false;
+    }
+};
+
+// Traversal kernels run a resident grid whose wavefronts draw 64 items at a time from
+// the launch's queue until none is left: a wavefront that drew short rays takes more,
+// so the launch ends with the last rays rather than with the unluckiest static slice.
+// Held to 128 VGPRs (4 wavefronts per SIMD instead of 3, for a few bytes of spill).
+template <bool kCount>
+__global__ __launch_bounds__(kClosestBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_closest(SceneView S,
+                                                                                                   FrameView F,
+                                                                                                   int level) {
+    __shared__ uint32_t s_node[kMaxKdDepth * kClosestBlock];
+    __shared__ float s_t0[kMaxKdDepth * kClosestBlock];
+    StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kClosestBlock};
+    Work w;
+    WaveQueue<kClosestSlices> queue(&F.take[2 * level * kXcd * kQueuePad], F.counts[level]);
+    const int lane = __lane_id();
+    // the loop is wavefront-uniform, so the ballot in wave_append sees the whole wavefront
+    int b, e;
+    while (queue.next(b, e))
+        for (; b < e; b += 64) {
+            const int slot = b + lane;
+            bool hit = false, tie = false;
+            if (slot < e) hit = closest_item<kCount>(S, F, slot, level, stk, w, &tie);
+            int pos = wave_append(&F.hit_count[level], hit);
+            if (hit) F.hitq[pos] = slot;
+            int tpos = wave_append(&F.tie_count[level], tie);
+            if (tie) F.tieq[tpos] = slot;
+        }
     if (kCount) flush_work(F, w, 0);
 }
 
@@ -100,10 +149,13 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
     __shared__ float s_t0[kMaxKdDepth * kBlock];
     StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
     Work w;
-    const long long n = (long long)F.hit_count[level] * S.shadow_per_hit;
-    const long long stride = (long long)gridDim.x * kBlock;
-    for (long long t = (long long)xcd_block() * kBlock + threadIdx.x; t < n; t += stride)
-        shadow_item<kCount>(S, F, (int)t, stk, w);
+    // < 2^31 items: checked at frame set-up
+    WaveQueue<kShadowSlices> queue(&F.take[(2 * level + 1) * kXcd * kQueuePad], F.hit_count[level] * S.shadow_per_hit);
+    const int lane = __lane_id();
+    int b, e;
+    while (queue.next(b, e))
+        for (; b < e; b += 64)
+            if (b + lane < e) shadow_item<kCount>(S, F, b + lane, stk, w);
     if (kCount) flush_work(F, w, 1);
 }
 
@@ -206,6 +258,17 @@ int grid_for(long long items, int block, int cap) {
     return (int)g;
 }
 
+// Workgroups of `kern` resident on the whole device at once (every CU at the kernel's
+// occupancy), and no more than `items` need at 64 items per wavefront.
+int resident_grid(const void* kern, int block, long long items) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    const long long need = (items + block - 1) / block;
+    return (int)std::max<long long>(1, std::min<long long>(need, (long long)cus * per_cu));
+}
+
 }  // namespace
 
 hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1, hipStream_t st, StageTimer* tm) {
@@ -225,24 +288,27 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
         long long cap_rays = level == 0 ? F.n_q0 : F.n_paths;
         mark(1, true);
         if (F.count_work)
-            hipLaunchKernelGGL(k_closest<true>, dim3(grid_for(cap_rays, kBlock, kCap * 2)), dim3(kBlock), 0, st, S, F,
-                               level);
+            hipLaunchKernelGGL(k_closest<true>, dim3(resident_grid((const void*)k_closest<true>, kClosestBlock, cap_rays)),
+                               dim3(kClosestBlock), 0, st, S, F, level);
         else
-            hipLaunchKernelGGL(k_closest<false>, dim3(grid_for(cap_rays, kBlock, kCap * 2)), dim3(kBlock), 0, st, S, F,
-                               level);
+            hipLaunchKernelGGL(k_closest<false>,
+                               dim3(resident_grid((const void*)k_closest<false>, kClosestBlock, cap_rays)),
+                               dim3(kClosestBlock), 0, st, S, F, level);
         hipLaunchKernelGGL(k_resolve, dim3(64), dim3(kBlock), 0, st, S, F, level);
         mark(1, false);
         mark(2, true);
         hipLaunchKernelGGL(k_shade_prepare, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);
         mark(2, false);
         mark(3, true);
-        dim3 gs(grid_for((long long)F.n_paths * S.shadow_per_hit, kBlock, kCap * 2));
+        const long long cap_shadow = (long long)F.n_paths * S.shadow_per_hit;
         if (S.n_lights == 0) {
             // no shadow rays
         } else if (F.count_work) {
-            hipLaunchKernelGGL(k_shadow<true>, gs, dim3(kBlock), 0, st, S, F, level);
+            hipLaunchKernelGGL(k_shadow<true>, dim3(resident_grid((const void*)k_shadow<true>, kBlock, cap_shadow)),
+                               dim3(kBlock), 0, st, S, F, level);
         } else {
-            hipLaunchKernelGGL(k_shadow<false>, gs, dim3(kBlock), 0, st, S, F, level);
+            hipLaunchKernelGGL(k_shadow<false>, dim3(resident_grid((const void*)k_shadow<false>, kBlock, cap_shadow)),
+                               dim3(kBlock), 0, st, S, F, level);
         }
         mark(3, false);
         mark(4, true);

@@ -96,6 +96,7 @@ struct FrameView {
     int32_t* hit_count;   // [max_depth]
     int32_t* tieq;        // [queue capacity] slots whose mesh tie is pending
     int32_t* tie_count;   // [max_depth]
+    int32_t* take;        // [max_depth][closest, shadow][8 slices][32] items handed out
     ShadeRec* shade;      // [queue capacity]
     double* spt;          // [queue capacity * shadow_per_hit * 3] sample points
     uint8_t* vis;         // [queue capacity * shadow_per_hit]
@@ -679,6 +680,9 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
 }
 
 // Shadow query: World::hit(shadow_ray, 0.001, dist).is_some() (src/lighting.rs:120,185).
+// The scene-order fold finds a hit exactly when some object hits in (tmin, tmax): the
+// first such object sees the full range. So the order is free, and the analytic objects
+// are tested before the meshes, the costly tests.
 template <bool count, bool kMeshes = true>
 RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w) {
     RayF f = make_rayf(r);
@@ -695,6 +699,7 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
             return hit;
         });
         if (hit) return true;
+        bool meshes = false;
         for (int k = 0; k < S.n_others; ++k) {
             const ObjDev& ob = S.objs[S.others[k]];
             double t;
@@ -706,14 +711,21 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
                 bool may = cube_may_hit(wq, ob);
                 if (count && may) w.v[kWorkCube]++;
                 if (may && cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
-            } else if (kMeshes) {
+            } else {
+                meshes = true;
+            }
+        }
+        if (kMeshes && meshes)
+            for (int k = 0; k < S.n_others; ++k) {
+                const ObjDev& ob = S.objs[S.others[k]];
+                if (ob.kind != kMesh) continue;
                 int32_t tri = -1;
                 double best = tmax;
                 if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
             }
-        }
         return false;
     }
+    bool meshes = false;
     for (int i = 0; i < S.n_obj; ++i) {
         const ObjDev& ob = S.objs[i];
         double t;
@@ -728,12 +740,18 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
             bool may = cube_may_hit(wq, ob);
             if (count && may) w.v[kWorkCube]++;
             if (may && cube_exact(ob, r, tmin, tmax, t, p, n)) return true;
-        } else if (kMeshes) {
+        } else {
+            meshes = true;
+        }
+    }
+    if (kMeshes && meshes)
+        for (int i = 0; i < S.n_obj; ++i) {
+            const ObjDev& ob = S.objs[i];
+            if (ob.kind != kMesh) continue;
             int32_t tri = -1;
             double best = tmax;
             if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
         }
-    }
     return false;
 }
 

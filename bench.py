@@ -82,9 +82,16 @@ def cpu_baseline(path, cfg, budget_s=20.0):
         if st["render_seconds"] * 4 > budget_s - spent or w >= cfg["width"]:
             break
         w, h = w * 2, h * 2
+    # one thread (the scalar baseline) on the smallest sample
+    opts = O.make_opts(64, 36, max_depth=cfg["max_depth"], samples=cfg["samples"], aa=cfg["aa"], threads=1,
+                       outline=outline)
+    _, st1, _ = O.render(path, opts)
+    rays1 = st1["rays_primary"] + st1["rays_reflect"] + st1["rays_shadow"]
     return {"value": best[0], "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle render of the same scene at {best[1]}x{best[2]} ({best[4]} rays, {best[3]:.1f} s, "
-                      f"{threads} threads)"}
+                      f"{threads} threads)",
+            "one_thread": {"value": rays1 / st1["render_seconds"] / 1e6, "unit": "Mrays/s", "cores": 1,
+                           "sample": f"64x36 ({rays1} rays, {st1['render_seconds']:.1f} s)"}}
 
 
 COPY_BYTES = 1 << 30

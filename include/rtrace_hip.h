@@ -164,7 +164,7 @@ enum {
 enum { RT_KERNEL_CLOSEST = 0, RT_KERNEL_SHADOW = 1 };
 
 typedef struct {
-    uint64_t nodes;    /* k-d nodes fetched (16 B each) */
+    uint64_t nodes;    /* k-d nodes fetched (32 B box and links, + the 8 B split of an inner node) */
     uint64_t tris;     /* FP32 triangle tests (48 B record each) */
     uint64_t retests;  /* FP64 exact triangle re-tests (72 B of f64 vertices each) */
     uint64_t spheres;  /* sphere tests */

@@ -474,7 +474,7 @@ size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char*
     F.hit_count = (int32_t*)take((depth + 1) * sizeof(int32_t));
     F.tieq = (int32_t*)take(cap * sizeof(int32_t));
     F.tie_count = (int32_t*)take((depth + 1) * sizeof(int32_t));
-    F.take = (int32_t*)take((2 * depth + 2) * 8 * 32 * sizeof(int32_t));
+    F.take = (int32_t*)take((3 * depth + 3) * 8 * 32 * sizeof(int32_t));
     F.shade = (ShadeRec*)take(cap * sizeof(ShadeRec));
     F.spt = (double*)take(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
     F.vis = (uint8_t*)take(cap * L.K);

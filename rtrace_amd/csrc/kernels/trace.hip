@@ -45,7 +45,7 @@ __global__ void k_init_counts(FrameView F) {
             F.tie_count[i] = 0;
         }
     }
-    for (int i = threadIdx.x; i < 2 * kXcd * kQueuePad * F.max_depth; i += blockDim.x) F.take[i] = 0;
+    for (int i = threadIdx.x; i < 3 * kXcd * kQueuePad * F.max_depth; i += blockDim.x) F.take[i] = 0;
     if (threadIdx.x < 2 * kWorkCount) F.work[threadIdx.x] = 0;
 }
 
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kClosestBlock) __attribute__((amdgpu_waves_per_eu(4
     __shared__ float s_t0[kMaxKdDepth * kClosestBlock];
     StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kClosestBlock};
     Work w;
-    WaveQueue<kClosestSlices> queue(&F.take[2 * level * kXcd * kQueuePad], F.counts[level]);
+    WaveQueue<kClosestSlices> queue(&F.take[3 * level * kXcd * kQueuePad], F.counts[level]);
     const int lane = __lane_id();
     // the loop is wavefront-uniform, so the ballot in wave_append sees the whole wavefront
     int b, e;
@@ -123,9 +123,12 @@ __global__ __launch_bounds__(kBlock) void k_resolve(SceneView S, FrameView F, in
     __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
     __shared__ float s_t0[kMaxKdDepth * kBlock];
     StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
-    const int n = F.tie_count[level];
-    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
-        resolve_item(S, F, F.tieq[i], level, stk);
+    WaveQueue<1> queue(&F.take[(3 * level + 2) * kXcd * kQueuePad], F.tie_count[level]);
+    const int lane = __lane_id();
+    int b, e;
+    while (queue.next(b, e))
+        for (; b < e; b += 64)
+            if (b + lane < e) resolve_item(S, F, F.tieq[b + lane], level, stk);
 }
 
 __global__ __launch_bounds__(256) void k_shade_prepare(SceneView S, FrameView F, int level) {
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
     StackRef stk{s_node + threadIdx.x, s_t0 + threadIdx.x, kBlock};
     Work w;
     // < 2^31 items: checked at frame set-up
-    WaveQueue<kShadowSlices> queue(&F.take[(2 * level + 1) * kXcd * kQueuePad], F.hit_count[level] * S.shadow_per_hit);
+    WaveQueue<kShadowSlices> queue(&F.take[(3 * level + 1) * kXcd * kQueuePad], F.hit_count[level] * S.shadow_per_hit);
     const int lane = __lane_id();
     int b, e;
     while (queue.next(b, e))
@@ -294,7 +297,8 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
             hipLaunchKernelGGL(k_closest<false>,
                                dim3(resident_grid((const void*)k_closest<false>, kClosestBlock, cap_rays)),
                                dim3(kClosestBlock), 0, st, S, F, level);
-        hipLaunchKernelGGL(k_resolve, dim3(64), dim3(kBlock), 0, st, S, F, level);
+        hipLaunchKernelGGL(k_resolve, dim3(resident_grid((const void*)k_resolve, kBlock, cap_rays)), dim3(kBlock), 0, st,
+                           S, F, level);
         mark(1, false);
         mark(2, true);
         hipLaunchKernelGGL(k_shade_prepare, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);

@@ -96,7 +96,7 @@ struct FrameView {
     int32_t* hit_count;   // [max_depth]
     int32_t* tieq;        // [queue capacity] slots whose mesh tie is pending
     int32_t* tie_count;   // [max_depth]
-    int32_t* take;        // [max_depth][closest, shadow][8 slices][32] items handed out
+    int32_t* take;        // [max_depth][closest, shadow, resolve][8 slices][32] items handed out
     ShadeRec* shade;      // [queue capacity]
     double* spt;          // [queue capacity * shadow_per_hit * 3] sample points
     uint8_t* vis;         // [queue capacity * shadow_per_hit]
@@ -241,37 +241,69 @@ RT_HD bool line_hits_box64(const RayD& r, const double* b) {
     return t1 >= 0.0;
 }
 
+// FP32 Moller-Trumbore with error bounds on TriRec `rec`: false when the triangle
+// certainly has no hit with t in [lo, hi] (the same bounds the closest-hit walk uses).
+RT_HD bool tri_may_hit_in(const MeshRay& q, float R, const float* recs, uint32_t rec, float lo, float hi) {
+    const float4* rp = reinterpret_cast<const float4*>(recs) + (size_t)rec * 3;
+    const float4 r0 = rp[0], r1 = rp[1], r2 = rp[2];
+    const float e1x = r0.w, e1y = r1.x, e1z = r1.y, e2x = r1.z, e2y = r1.w, e2z = r2.x, E = r2.z;
+    const float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
+    const float a = e1x * hx + e1y * hy + e1z * hz;
+    const float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
+    const float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + R;
+    if (!(fabsf(a) > 4e-8f + kTolC * E * E)) return true;  // near-parallel: no bound, keep it
+    const float f = 1.0f / a, g = E * fabsf(f);
+    const float du = kTolC * (S + E) * g + 1e-7f;
+    const float u = f * (sx * hx + sy * hy + sz * hz);
+    if (u < -du || u > 1.0f + du) return false;
+    const float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
+    const float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
+    if (v < -du || u + v > 1.0f + 2.0f * du) return false;
+    const float t = f * (e2x * qx + e2y * qy + e2z * qz);
+    const float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
+    return !(t + dt < lo || t - dt > hi);
+}
+
 // Exact tie-break: several triangles of this mesh hit at exactly `best`. The reference
 // keeps the first one its traversal reaches (strict `<`, src/ray.rs:633), so walk the
 // tree in the reference's order (src/mesh.rs:399-465: origin side first, far side when
-// t_split >= 0, leaves gated by their line-vs-box test) without any pruning and return
-// the first triangle whose exact t equals `best`. Runs only for tied rays.
+// t_split >= 0, leaves gated by their line-vs-box test) and return the first triangle
+// whose exact t equals `best`. Subtrees whose (conservative FP32) box interval does not
+// hold `best` are skipped: a triangle hit at `best` lies inside the box of every node that
+// holds it, so the walk still reaches the winner first. Inside a leaf, an FP32 bound
+// keeps the f64 test for the triangles that may hit at `best`. Runs only for tied rays.
 RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, double best, StackRef stk) {
     const double rox = r.o.x, roy = r.o.y, roz = r.o.z, rdx = r.d.x, rdy = r.d.y, rdz = r.d.z;
+    const MeshRay q = make_mesh_ray(M, r);
+    const float blo = (float)best * (1.0f - kTolC) - kTolC, bhi = (float)best * (1.0f + kTolC) + kTolC;
     uint32_t node = M.root_kd;
     int sp = 0;
     for (;;) {
         const KdNode& nd = M.nodes[node];
-        uint32_t ax = nd.b & 3u;
-        if (ax != kAxisLeaf) {
-            double oa = ax == 0 ? rox : (ax == 1 ? roy : roz);
-            double da = ax == 0 ? rdx : (ax == 1 ? rdy : rdz);
-            bool left_near = oa <= nd.split;
-            uint32_t nearc = left_near ? node + 1 : nd.a;
-            uint32_t farc = left_near ? nd.a : node + 1;
-            bool far_ok = !(fabs(da) < 1e-9) && ((oa == nd.split) || ((oa < nd.split) == (da > 0.0)));
-            if (far_ok) stk.node[(sp++) * stk.stride] = farc;
-            node = nearc;
-            continue;
-        }
-        if (line_hits_box64(r, M.nb64 + (size_t)node * 6)) {
-            uint32_t first = nd.a, cnt = nd.b >> 2;
-            for (uint32_t k = 0; k < cnt; ++k) {
-                uint32_t ti = f32_to_bits(M.recs[(size_t)(first + k) * 12 + 9]);
-                double tt, uu, vv;
-                d3 nn;
-                if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv) && tt == best)
-                    return (int32_t)ti;
+        float c0 = blo, c1 = bhi;
+        if (box_clip(q, nd.bmin, nd.bmax, c0, c1)) {
+            const uint32_t ax = nd.b & 3u;
+            if (ax != kAxisLeaf) {
+                double oa = ax == 0 ? rox : (ax == 1 ? roy : roz);
+                double da = ax == 0 ? rdx : (ax == 1 ? rdy : rdz);
+                bool left_near = oa <= nd.split;
+                uint32_t nearc = left_near ? node + 1 : nd.a;
+                uint32_t farc = left_near ? nd.a : node + 1;
+                bool far_ok = !(fabs(da) < 1e-9) && ((oa == nd.split) || ((oa < nd.split) == (da > 0.0)));
+                if (far_ok) stk.node[(sp++) * stk.stride] = farc;
+                node = nearc;
+                continue;
+            }
+            if (line_hits_box64(r, M.nb64 + (size_t)node * 6)) {
+                uint32_t first = nd.a, cnt = nd.b >> 2;
+                for (uint32_t k = 0; k < cnt; ++k) {
+                    if (!tri_may_hit_in(q, M.radius, M.recs, first + k, blo, bhi)) continue;
+                    uint32_t ti = f32_to_bits(M.recs[(size_t)(first + k) * 12 + 9]);
+                    double tt, uu, vv;
+                    d3 nn;
+                    if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv) && tt == best)
+                        return (int32_t)ti;
+                }
             }
         }
         if (sp == 0) return -1;

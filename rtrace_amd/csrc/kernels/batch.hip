@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_plain(BATCH_PARAMS) {
             const long long i = i0 + k + threadIdx.x;
             const bool live = i < n;
             if (kStage == kStRaygen) {
-                if (live) raygen_item(S, F, (int)i);
+                if (live) F.rays[i] = raygen_item(S, F, (int)i);
             } else if (kStage == kStShadePrep) {
                 RayRec refl;
                 bool spawn = false;

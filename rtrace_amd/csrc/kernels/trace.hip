@@ -39,7 +39,7 @@ __device__ __forceinline__ int xcd_block() {
 
 __global__ void k_init_counts(FrameView F) {
     for (int i = threadIdx.x; i <= F.max_depth; i += blockDim.x) {
-        F.counts[i] = i == 0 ? F.n_q0 : 0;
+        F.counts[i] = i == 0 && !F.tmask ? F.n_q0 : 0;
         if (i < F.max_depth) {
             F.hit_count[i] = 0;
             F.tie_count[i] = 0;
@@ -51,7 +51,16 @@ __global__ void k_init_counts(FrameView F) {
 
 __global__ __launch_bounds__(256) void k_raygen(SceneView S, FrameView F) {
     int stride = gridDim.x * blockDim.x;
-    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < F.n_q0; q += stride) raygen_item(S, F, q);
+    for (int q = xcd_block() * blockDim.x + threadIdx.x; q < F.n_q0; q += stride) {
+        const RayRec rec = raygen_item(S, F, q);
+        if (F.tmask) {
+            // a tile-masked render queues only the rays of its masked pixels
+            const int pos = wave_append(&F.counts[0], rec.path >= 0);
+            if (rec.path >= 0) F.rays[pos] = rec;
+        } else {
+            F.rays[q] = rec;
+        }
+    }
 }
 
 // Work queue of a traversal launch: items [0, n) in kSlices contiguous slices (8: one per
@@ -68,6 +77,10 @@ struct WaveQueue {
     __device__ WaveQueue(int32_t* c, int items) : ctr(c), n(items), slice((int)(blockIdx.x % kSlices)), left(kSlices) {
         const long long waves = (long long)gridDim.x * (blockDim.x / 64);
         step = (int)min(256ll, max(64ll, ((long long)n / (waves * 16)) & ~63ll));
+        // wavefronts beyond one per 64 items stay out (no atomics): the others drain every
+        // slice, and wavefront 0 always takes part
+        const long long wave = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+        if (wave * 64 >= n) left = 0;
     }
     // [b, e): the next items of this wavefront; false when every slice is drained
     __device__ bool next(int& b, int& e) {
@@ -297,8 +310,9 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
             hipLaunchKernelGGL(k_closest<false>,
                                dim3(resident_grid((const void*)k_closest<false>, kClosestBlock, cap_rays)),
                                dim3(kClosestBlock), 0, st, S, F, level);
-        hipLaunchKernelGGL(k_resolve, dim3(resident_grid((const void*)k_resolve, kBlock, cap_rays)), dim3(kBlock), 0, st,
-                           S, F, level);
+        // ties are rare: a few workgroups per XCD drain the resolver's queue
+        hipLaunchKernelGGL(k_resolve, dim3(std::min(256, resident_grid((const void*)k_resolve, kBlock, cap_rays))),
+                           dim3(kBlock), 0, st, S, F, level);
         mark(1, false);
         mark(2, true);
         hipLaunchKernelGGL(k_shade_prepare, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);

@@ -830,9 +830,9 @@ RT_HD bool grid_color(const CameraDev& c, const RayD& r, d3& out) {
 
 // ------------------------------------------------------------ raygen
 
-// Level-0 queue slot q -> (path, ray). Slots enumerate 8x8 tiles (holes outside the
+// Level-0 queue slot q -> its camera ray record. Slots enumerate T x T tiles (holes outside
 // image get path = -1); quincunx puts centres first, then the (W+1)x(H+1) corners.
-RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
+RT_HD RayRec raygen_item(const SceneView& S, const FrameView& F, int q) {
     const int T = F.tile, T2 = T * T;  // slots run tile by tile, T x T pixels each
     RayRec rec;
     rec.level = 0;
@@ -925,7 +925,7 @@ RT_HD void raygen_item(const SceneView& S, const FrameView& F, int q) {
             }
         }
     }
-    F.rays[q] = rec;
+    return rec;
 }
 
 // ------------------------------------------------------------ closest hit stage

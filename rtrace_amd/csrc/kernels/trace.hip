@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "launch.hpp"
 #include "trace_core.hpp"
@@ -26,7 +28,7 @@ constexpr int kXcd = 8;      // MI355X: 8 XCDs, workgroups dispatched round-robi
 // traversal work queues (WaveQueue): per launch up to kXcd slice counters, kQueuePad
 // int32 apart (one 128-byte line each)
 constexpr int kQueuePad = 32;
-constexpr int kClosestSlices = 1;
+constexpr int kClosestSlices = kXcd;
 constexpr int kShadowSlices = kXcd;
 
 // Workgroup id remapped so that each XCD (and its L2) walks one contiguous slice of
@@ -65,35 +67,55 @@ __global__ __launch_bounds__(256) void k_raygen(SceneView S, FrameView F) {
 
 // Work queue of a traversal launch: items [0, n) in kSlices contiguous slices (8: one per
 // XCD, as workgroups go to the XCDs round-robin by block id, so each L2 serves one part
-// of the spatially ordered queue). A wavefront takes `step` items at a time from its own
-// slice and moves on to the others once it is drained. The step is sized for about 16
-// takes per wavefront (64 to 256 items): small enough to balance short launches, large
-// enough that the atomic round trip of a take is paid rarely on long ones. Each slice
-// counter has its own 128-byte line. Called where every lane of the wavefront is active.
+// of the spatially ordered queue). Items go out in runs of `step`, sized for about 16 runs
+// per wavefront (64 to 256 items). Each wavefront's first run is fixed: the r-th wavefront
+// of a slice takes the slice's r-th run, with no atomic, so a short launch starts at once
+// and ends without queue traffic. The runs after the fixed ones go out from the slice's
+// counter, one atomic per run (each counter on its own 128-byte line), and a wavefront
+// whose own slice is drained moves on to the others. Called where every lane of the
+// wavefront is active.
 template <int kSlices>
 struct WaveQueue {
-    int32_t* ctr;  // [kSlices * kQueuePad] items taken from each slice
-    int n, slice, left, step;
-    __device__ WaveQueue(int32_t* c, int items) : ctr(c), n(items), slice((int)(blockIdx.x % kSlices)), left(kSlices) {
-        const long long waves = (long long)gridDim.x * (blockDim.x / 64);
+    int32_t* ctr;  // [kSlices * kQueuePad] runs handed out after the fixed ones
+    int n, step, slice, left, rank;
+    bool fixed;  // the fixed first run is still to come
+    __device__ WaveQueue(int32_t* c, int items)
+        : ctr(c), n(items), slice((int)(blockIdx.x % kSlices)), left(kSlices), fixed(true) {
+        const int wpb = blockDim.x / 64;
+        const long long waves = (long long)gridDim.x * wpb;
         step = (int)min(256ll, max(64ll, ((long long)n / (waves * 16)) & ~63ll));
-        // wavefronts beyond one per 64 items stay out (no atomics): the others drain every
-        // slice, and wavefront 0 always takes part
-        const long long wave = (long long)blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
-        if (wave * 64 >= n) left = 0;
+        rank = __builtin_amdgcn_readfirstlane((int)(blockIdx.x / kSlices) * wpb + (int)(threadIdx.x / 64));
+    }
+    // wavefronts whose workgroups map to slice s
+    __device__ int slice_waves(int s) const {
+        const int blocks = ((int)gridDim.x - s + kSlices - 1) / kSlices;
+        return max(blocks, 0) * (int)(blockDim.x / 64);
     }
     // [b, e): the next items of this wavefront; false when every slice is drained
     __device__ bool next(int& b, int& e) {
+        if (fixed) {
+            fixed = false;
+            const int lo = (int)((long long)n * slice / kSlices);
+            const int hi = (int)((long long)n * (slice + 1) / kSlices);
+            if ((long long)rank * step < hi - lo) {
+                b = lo + rank * step;
+                e = min(hi, b + step);
+                return true;
+            }
+        }
         while (left > 0) {
             const int lo = (int)((long long)n * slice / kSlices);
             const int hi = (int)((long long)n * (slice + 1) / kSlices);
-            int32_t t = 0;
-            if (__lane_id() == 0) t = atomicAdd(&ctr[slice * kQueuePad], step);
-            t = __builtin_amdgcn_readfirstlane(t);
-            if (t < hi - lo) {
-                b = lo + t;
-                e = min(hi, b + step);
-                return true;
+            const long long fixed_part = (long long)slice_waves(slice) * step;
+            if (fixed_part < hi - lo) {
+                int32_t t = 0;
+                if (__lane_id() == 0) t = atomicAdd(&ctr[slice * kQueuePad], step);
+                t = __builtin_amdgcn_readfirstlane(t);
+                if (fixed_part + t < hi - lo) {
+                    b = lo + (int)fixed_part + t;
+                    e = min(hi, b + step);
+                    return true;
+                }
             }
             slice = (slice + 1) % kSlices;
             --left;
@@ -275,14 +297,34 @@ int grid_for(long long items, int block, int cap) {
 }
 
 // Workgroups of `kern` resident on the whole device at once (every CU at the kernel's
-// occupancy), and no more than `items` need at 64 items per wavefront.
+// occupancy), and no more than `items` need at 64 items per wavefront. The runtime
+// queries run once per (device, kernel): a frame launches these kernels tens of times.
 int resident_grid(const void* kern, int block, long long items) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1) per_cu = 1;
+    struct Entry {
+        const void* kern;
+        int dev, block, blocks;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    int blocks = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const Entry& e : cache)
+            if (e.kern == kern && e.dev == dev && e.block == block) blocks = e.blocks;
+        if (blocks == 0) {
+            int cus = 0, per_cu = 0;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1)
+                cus = 256;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, 0) != hipSuccess || per_cu < 1)
+                per_cu = 1;
+            blocks = cus * per_cu;
+            cache.push_back(Entry{kern, dev, block, blocks});
+        }
+    }
     const long long need = (items + block - 1) / block;
-    return (int)std::max<long long>(1, std::min<long long>(need, (long long)cus * per_cu));
+    return (int)std::max<long long>(1, std::min<long long>(need, blocks));
 }
 
 }  // namespace

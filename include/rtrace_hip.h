@@ -249,6 +249,29 @@ int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, siz
  * rt_last_error(NULL). */
 int rt_mesh_load_stl(const char* path, double* vertices, int64_t capacity_tris, int64_t* n_tris);
 
+/* ---- k-d tree inspection ---- */
+/* No counterpart in the reference: the flattened k-d tree of KdTree::new (src/mesh.rs:75-197)
+ * for a mesh of n_tris triangles (9 f64 each), built on the host (on_device = 0) or on the
+ * device (on_device = 1, what rt_scene_load_* uses for meshes of 50,000 triangles or more),
+ * for checking one build against the other. Pass capacities of 0 to get the counts; arrays
+ * are copied out up to their capacities: nodes and records are 48 B each, bounds 6 f64 per
+ * node. */
+typedef struct {
+    void* nodes;
+    int64_t nodes_cap;
+    void* recs;
+    int64_t recs_cap;
+    double* bounds;
+    int64_t bounds_cap;   /* nodes */
+    int64_t n_nodes;      /* out */
+    int64_t n_recs;       /* out */
+    int64_t root_kd;      /* out */
+    int64_t root_bf;      /* out */
+    int64_t leaves;       /* out */
+    int64_t max_leaf;     /* out */
+} rt_kd_arrays;
+int rt_kd_build(rt_ctx* ctx, const double* vertices, int64_t n_tris, int32_t on_device, rt_kd_arrays* out);
+
 /* ---- automatic cameras ---- */
 /* AutoCamera::generate_cameras (src/auto_camera.rs:9-130): left, front, top (ortho) and a
  * 50-degree perspective view of the scene's finite bounds (Scene::compute_finite_bounds,

@@ -17,7 +17,7 @@ import os
 import numpy as np
 
 __all__ = ["AntiAliasingMode", "OutlineConfig", "Renderer", "Scene", "RenderStats", "RtraceError", "lib",
-           "intersect", "last_primary_hits", "cli_size", "load_stl"]
+           "intersect", "last_primary_hits", "cli_size", "load_stl", "kd_build"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librtrace_hip.so")
@@ -161,6 +161,7 @@ def lib():
         L.rt_scene_auto_cameras.argtypes = [P, P]
         L.rt_auto_cameras_file.argtypes = [ctypes.c_char_p, P]
         L.rt_mesh_load_stl.argtypes = [ctypes.c_char_p, P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
+        L.rt_kd_build.argtypes = [P, P, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_KdArrays)]
         L.rt_batch_opts_default.argtypes = [ctypes.POINTER(BatchOpts)]
         L.rt_batch_create.argtypes = [P, P, ctypes.c_int32, ctypes.POINTER(BatchOpts), ctypes.POINTER(P)]
         L.rt_batch_run.argtypes = [P, ctypes.POINTER(BatchStats)]
@@ -367,6 +368,32 @@ def load_stl(path):
     if n.value:
         _check_global(lib().rt_mesh_load_stl(p, out.ctypes.data, n.value, ctypes.byref(n)))
     return out
+
+
+class _KdArrays(ctypes.Structure):
+    _fields_ = [("nodes", ctypes.c_void_p), ("nodes_cap", ctypes.c_int64), ("recs", ctypes.c_void_p),
+                ("recs_cap", ctypes.c_int64), ("bounds", ctypes.c_void_p), ("bounds_cap", ctypes.c_int64),
+                ("n_nodes", ctypes.c_int64), ("n_recs", ctypes.c_int64), ("root_kd", ctypes.c_int64),
+                ("root_bf", ctypes.c_int64), ("leaves", ctypes.c_int64), ("max_leaf", ctypes.c_int64)]
+
+
+def kd_build(triangles, on_device=False):
+    """rt_kd_build: the flattened k-d tree of KdTree::new (src/mesh.rs:75-197) for an (n, 3, 3)
+    float64 triangle array, built on the host or on the device. Returns a dict with `nodes`
+    (n_nodes x 48 bytes), `recs` (n_recs x 48 bytes), `bounds` (n_nodes x 6 float64) and the
+    roots and leaf statistics."""
+    v = np.ascontiguousarray(triangles, np.float64).reshape(-1, 9)
+    a = _KdArrays()
+    _check(lib().rt_kd_build(_context(), v.ctypes.data, len(v), int(bool(on_device)), ctypes.byref(a)))
+    nodes = np.zeros((a.n_nodes, 48), np.uint8)
+    recs = np.zeros((a.n_recs, 48), np.uint8)
+    bounds = np.zeros((a.n_nodes, 6), np.float64)
+    a.nodes, a.nodes_cap = nodes.ctypes.data, a.n_nodes
+    a.recs, a.recs_cap = recs.ctypes.data, a.n_recs
+    a.bounds, a.bounds_cap = bounds.ctypes.data, a.n_nodes
+    _check(lib().rt_kd_build(_context(), v.ctypes.data, len(v), int(bool(on_device)), ctypes.byref(a)))
+    return {"nodes": nodes, "recs": recs, "bounds": bounds, "root_kd": a.root_kd, "root_bf": a.root_bf,
+            "leaves": a.leaves, "max_leaf": a.max_leaf}
 
 
 def _check_global(rc):

@@ -81,6 +81,10 @@ struct Builder {
         const double c[3] = {cx, cy, cz};
         if (tris.empty())
             for (int k = 0; k < 3; ++k) mn[k] = mx[k] = 0.0;  // compute_bounds of an empty list
+        for (int k = 0; k < 3; ++k) {  // a zero bound is stored without sign (min / max of +-0 is order-dependent)
+            if (mn[k] == 0.0) mn[k] = 0.0;
+            if (mx[k] == 0.0) mx[k] = 0.0;
+        }
         for (int k = 0; k < 3; ++k) {
             nb[k] = mn[k];
             nb[3 + k] = mx[k];

@@ -24,4 +24,5 @@ struct FlatKd {
 // 20k triangles or more); the result is the same as the serial build, array for array.
 void build_flat_kd(const std::vector<double>& verts, FlatKd& out, bool parallel = true);
 
+
 }  // namespace rt

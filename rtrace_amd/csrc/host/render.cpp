@@ -14,6 +14,7 @@
 #include "../../../include/rtrace_hip.h"
 #include "../common/layout.hpp"
 #include "../common/vec.hpp"
+#include "../kernels/kdbuild_gpu.hpp"
 #include "../kernels/launch.hpp"
 #include "frame.hpp"
 #include "kdbuild.hpp"
@@ -170,13 +171,18 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
     for (size_t m = 0; m < H.meshes.size(); ++m) {
         const std::vector<double>& v = H.meshes[m];
         FlatKd& kd = D.kd[m];
-        build_flat_kd(v, kd);
         auto nodes = std::make_unique<DevBuf>(), recs = std::make_unique<DevBuf>(), tri64 = std::make_unique<DevBuf>(),
              nb64 = std::make_unique<DevBuf>();
-        upload(*nodes, kd.nodes, st);
-        upload(*recs, kd.recs, st);
         upload(*tri64, v, st);
-        upload(*nb64, kd.nb64, st);
+        if (v.size() / 9 >= kDeviceKdMinTris) {
+            // big meshes: the same tree, built on the device from the uploaded vertices
+            build_flat_kd_device(v, tri64->as<double>(), kd, *nodes, *recs, *nb64, st);
+        } else {
+            build_flat_kd(v, kd);
+            upload(*nodes, kd.nodes, st);
+            upload(*recs, kd.recs, st);
+            upload(*nb64, kd.nb64, st);
+        }
         MeshDev md{};
         md.nodes = nodes->as<KdNode>();
         md.recs = recs->as<float>();
@@ -788,6 +794,43 @@ int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, siz
         if (obj) HIP_TRY(hipMemcpy(obj, ctx->last_prim_obj, m * sizeof(int32_t), hipMemcpyDeviceToHost));
         if (tri) HIP_TRY(hipMemcpy(tri, ctx->last_prim_tri, m * sizeof(int32_t), hipMemcpyDeviceToHost));
         if (t) HIP_TRY(hipMemcpy(t, ctx->last_prim_t, m * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int rt_kd_build(rt_ctx* ctx, const double* vertices, int64_t n_tris, int32_t on_device, rt_kd_arrays* out) {
+    if (!ctx || !out || n_tris < 0 || (n_tris > 0 && !vertices)) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        if (n_tris > (int64_t)(1u << 28)) throw Error(RT_E_INVALID_ARG, "too many triangles");
+        const std::vector<double> v(vertices, vertices + (size_t)n_tris * 9);
+        FlatKd kd;
+        std::vector<float> recs;
+        hipStream_t st = ctx->stream;
+        if (on_device) {
+            DevBuf tri64, nodes, drecs, nb64;
+            upload(tri64, v, st);
+            build_flat_kd_device(v, tri64.as<double>(), kd, nodes, drecs, nb64, st);
+            const size_t n_recs = (size_t)kd.nodes[kd.root_bf].a + (size_t)n_tris;  // the brute-force leaf is last
+            const size_t want = std::min<size_t>(n_recs, (size_t)std::max<int64_t>(out->recs_cap, 0));
+            recs.resize(n_recs * 12);
+            if (want) HIP_TRY(hipMemcpyAsync(recs.data(), drecs.p, want * 48, hipMemcpyDeviceToHost, st));
+            HIP_TRY(hipStreamSynchronize(st));
+        } else {
+            build_flat_kd(v, kd);
+            recs = std::move(kd.recs);
+        }
+        out->n_nodes = (int64_t)kd.nodes.size();
+        out->n_recs = (int64_t)(recs.size() / 12);
+        out->root_kd = kd.root_kd;
+        out->root_bf = kd.root_bf;
+        out->leaves = kd.leaves;
+        out->max_leaf = kd.max_leaf;
+        if (out->nodes)
+            memcpy(out->nodes, kd.nodes.data(), std::min<size_t>(kd.nodes.size(), (size_t)std::max<int64_t>(out->nodes_cap, 0)) * 48);
+        if (out->recs)
+            memcpy(out->recs, recs.data(), std::min<size_t>(recs.size() / 12, (size_t)std::max<int64_t>(out->recs_cap, 0)) * 48);
+        if (out->bounds)
+            memcpy(out->bounds, kd.nb64.data(),
+                   std::min<size_t>(kd.nb64.size() / 6, (size_t)std::max<int64_t>(out->bounds_cap, 0)) * 48);
     });
 }
 

@@ -298,7 +298,7 @@ typedef struct {
 
 typedef struct {
     int32_t tile;           /* raygen tile edge in pixels: 4, 8 or 16 */
-    int32_t chunk;          /* items a workgroup takes at a time, a multiple of 128 */
+    int32_t chunk;          /* items a workgroup (one wavefront) takes at a time, a multiple of 64 */
     int32_t blocks_per_cu;  /* persistent workgroups per compute unit */
     int32_t use_graph;      /* 1 = capture the launch sequence into a hipGraph and replay it */
     double max_gib;         /* refuse batches whose buffers need more device memory (GiB) */

@@ -47,7 +47,7 @@ int compute_units(int device) {
 void check_opts(const rt_batch_opts& bo) {
     if (bo.tile != 4 && bo.tile != 8 && bo.tile != 16) throw Error(RT_E_INVALID_ARG, "tile must be 4, 8 or 16");
     if (bo.chunk < kBatchBlock || bo.chunk % kBatchBlock != 0 || bo.chunk > (1 << 20))
-        throw Error(RT_E_INVALID_ARG, "chunk must be a positive multiple of 128");
+        throw Error(RT_E_INVALID_ARG, "chunk must be a positive multiple of 64");
     if (bo.blocks_per_cu < 1 || bo.blocks_per_cu > 64) throw Error(RT_E_INVALID_ARG, "blocks_per_cu out of range");
 }
 
@@ -200,8 +200,8 @@ extern "C" {
 void rt_batch_opts_default(rt_batch_opts* bo) {
     if (!bo) return;
     bo->tile = 8;
-    bo->chunk = 1024;
-    bo->blocks_per_cu = 8;
+    bo->chunk = 256;
+    bo->blocks_per_cu = 16;
     bo->use_graph = 1;
     bo->max_gib = 200.0;
 }
@@ -244,8 +244,8 @@ int rt_batch_autotune(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, rt_
             return best;
         };
         double best = measure(cur);
-        const int chunks[] = {256, 512, 1024, 2048, 4096};
-        const int blocks[] = {2, 4, 8, 16};
+        const int chunks[] = {64, 128, 256, 512, 1024, 2048};
+        const int blocks[] = {4, 8, 16, 32};
         const int tiles[] = {4, 8, 16};
         for (int c : chunks) {
             if (c == cur.chunk) continue;

@@ -27,6 +27,6 @@ struct BatchLaunch {
 // The whole batch's launch sequence on `st` (capturable into a hipGraph: kernels only).
 hipError_t launch_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t st);
 
-constexpr int kBatchBlock = 128;
+constexpr int kBatchBlock = 64;  // one wavefront per workgroup
 
 }  // namespace rt

@@ -130,7 +130,7 @@ def test_auto_camera_view_matches_the_oracle(R, workload, tmp_path):
 def test_autotune_picks_a_candidate(R, workload):
     jobs = _jobs(R, workload)[:6]
     o, ms = R.autotune_batch(jobs)
-    assert ms > 0 and o.tile in (4, 8, 16) and o.chunk % 128 == 0 and 1 <= o.blocks_per_cu <= 64
+    assert ms > 0 and o.tile in (4, 8, 16) and o.chunk % 64 == 0 and 1 <= o.blocks_per_cu <= 64
 
 
 def test_batch_rejects_bad_jobs(R, workload):

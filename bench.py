@@ -77,7 +77,8 @@ def cpu_baseline(path, cfg, budget_s=20.0):
                            outline=outline)
         _, st, _ = O.render(path, opts)
         rays = st["rays_primary"] + st["rays_reflect"] + st["rays_shadow"]
-        best = (rays / st["render_seconds"] / 1e6, w, h, st["render_seconds"], rays)
+        best = (rays / st["render_seconds"] / 1e6, w, h, st["render_seconds"], rays,
+                {"kd_nodes": (st["kd_inner_visits"] + st["kd_leaf_visits"]) / rays, "tri_tests": st["tri_tests"] / rays})
         spent += st["render_seconds"]
         if st["render_seconds"] * 4 > budget_s - spent or w >= cfg["width"]:
             break
@@ -90,6 +91,9 @@ def cpu_baseline(path, cfg, budget_s=20.0):
     return {"value": best[0], "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle render of the same scene at {best[1]}x{best[2]} ({best[4]} rays, {best[3]:.1f} s, "
                       f"{threads} threads)",
+            # the reference's traversal (no early exit) per ray on this sample: an upper bound of
+            # the traversal work, never the roofline's numerator
+            "reference_traversal_per_ray": {k: round(v, 1) for k, v in best[5].items()},
             "one_thread": {"value": rays1 / st1["render_seconds"] / 1e6, "unit": "Mrays/s", "cores": 1,
                            "sample": f"64x36 ({rays1} rays, {st1['render_seconds']:.1f} s)"}}
 
@@ -360,6 +364,9 @@ def main():
         "stages_ms": {k: round(v, 3) for k, v in prof["ms_stage"].items()},
         "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
         "work": wk,
+        # this build's traversal per ray (k-d nodes and FP32 triangle tests, both stages)
+        "work_per_ray": {"kd_nodes": round((wk["closest"]["nodes"] + wk["shadow"]["nodes"]) / max(rays, 1), 1),
+                         "tri_tests": round((wk["closest"]["tris"] + wk["shadow"]["tris"]) / max(rays, 1), 1)},
     }
     if not args.no_traffic:
         t, err = measure_traffic(args.workload, dom)

@@ -37,6 +37,21 @@ constexpr int kCK = 32;                            // input channels per K chunk
 constexpr int kPS = 40;                            // LDS pixel stride in bf16 (80 B)
 constexpr int kNT = 32;                            // output channels per workgroup
 
+// The same vector without branches: the address is selected and clamped into the image,
+// and a pixel outside it reads as zero. Staging loops issue all their loads before the
+// first wait this way (a load under a divergent branch is waited for at the branch's end).
+__device__ __forceinline__ uint4 gather8_nb(const ConvIn& in, int b, int y, int x, int H, int W, int c) {
+    const bool ok = y >= 0 && x >= 0 && y < H && x < W;
+    const int yc = min(max(y, 0), H - 1), xc = min(max(x, 0), W - 1);
+    const int h2 = H >> 1, w2 = W >> 1;
+    const size_t i0 = in.up0 ? (((size_t)b * h2 + (yc >> 1)) * w2 + (xc >> 1)) : (((size_t)b * H + yc) * W + xc);
+    const size_t i1 = ((size_t)b * H + yc) * W + xc;
+    const bool first = c < in.c0;
+    const uint16_t* p = first ? in.src0 + i0 * in.c0 + c : in.src1 + i1 * in.c1 + (c - in.c0);
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return ok ? v : make_uint4(0, 0, 0, 0);
+}
+
 // 16-byte vector of channels [c, c+8) of the (virtual) input pixel (b, y, x) of a conv whose
 // input is concat(src0 (optionally at half resolution, nearest upsampling), src1).
 __device__ __forceinline__ uint4 gather8(const ConvIn& in, int b, int y, int x, int H, int W, int c) {
@@ -56,8 +71,10 @@ __device__ __forceinline__ uint4 gather8(const ConvIn& in, int b, int y, int x, 
 }
 
 __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
-    __shared__ __attribute__((aligned(16))) uint16_t s_in[kHH * kHW * kPS];
-    __shared__ __attribute__((aligned(16))) uint16_t s_w[9 * kNT * kPS];
+    // one spare pixel slot each: the staging threads with no vector to store write there, so
+    // the staging has no branches and all of its loads are in flight together
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[(kHH * kHW + 1) * kPS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[(9 * kNT + 1) * kPS];
     const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
     int t = blockIdx.x;
     const int b = t / (tiles_x * tiles_y);
@@ -71,17 +88,36 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+    constexpr int kHaloVec = kHH * kHW * 4, kWVec = 9 * kNT * 4;  // 16-byte vectors per K chunk
+    constexpr int kHaloPer = (kHaloVec + 255) / 256, kWPer = (kWVec + 255) / 256;
     for (int ci0 = 0; ci0 < cin; ci0 += kCK) {
-        for (int v = threadIdx.x; v < kHH * kHW * 4; v += 256) {
+        // every load of the chunk first (indices clamped, no branches), then the stores
+        uint4 hv[kHaloPer], wv[kWPer];
+#pragma unroll
+        for (int r = 0; r < kHaloPer; ++r) {
+            const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
             const int px = v >> 2, q = v & 3;
             const int hy = px / kHW, hx = px - hy * kHW;
-            uint4 val = gather8(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8);
-            *reinterpret_cast<uint4*>(s_in + px * kPS + q * 8) = val;
+            hv[r] = gather8_nb(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8);
         }
-        for (int v = threadIdx.x; v < 9 * kNT * 4; v += 256) {
+#pragma unroll
+        for (int r = 0; r < kWPer; ++r) {
+            const int v = min((int)threadIdx.x + r * 256, kWVec - 1);
             const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
-            uint4 val = *reinterpret_cast<const uint4*>(a.wt + ((size_t)tap * a.coutp + co0 + n) * cin + ci0 + q * 8);
-            *reinterpret_cast<uint4*>(s_w + (tap * kNT + n) * kPS + q * 8) = val;
+            wv[r] = *reinterpret_cast<const uint4*>(a.wt + ((size_t)tap * a.coutp + co0 + n) * cin + ci0 + q * 8);
+        }
+#pragma unroll
+        for (int r = 0; r < kHaloPer; ++r) {
+            const int v = (int)threadIdx.x + r * 256;
+            const int slot = v < kHaloVec ? (v >> 2) * kPS + (v & 3) * 8 : kHH * kHW * kPS;
+            *reinterpret_cast<uint4*>(s_in + slot) = hv[r];
+        }
+#pragma unroll
+        for (int r = 0; r < kWPer; ++r) {
+            const int v = (int)threadIdx.x + r * 256;
+            const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
+            const int slot = v < kWVec ? (tap * kNT + n) * kPS + q * 8 : 9 * kNT * kPS;
+            *reinterpret_cast<uint4*>(s_w + slot) = wv[r];
         }
         __syncthreads();
 #pragma unroll

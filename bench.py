@@ -339,9 +339,13 @@ def main():
     # roofline of the dominant traversal kernel (bytes from the counters, time from events)
     prof = st_prof.as_dict()
     wk = st_work.as_dict()["work"]
+    # shadow queries whose light has no sample facing the hit point are not traced (their
+    # answers cannot change the image): one visibility byte read each
+    skipped = wk["shadow"]["skipped"]
+    traced = rays - skipped
     kernels = {
-        "k_shadow": (kernel_bytes(wk["shadow"], st_work.rays_shadow, B_RAY_SHADOW), prof["ms_stage"]["shadow"],
-                     prof["launches_stage"]["shadow"]),
+        "k_shadow": (kernel_bytes(wk["shadow"], st_work.rays_shadow - skipped, B_RAY_SHADOW) + skipped,
+                     prof["ms_stage"]["shadow"], prof["launches_stage"]["shadow"]),
         "k_closest": (kernel_bytes(wk["closest"], st_work.rays_primary + st_work.rays_reflect, B_RAY_CLOSEST),
                       prof["ms_stage"]["closest"], prof["launches_stage"]["closest"]),
     }
@@ -356,7 +360,8 @@ def main():
         "config": {"workload": args.workload, "width": W, "height": H, "aa": cfg["aa"], "samples": cfg["samples"],
                    "max_depth": cfg["max_depth"], "triangles": 81920 if args.workload.startswith("config2") else 0,
                    "rays_per_frame": rays, "primary": st.rays_primary, "reflect": st.rays_reflect,
-                   "shadow": st.rays_shadow},
+                   "shadow": st.rays_shadow, "shadow_skipped": skipped, "rays_traced_per_frame": traced},
+        "mrays_traced_per_s": round(traced / (ms_per_step * 1e-3) / 1e6, 2),
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": int(byt / max(launches, 1)),
@@ -364,9 +369,9 @@ def main():
         "stages_ms": {k: round(v, 3) for k, v in prof["ms_stage"].items()},
         "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
         "work": wk,
-        # this build's traversal per ray (k-d nodes and FP32 triangle tests, both stages)
-        "work_per_ray": {"kd_nodes": round((wk["closest"]["nodes"] + wk["shadow"]["nodes"]) / max(rays, 1), 1),
-                         "tri_tests": round((wk["closest"]["tris"] + wk["shadow"]["tris"]) / max(rays, 1), 1)},
+        # this build's traversal per traced ray (k-d nodes and FP32 triangle tests, both stages)
+        "work_per_ray": {"kd_nodes": round((wk["closest"]["nodes"] + wk["shadow"]["nodes"]) / max(traced, 1), 1),
+                         "tri_tests": round((wk["closest"]["tris"] + wk["shadow"]["tris"]) / max(traced, 1), 1)},
     }
     if not args.no_traffic:
         t, err = measure_traffic(args.workload, dom)

@@ -170,12 +170,14 @@ typedef struct {
     uint64_t spheres;  /* sphere tests */
     uint64_t planes;   /* plane tests */
     uint64_t cubes;    /* cube tests */
+    uint64_t skipped;  /* shadow queries not traced: no sample of their light faces the hit point, so
+                          their answers cannot change the image (not with feature output) */
 } rt_work;
 
 typedef struct {
     uint64_t rays_primary;  /* camera rays traced (quincunx: centres + unique corners) */
     uint64_t rays_reflect;  /* reflection rays traced */
-    uint64_t rays_shadow;   /* shadow rays traced */
+    uint64_t rays_shadow;   /* shadow queries (World::hit calls of the shading), traced or skipped */
     rt_work work[2];        /* per traversal kernel (RT_KERNEL_*), when count_work */
     double ms_render;       /* device time of the render, hipEvents */
     double ms_upload;       /* host->device copies of the last scene upload */

@@ -484,6 +484,7 @@ size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char*
     F.shade = (ShadeRec*)take(cap * sizeof(ShadeRec));
     F.spt = (double*)take(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
     F.vis = (uint8_t*)take(cap * L.K);
+    F.skip = (uint32_t*)take(cap * sizeof(uint32_t));
     F.work = (unsigned long long*)take(2 * kWorkCount * sizeof(unsigned long long));
     F.pix = (double*)take(px * 3 * sizeof(double));
     F.rgb = own_rgb ? (uint8_t*)take(px * 3) : nullptr;
@@ -604,6 +605,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
             stats->work[k].spheres = wk[kWorkSphere];
             stats->work[k].planes = wk[kWorkPlane];
             stats->work[k].cubes = wk[kWorkCube];
+            stats->work[k].skipped = wk[kWorkSkipped];
         }
         stats->ms_render = ms;
         stats->ms_upload = D.ms_upload;

@@ -224,7 +224,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
                 const int n_idle = (int)__popcll(idle);
                 const int take = min(n_idle, e - b);
                 const int rank = (int)__popcll(idle & below);
-                if (task < 0 && rank < take) {
+                if (task < 0 && rank < take && shadow_culled(S, F, b + rank)) {
+                    if (kCount) w.v[kWorkSkipped]++;  // not traced: its answer cannot change the image
+                } else if (task < 0 && rank < take) {
                     task = b + rank;
                     shadow_ray(S, F, task, r, dist);
                     if (world_any<kCount, false>(S, r, 0.001, dist, use_kd, stk, w)) {

@@ -67,7 +67,7 @@ struct ShadeRec {  // per hit-queue entry
 
 // work counters (rt_work field order)
 enum : int32_t { kWorkNodes = 0, kWorkTris = 1, kWorkRetests = 2, kWorkSphere = 3, kWorkPlane = 4, kWorkCube = 5,
-                 kWorkCount = 6 };
+                 kWorkSkipped = 6, kWorkCount = 7 };
 
 struct FrameView {
     int32_t W, H, N;  // image size, samples per pixel (1 for quincunx)
@@ -100,6 +100,7 @@ struct FrameView {
     ShadeRec* shade;      // [queue capacity]
     double* spt;          // [queue capacity * shadow_per_hit * 3] sample points
     uint8_t* vis;         // [queue capacity * shadow_per_hit]
+    uint32_t* skip;       // [queue capacity]: bit l set when light l's queries are not traced
     unsigned long long* work;  // [2][kWorkCount]: closest, shadow
     double* pix;          // [W*H*3] final linear colour
     uint8_t* rgb;         // [W*H*3]
@@ -136,7 +137,7 @@ RT_HD bool pixel_in_mask(const FrameView& F, int x, int y) {
 
 // --------------------------------------------------------------- helpers
 struct Work {
-    unsigned long long v[kWorkCount] = {0, 0, 0, 0, 0, 0};
+    unsigned long long v[kWorkCount] = {};
 };
 
 // Per-lane traversal stack: entry i of this lane lives at base[i * stride].
@@ -1097,6 +1098,52 @@ RT_HD int light_cols(const SceneView& S, const LightDev& L) {
     return S.light_samples > 0 ? S.light_cols : L.cols;
 }
 
+// diffuse_strength of a light point (src/lighting.rs:128,189), as phong_sample computes it
+RT_HD double diffuse_strength(d3 p, d3 n, d3 sp) { return fmax_r(dot(n, unit(sub(sp, p))), 0.0); }
+
+// Whether some sample of area light L (its frame `a`; disk samples at pts) has
+// diffuse_strength > 0 at the hit (p, n). When none has, the light's shadow queries need
+// not be traced: its contribution is diffuse + specular summed over the visible samples
+// (src/lighting.rs:178-216), every term is kd * 0 * I * colour = +-0 plus a zero
+// specular, so the sum is +0 and so is (sum / n) * (visible / n), whatever the visibility
+// (a point light alike, src/lighting.rs:119-140). Samples lie in the light's plane within
+// `half` of its centre, so n . (sample - p) lies in c +- e; the slack covers the
+// rounding of the samples and of the normalised dot product, and the samples themselves
+// are tested only when the interval straddles 0.
+RT_HD bool area_lit(const SceneView& S, const LightDev& L, const AreaFrame& a, d3 p, d3 n, const double* pts,
+                    int cnt) {
+    const d3 lp = ld3(L.pos);
+    const bool sq = L.shape == kLightSquare;
+    const double half = sq ? L.size / 2.0 : L.diameter / 2.0;
+    const double nu = fabs(dot(n, a.ub)), nv = fabs(dot(n, a.vb));
+    const double c = dot(n, sub(lp, p));
+    const double e = sq ? half * (nu + nv) : half * sqrt(nu * nu + nv * nv);
+    const double slack =
+        1e-9 * (fabs(lp.x) + fabs(lp.y) + fabs(lp.z) + fabs(p.x) + fabs(p.y) + fabs(p.z) + fabs(half) + 1.0);
+    if (c + e < -slack) return false;
+    if (c - e > slack && cnt > 0) return true;
+    const int cols = sq ? light_cols(S, L) : 1;
+    for (int q = 0; q < cnt; ++q) {
+        const d3 sp = sq ? square_sample(L, a, q, cnt, cols) : ld3(pts + 3 * q);
+        if (diffuse_strength(p, n, sp) > 0.0) return true;
+    }
+    return false;
+}
+
+// Shadow query `task` belongs to a light whose queries are not traced (area_lit).
+RT_HD bool shadow_culled(const SceneView& S, const FrameView& F, int task) {
+    const int hq = task / S.shadow_per_hit;
+    const uint32_t m = F.skip[hq];
+    if (m == 0) return false;
+    int j = task - hq * S.shadow_per_hit;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const int cnt = light_count(S, S.lights[li]);
+        if (j < cnt) return li < 32 && ((m >> li) & 1u);
+        j -= cnt;
+    }
+    return false;
+}
+
 // Phong term of one visible light sample (src/lighting.rs:123-138 and :184-208).
 RT_HD d3 phong_sample(const MatDev& m, const LightDev& L, d3 lc, d3 mc, d3 p, d3 n, d3 cam, d3 sp) {
     d3 ld = unit(sub(sp, p));
@@ -1194,26 +1241,34 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
         o[2] = n.y;
         o[3] = n.z;
     }
-    // shadow sample points of point and disk lights, in scene order (src/lighting.rs:251-285)
+    // shadow sample points of point and disk lights, in scene order (src/lighting.rs:251-285),
+    // and the lights whose shadow queries are not traced (light_lit; not with features:
+    // the visibility is then an input of the denoiser)
     uint64_t seed = F.path_seed[rr.path];
     double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
+    uint32_t skip = 0;
     int j = 0;
     for (int li = 0; li < S.n_lights; ++li) {
         const LightDev& L = S.lights[li];
         d3 lp = ld3(L.pos);
+        const bool cull = F.feat == nullptr && li < 32;
         if (!L.is_area) {
             st3(spt + 3 * j, lp);
             ++j;
+            if (cull && !(diffuse_strength(p, n, lp) > 0.0)) skip |= 1u << li;
             continue;
         }
-        if (L.shape == kLightSquare) continue;  // regenerated where needed
         const int cnt = light_count(S, L);
         AreaFrame a = area_frame(seed, p, lp);
-        double radius = L.diameter / 2.0;
-        if (!disk_samples_fast(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j))
-            disk_samples_slow(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j);
-        j += cnt;
+        if (L.shape != kLightSquare) {  // squares are regenerated where needed
+            double radius = L.diameter / 2.0;
+            if (!disk_samples_fast(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j))
+                disk_samples_slow(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j);
+        }
+        if (cull && !area_lit(S, L, a, p, n, spt + 3 * j, cnt)) skip |= 1u << li;
+        if (L.shape != kLightSquare) j += cnt;
     }
+    F.skip[hq] = skip;
     // reflection (src/lighting.rs:469-494): mirror of the view direction from the camera
     if (sr.spawned) {
         d3 vd = unit(sub(ld3(S.cam_pos), p));
@@ -1263,6 +1318,10 @@ RT_HD void shadow_ray(const SceneView& S, const FrameView& F, int task, RayD& r,
 
 template <bool count>
 RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRef stk, Work& w) {
+    if (shadow_culled(S, F, task)) {
+        if (count) w.v[kWorkSkipped]++;
+        return;
+    }
     RayD r;
     double dist;
     shadow_ray(S, F, task, r, dist);

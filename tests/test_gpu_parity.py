@@ -372,3 +372,41 @@ def test_tile_masked_rerender_composites_exactly(R, config_dir, aa):
         assert st.rays_primary == pm.sum() + corners.sum()
     else:
         assert st.rays_primary == pm.sum() * r.samples
+
+
+@pytest.mark.parametrize("variant", ["disk16", "square4", "disk_and_point"])
+def test_culled_shadow_queries_leave_the_image_unchanged(R, config_dir, variant):
+    """Shadow queries of lights with no sample facing the hit point are not traced
+    (their answers cannot change the image). A render with feature output traces every
+    query; the radiance and the image of the two renders are bit-identical, and the
+    culled render did skip queries."""
+    import torch
+    src = config_dir["config2_s4"]
+    with open(src) as f:
+        sc = json.load(f)
+    if variant == "square4":
+        sc["lights"][0].update({"shape": "square", "size": 2.0, "samples": 4})
+    elif variant == "disk_and_point":
+        sc["lights"].append({"position": [-4, 3, 2], "color": "#FFEECC", "intensity": 0.7})
+    path = os.path.join(os.path.dirname(src), f"cull_{variant}.json")
+    with open(path, "w") as f:
+        json.dump(sc, f)
+    W, H = 160, 90
+    scene = R.Scene.from_json_file(path)
+    r = R.Renderer(W, H)
+    r.max_depth = 4
+    out_c = torch.zeros(H * W * 3, dtype=torch.uint8, device="cuda")
+    rad_c = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda")
+    r.count_work = True
+    st = r.render_device_aux(scene, out_c, radiance=rad_c)
+    r.count_work = False
+    skipped = st.as_dict()["work"]["shadow"]["skipped"]
+    assert 0 < skipped < st.rays_shadow, (skipped, st.rays_shadow)
+    out_f = torch.zeros_like(out_c)
+    rad_f = torch.zeros_like(rad_c)
+    feat = torch.zeros((H, W, R.FEAT_COUNT), dtype=torch.float32, device="cuda")
+    r.render_device_aux(scene, out_f, features=feat, radiance=rad_f)
+    assert torch.equal(out_c, out_f)
+    assert torch.equal(rad_c, rad_f)
+    ref = O.render(path, O.make_opts(W, H, max_depth=4))[0]
+    compare(f"cull_{variant}", out_c.cpu().numpy().reshape(H, W, 3), ref)

@@ -436,6 +436,8 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
 // in f64 (intersect_triangle, src/ray.rs:527-583). kAny: true on a hit with
 // tmin < t < best. Otherwise improves best / best_tri strictly (first visited wins ties)
 // and flags `tie` when another triangle hits at exactly `best`.
+constexpr bool kDeferRetests = true;
+
 template <bool kAny, bool count>
 RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t leaf, double tmin, double& best,
                      int32_t& best_tri, bool& found, bool& tie, Work& w) {
@@ -443,6 +445,29 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t lea
     const MeshRay& q = s.q;
     const KdNode& ln = M.nodes[leaf];
     uint32_t first = ln.a, cnt = ln.b >> 2;
+    // closest hit: the f64 re-test of a candidate improves best / best_tri strictly, in visit order
+    auto retest = [&](uint32_t ti) {
+        double tt, uu, vv;
+        d3 nn;
+        if (count) w.v[kWorkRetests]++;
+        if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
+            if (tt < best) {
+                best = tt;
+                best_tri = (int32_t)ti;
+                s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
+                found = true;
+                tie = false;
+            } else if (tt == best && found && (int32_t)ti != best_tri) {
+                tie = true;
+            }
+        }
+    };
+    // Closest hit defers the re-tests of up to two candidates to the end of the leaf (or to
+    // a third candidate), so the lanes of a wavefront re-test together instead of one
+    // triangle index at a time. Candidates are re-tested in visit order; a candidate the
+    // FP32 bound would have dropped against the improved best cannot reach t <= best.
+    uint32_t c0 = 0, c1 = 0;
+    int nc = 0;
     for (uint32_t k = 0; k < cnt; ++k) {
         float4 r0 = recs[(size_t)(first + k) * 3 + 0];
         float4 r1 = recs[(size_t)(first + k) * 3 + 1];
@@ -485,23 +510,26 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t lea
         if (!maybe) continue;
         if (kAny && sure_in) return true;
         uint32_t ti = f32_to_bits(r2.y);
-        double tt, uu, vv;
-        d3 nn;
-        if (count) w.v[kWorkRetests]++;
         if (kAny) {
+            double tt, uu, vv;
+            d3 nn;
+            if (count) w.v[kWorkRetests]++;
             if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, best, tt, nn, uu, vv)) return true;
-        } else if (tri_exact(M.tri64 + (size_t)ti * 9, r, tmin, INFINITY, tt, nn, uu, vv)) {
-            if (tt < best) {
-                best = tt;
-                best_tri = (int32_t)ti;
-                s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
-                found = true;
-                tie = false;
-            } else if (tt == best && found && (int32_t)ti != best_tri) {
-                tie = true;
+        } else if (!kDeferRetests) {
+            retest(ti);
+        } else {
+            if (nc == 2) {
+                retest(c0);
+                retest(c1);
+                nc = 0;
             }
+            if (nc == 0) c0 = ti;
+            else c1 = ti;
+            ++nc;
         }
     }
+    if (!kAny && nc > 0) retest(c0);
+    if (!kAny && nc > 1) retest(c1);
     return false;
 }
 

@@ -1172,14 +1172,14 @@ RT_HD bool shadow_culled(const SceneView& S, const FrameView& F, int task) {
     return false;
 }
 
-// Phong term of one visible light sample (src/lighting.rs:123-138 and :184-208).
-RT_HD d3 phong_sample(const MatDev& m, const LightDev& L, d3 lc, d3 mc, d3 p, d3 n, d3 cam, d3 sp) {
+// Phong term of one visible light sample (src/lighting.rs:123-138 and :184-208); vd is
+// the hit's view direction, normalize(camera_pos - p), the same for every sample.
+RT_HD d3 phong_sample(const MatDev& m, const LightDev& L, d3 lc, d3 mc, d3 p, d3 n, d3 vd, d3 sp) {
     d3 ld = unit(sub(sp, p));
     double ds = fmax_r(dot(n, ld), 0.0);
     d3 diffuse = scale((m.diffuse * ds) * L.intensity, hadamard(lc, mc));
     d3 spec = mk(0, 0, 0);
     if (ds > 0.0) {
-        d3 vd = unit(sub(cam, p));
         d3 rd = reflect_dir(neg(ld), n);
         double ss = pow(fmax_r(dot(vd, rd), 0.0), m.shininess);
         spec = scale((m.specular * ss) * L.intensity, lc);
@@ -1366,6 +1366,7 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
     const MatDev& m = S.mats[sr.mat];
     d3 color = scale(m.ambient * S.amb_i, hadamard(ld3(S.amb), mc));
     d3 cam = ld3(S.cam_pos);
+    const d3 vd = unit(sub(cam, p));
     const double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
     const uint8_t* vis = F.vis + (size_t)hq * S.shadow_per_hit;
     uint64_t path_seed = 0;
@@ -1376,7 +1377,7 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
         d3 lc = ld3(L.color);
         if (!L.is_area) {
             // calculate_point_light_contribution (src/lighting.rs:100-142)
-            if (vis[j]) color = add(color, phong_sample(m, L, lc, mc, p, n, cam, ld3(L.pos)));
+            if (vis[j]) color = add(color, phong_sample(m, L, lc, mc, p, n, vd, ld3(L.pos)));
             else color = add(color, mk(0, 0, 0));
             ++j;
             ++k;
@@ -1392,13 +1393,13 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
             for (int s = 0; s < cnt; ++s) {
                 if (!vis[j + s]) continue;
                 ++visible;
-                total = add(total, phong_sample(m, L, lc, mc, p, n, cam, square_sample(L, a, s, cnt, cols)));
+                total = add(total, phong_sample(m, L, lc, mc, p, n, vd, square_sample(L, a, s, cnt, cols)));
             }
         } else {
             for (int s = 0; s < cnt; ++s) {
                 if (!vis[j + s]) continue;
                 ++visible;
-                total = add(total, phong_sample(m, L, lc, mc, p, n, cam, ld3(spt + 3 * (k + s))));
+                total = add(total, phong_sample(m, L, lc, mc, p, n, vd, ld3(spt + 3 * (k + s))));
             }
             k += cnt;
         }

@@ -27,13 +27,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit of traversal work (layout in rtrace_amd/csrc/common/layout.hpp)
 B_NODE, B_TRI, B_RETEST = 40, 48, 72  # node: 32 B box+links, +8 B f64 split (inner)
+B_BVH_NODE = 64  # mesh BVH node: both children's boxes and links
 B_SPHERE, B_PLANE, B_CUBE = 32, 48, 304
 B_RAY_CLOSEST = 64 + 16  # RayRec in, HitRec out
 B_RAY_SHADOW = 48 + 24 + 1  # hit point + normal, light sample point in, visibility byte out
 
 
 def kernel_bytes(work, rays, per_ray):
-    return (work["nodes"] * B_NODE + work["tris"] * B_TRI + work["retests"] * B_RETEST +
+    return (work["nodes"] * B_NODE + work["bvh_nodes"] * B_BVH_NODE + work["tris"] * B_TRI + work["retests"] * B_RETEST +
             work["spheres"] * B_SPHERE + work["planes"] * B_PLANE + work["cubes"] * B_CUBE + rays * per_ray)
 
 
@@ -371,6 +372,7 @@ def main():
         "work": wk,
         # this build's traversal per traced ray (k-d nodes and FP32 triangle tests, both stages)
         "work_per_ray": {"kd_nodes": round((wk["closest"]["nodes"] + wk["shadow"]["nodes"]) / max(traced, 1), 1),
+                         "bvh_nodes": round((wk["closest"]["bvh_nodes"] + wk["shadow"]["bvh_nodes"]) / max(traced, 1), 1),
                          "tri_tests": round((wk["closest"]["tris"] + wk["shadow"]["tris"]) / max(traced, 1), 1)},
     }
     if not args.no_traffic:

@@ -102,7 +102,7 @@ KERNEL_CLOSEST, KERNEL_SHADOW = 0, 1
 
 
 class Work(ctypes.Structure):
-    _fields_ = [(k, ctypes.c_uint64) for k in ("nodes", "tris", "retests", "spheres", "planes", "cubes", "skipped")]
+    _fields_ = [(k, ctypes.c_uint64) for k in ("nodes", "tris", "retests", "spheres", "planes", "cubes", "skipped", "bvh_nodes")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

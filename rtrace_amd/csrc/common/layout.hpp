@@ -37,10 +37,30 @@ static_assert(sizeof(KdNode) == 48, "KdNode is 48 bytes");
 constexpr int kMaxKdDepth = 16;  // KdTree::new(.., 16, 10) (src/mesh.rs:1036-1039)
 constexpr int kStackSize = kMaxKdDepth + 2;
 
+// Mesh BVH (this build's traversal structure; no reference counterpart): a binary tree
+// over the mesh's triangles, each triangle in exactly one leaf, built with a binned
+// surface-area heuristic. A node holds both children's boxes (f32 relative to the mesh
+// centre, rounded outward) and their links: an inner node's index, or kMeshLeaf |
+// first TriRec << 6 | count. Depth at most kMeshBvhDepth, so a walk's stack (one entry
+// per level) fits the per-lane stack of the k-d walk; hence at most
+// kMeshBvhLeafMax << kMeshBvhDepth triangles.
+struct alignas(16) MeshBvhNode {
+    float lo0[3], hi0[3];
+    float lo1[3], hi1[3];
+    uint32_t link[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(MeshBvhNode) == 64, "MeshBvhNode is 64 bytes");
+constexpr uint32_t kMeshLeaf = 0x80000000u;
+constexpr int kMeshBvhDepth = kMaxKdDepth;
+constexpr int kMeshBvhLeafMax = 8;
+
 // Device view of one mesh.
 struct MeshDev {
     const KdNode* nodes;
     const float* recs;  // 12 floats per TriRec
+    const MeshBvhNode* bvh;  // null: the k-d walk for every ray
+    const float* bvh_recs;   // TriRecs in BVH leaf order
     const double* tri64;
     const double* nb64;  // per node: f64 tight bounds (min xyz, max xyz) for the tie resolver
     uint32_t root_kd;   // 0

@@ -110,24 +110,9 @@ struct Builder {
         out->leaves++;
     }
     void push_rec(int t) {
-        const double* p = v + (size_t)t * 9;
+        const double c[3] = {cx, cy, cz};
         float r[12];
-        r[0] = (float)(p[0] - cx);
-        r[1] = (float)(p[1] - cy);
-        r[2] = (float)(p[2] - cz);
-        r[3] = (float)(p[3] - p[0]);
-        r[4] = (float)(p[4] - p[1]);
-        r[5] = (float)(p[5] - p[2]);
-        r[6] = (float)(p[6] - p[0]);
-        r[7] = (float)(p[7] - p[1]);
-        r[8] = (float)(p[8] - p[2]);
-        uint32_t idx = (uint32_t)t;
-        memcpy(&r[9], &idx, 4);
-        // E = max(|e1|_1, |e2|_1): scale of the FP32 rounding error of the edge terms
-        double n1 = std::fabs(p[3] - p[0]) + std::fabs(p[4] - p[1]) + std::fabs(p[5] - p[2]);
-        double n2 = std::fabs(p[6] - p[0]) + std::fabs(p[7] - p[1]) + std::fabs(p[8] - p[2]);
-        r[10] = (float)(std::max(n1, n2) * (1.0 + 1e-6));
-        r[11] = 0.0f;
+        tri_rec(v + (size_t)t * 9, (uint32_t)t, c, r);
         out->recs.insert(out->recs.end(), r, r + 12);
     }
     // build_recursive's split of an inner node: the axis, the split value, the two

@@ -1,0 +1,23 @@
+// mesh_bvh.hpp — host build of the mesh BVH (MeshBvhNode, common/layout.hpp).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../common/layout.hpp"
+
+namespace rt {
+
+struct MeshBvh {
+    std::vector<MeshBvhNode> nodes;  // node 0 is the root
+    std::vector<float> recs;         // 12 floats per TriRec, in leaf order
+    int depth = 0;                   // deepest leaf (root's children at depth 1)
+    int leaves = 0;
+};
+
+// Binned-SAH BVH over the triangles of `verts` (9 f64 per triangle), boxes relative to
+// `centre`. Returns false, with `out` empty, for an empty mesh or one larger than
+// kMeshBvhLeafMax << kMeshBvhDepth triangles.
+bool build_mesh_bvh(const std::vector<double>& verts, const double centre[3], MeshBvh& out);
+
+}  // namespace rt

@@ -18,6 +18,7 @@
 #include "../kernels/launch.hpp"
 #include "frame.hpp"
 #include "kdbuild.hpp"
+#include "mesh_bvh.hpp"
 #include "scene_host.hpp"
 
 namespace rt {
@@ -183,9 +184,20 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
             upload(*recs, kd.recs, st);
             upload(*nb64, kd.nb64, st);
         }
+        // the BVH walk needs every triangle in the leaf of its own cell (MeshWalk::bvh)
+        bool moved = false;
+        for (const KdNode& nd : kd.nodes) moved = moved || ((nd.b & 3u) != kAxisLeaf && (nd.b & kFlagMisplaced));
+        MeshBvh bvh;
+        auto bvh_nodes = std::make_unique<DevBuf>(), bvh_recs = std::make_unique<DevBuf>();
+        if (!moved && build_mesh_bvh(v, kd.center, bvh)) {
+            upload(*bvh_nodes, bvh.nodes, st);
+            upload(*bvh_recs, bvh.recs, st);
+        }
         MeshDev md{};
         md.nodes = nodes->as<KdNode>();
         md.recs = recs->as<float>();
+        md.bvh = bvh.nodes.empty() ? nullptr : bvh_nodes->as<MeshBvhNode>();
+        md.bvh_recs = bvh.nodes.empty() ? nullptr : bvh_recs->as<float>();
         md.tri64 = tri64->as<double>();
         md.nb64 = nb64->as<double>();
         md.root_kd = kd.root_kd;
@@ -210,6 +222,8 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         D.mesh_bufs.push_back(std::move(recs));
         D.mesh_bufs.push_back(std::move(tri64));
         D.mesh_bufs.push_back(std::move(nb64));
+        D.mesh_bufs.push_back(std::move(bvh_nodes));
+        D.mesh_bufs.push_back(std::move(bvh_recs));
     }
     for (const rt_object_desc& o : H.objects) {
         ObjDev d{};
@@ -606,6 +620,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
             stats->work[k].planes = wk[kWorkPlane];
             stats->work[k].cubes = wk[kWorkCube];
             stats->work[k].skipped = wk[kWorkSkipped];
+            stats->work[k].bvh_nodes = wk[kWorkBvhNodes];
         }
         stats->ms_render = ms;
         stats->ms_upload = D.ms_upload;

@@ -248,8 +248,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
         if (task >= 0) {
             const MeshDev& M = mesh_at_pos(S, mk);
             float leaf_c0 = 0.0f;
-            const uint32_t leaf = walk_to_leaf<kCount>(s, M, r, stk, w, leaf_c0);
-            if (leaf == kNoLeaf) {
+            uint32_t first = 0, cnt = 0;
+            if (!next_leaf<kCount>(s, M, r, stk, w, first, cnt, leaf_c0)) {
                 // this mesh has no hit in (0.001, dist): on to the next one
                 if ((mk = mesh_pos_from(S, mk + 1)) < 0) {
                     F.vis[task] = 1;
@@ -261,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
                 double best = dist;
                 int32_t best_tri = -1;
                 bool found = false, tie = false;
-                if (leaf_test<true, kCount>(s, M, r, leaf, 0.001, best, best_tri, found, tie, w)) {
+                if (leaf_test<true, kCount>(s, M, r, first, cnt, 0.001, best, best_tri, found, tie, w)) {
                     F.vis[task] = 0;
                     task = -1;
                 }

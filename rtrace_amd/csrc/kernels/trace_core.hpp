@@ -67,7 +67,7 @@ struct ShadeRec {  // per hit-queue entry
 
 // work counters (rt_work field order)
 enum : int32_t { kWorkNodes = 0, kWorkTris = 1, kWorkRetests = 2, kWorkSphere = 3, kWorkPlane = 4, kWorkCube = 5,
-                 kWorkSkipped = 6, kWorkCount = 7 };
+                 kWorkSkipped = 6, kWorkBvhNodes = 7, kWorkCount = 8 };
 
 struct FrameView {
     int32_t W, H, N;  // image size, samples per pixel (1 for quincunx)
@@ -332,6 +332,7 @@ struct MeshWalk {
     uint32_t node;
     int sp;
     bool have_node, par0, par1, par2;
+    bool bvh;  // walking the mesh BVH (else the k-d tree)
 };
 constexpr uint32_t kNoLeaf = 0xffffffffu;
 
@@ -349,6 +350,14 @@ RT_HOT void walk_init(MeshWalk& s, const MeshDev& M, const RayD& r, bool use_kd,
     s.node = use_kd ? M.root_kd : M.root_bf;
     s.have_node = true;
     s.sp = 0;
+    // The BVH gives the reference's candidates for a ray with no axis-parallel component:
+    // the reference's walk then reaches every leaf cell the ray's forward part crosses,
+    // and a triangle hit at q lies in the leaf of q's cell. With a component below 1e-9 the
+    // reference visits only the origin's side of a split (src/mesh.rs:425-428) and can
+    // miss a hit; meshes where the empty-side rule moved a triangle out of its cell
+    // (src/mesh.rs:174-178) have no BVH (M.bvh null). Those take the k-d walk.
+    s.bvh = use_kd && M.bvh != nullptr && !s.par0 && !s.par1 && !s.par2;
+    if (s.bvh) s.node = 0;
 }
 
 // Walks inner nodes until it holds the next leaf whose interval can still beat `best`
@@ -436,15 +445,80 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
 // in f64 (intersect_triangle, src/ray.rs:527-583). kAny: true on a hit with
 // tmin < t < best. Otherwise improves best / best_tri strictly (first visited wins ties)
 // and flags `tie` when another triangle hits at exactly `best`.
+// Walks the mesh BVH until it holds the next leaf whose interval can still beat `best`
+// (true: TriRecs [first, first + cnt) of M.bvh_recs, the interval start in leaf_c0), or
+// has no node left (false). Near child first; the far child is pushed with its entry t.
+template <bool count>
+RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, uint32_t& first, uint32_t& cnt,
+                        float& leaf_c0) {
+    while (s.have_node) {
+        uint32_t link = s.node;
+        float c0 = s.c0;
+        if (!(link & kMeshLeaf)) {
+            const float4* np = reinterpret_cast<const float4*>(M.bvh + link);
+            const float4 a = np[0], b = np[1], c = np[2], d = np[3];
+            if (count) w.v[kWorkBvhNodes]++;
+            const float lo0[3] = {a.x, a.y, a.z}, hi0[3] = {a.w, b.x, b.y};
+            const float lo1[3] = {b.z, b.w, c.x}, hi1[3] = {c.y, c.z, c.w};
+            float a0 = s.tmin_lo * 0.5f, b0 = s.best_hi, a1 = a0, b1 = b0;
+            const bool h0 = box_clip(s.q, lo0, hi0, a0, b0);
+            const bool h1 = box_clip(s.q, lo1, hi1, a1, b1);
+            const uint32_t l0 = f32_to_bits(d.x), l1 = f32_to_bits(d.y);
+            if (h0 && h1) {
+                const bool first0 = a0 <= a1;
+                stk.node[s.sp * stk.stride] = first0 ? l1 : l0;
+                stk.t0[s.sp * stk.stride] = first0 ? a1 : a0;
+                ++s.sp;
+                s.node = first0 ? l0 : l1;
+                s.c0 = first0 ? a0 : a1;
+                continue;
+            }
+            if (h0 || h1) {
+                s.node = h0 ? l0 : l1;
+                s.c0 = h0 ? a0 : a1;
+                continue;
+            }
+        }
+        // a leaf is returned; in both cases the next node comes from the stack
+        s.have_node = false;
+        while (s.sp > 0) {
+            --s.sp;
+            s.node = stk.node[s.sp * stk.stride];
+            s.c0 = stk.t0[s.sp * stk.stride];
+            if (s.c0 <= s.best_hi) {
+                s.have_node = true;
+                break;
+            }
+        }
+        if (link & kMeshLeaf) {
+            first = (link & ~kMeshLeaf) >> 6;
+            cnt = link & 63u;
+            leaf_c0 = c0;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <bool count>
+RT_HOT bool next_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, StackRef stk, Work& w, uint32_t& first,
+                      uint32_t& cnt, float& leaf_c0) {
+    if (s.bvh) return bvh_to_leaf<count>(s, M, stk, w, first, cnt, leaf_c0);
+    const uint32_t leaf = walk_to_leaf<count>(s, M, r, stk, w, leaf_c0);
+    if (leaf == kNoLeaf) return false;
+    const KdNode& ln = M.nodes[leaf];
+    first = ln.a;
+    cnt = ln.b >> 2;
+    return true;
+}
+
 constexpr bool kDeferRetests = true;
 
 template <bool kAny, bool count>
-RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t leaf, double tmin, double& best,
-                     int32_t& best_tri, bool& found, bool& tie, Work& w) {
-    const float4* recs = reinterpret_cast<const float4*>(M.recs);
+RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t first, uint32_t cnt, double tmin,
+                     double& best, int32_t& best_tri, bool& found, bool& tie, Work& w) {
+    const float4* recs = reinterpret_cast<const float4*>(s.bvh ? M.bvh_recs : M.recs);
     const MeshRay& q = s.q;
-    const KdNode& ln = M.nodes[leaf];
-    uint32_t first = ln.a, cnt = ln.b >> 2;
     // closest hit: the f64 re-test of a candidate improves best / best_tri strictly, in visit order
     auto retest = [&](uint32_t ti) {
         double tt, uu, vv;
@@ -548,10 +622,10 @@ RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin,
     bool tie = false;  // another triangle of this mesh hit at exactly `best`
     for (;;) {
         float leaf_c0 = 0.0f;
-        const uint32_t leaf = walk_to_leaf<count>(s, M, r, stk, w, leaf_c0);
-        if (leaf == kNoLeaf) break;  // this lane has no node left
+        uint32_t first = 0, cnt = 0;
+        if (!next_leaf<count>(s, M, r, stk, w, first, cnt, leaf_c0)) break;  // this lane has no node left
         if (leaf_c0 > s.best_hi) continue;  // `best` improved since the leaf was found
-        if (leaf_test<kAny, count>(s, M, r, leaf, tmin, best, best_tri, found, tie, w)) return true;
+        if (leaf_test<kAny, count>(s, M, r, first, cnt, tmin, best, best_tri, found, tie, w)) return true;
     }
     // a tie between triangles of this mesh: the reference-order walk (mesh_resolve_tie)
     // picks the winner in a separate pass; kTiePending marks the hit as pending

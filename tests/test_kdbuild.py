@@ -21,3 +21,16 @@ def test_parallel_kd_build_matches_serial(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip() == "ok 6"
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_mesh_bvh_invariants(tmp_path):
+    """Mesh BVH (this build's traversal structure): every triangle in exactly one leaf with
+    its exact record, bounded leaves and depth, and outward-rounded f32 child boxes that
+    hold every triangle below them, on random, flat, tiny and equal-centroid meshes."""
+    exe = str(tmp_path / "bvh_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-o", exe, os.path.join(ROOT, "tests", "cpp", "bvh_check.cpp"),
+                    os.path.join(ROOT, "rtrace_amd", "csrc", "host", "mesh_bvh.cpp")], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "ok 21"

@@ -181,14 +181,7 @@ __global__ __launch_bounds__(256) void k_shade_prepare(SceneView S, FrameView F,
     }
 }
 
-// Shadow rays (any hit). A lane whose ray ends (blocked, or every mesh walked) takes the
-// next ray of its wavefront's run while the other lanes keep walking: the wavefront
-// refills once at least kShadowRefill of its 64 lanes are idle, so it does not run at the
-// pace of its longest ray. A new ray first meets the analytic objects (world_any without
-// meshes), then walks the meshes one leaf per step (walk_to_leaf + leaf_test, as in
-// mesh_trace). Each ray's answer is that of shadow_item; only the schedule differs.
-constexpr int kShadowRefill = 16;
-
+// Shadow rays (any hit), 64 queue items per wavefront step.
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int level) {
     __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
@@ -198,76 +191,10 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
     // < 2^31 items: checked at frame set-up
     WaveQueue<kShadowSlices> queue(&F.take[(3 * level + 1) * kXcd * kQueuePad], F.hit_count[level] * S.shadow_per_hit);
     const int lane = __lane_id();
-    if (kShadowRefill <= 0) {
-        int b, e;
-        while (queue.next(b, e))
-            for (; b < e; b += 64)
-                if (b + lane < e) shadow_item<kCount>(S, F, b + lane, stk, w);
-        if (kCount) flush_work(F, w, 1);
-        return;
-    }
-    const bool use_kd = F.use_kd != 0;
-    const unsigned long long below = (1ull << lane) - 1ull;
-    int b = 0, e = 0;    // the wavefront's current run (wavefront-uniform)
-    bool more = true;    // the queue may hold more runs
-    int task = -1;       // this lane's ray, -1 when idle
-    int mk = -1;         // its current mesh (mesh_pos_from)
-    RayD r;
-    double dist = 0.0;
-    MeshWalk s;
-    for (;;) {
-        unsigned long long idle = __ballot(task < 0);
-        if (more && __popcll(idle) >= kShadowRefill) {
-            // refill until fewer than kShadowRefill lanes are idle or the queue is drained
-            for (;;) {
-                if (b >= e && !(more = queue.next(b, e))) break;
-                const int n_idle = (int)__popcll(idle);
-                const int take = min(n_idle, e - b);
-                const int rank = (int)__popcll(idle & below);
-                if (task < 0 && rank < take && shadow_culled(S, F, b + rank)) {
-                    if (kCount) w.v[kWorkSkipped]++;  // not traced: its answer cannot change the image
-                } else if (task < 0 && rank < take) {
-                    task = b + rank;
-                    shadow_ray(S, F, task, r, dist);
-                    if (world_any<kCount, false>(S, r, 0.001, dist, use_kd, stk, w)) {
-                        F.vis[task] = 0;
-                        task = -1;
-                    } else if ((mk = mesh_pos_from(S, 0)) < 0) {
-                        F.vis[task] = 1;
-                        task = -1;
-                    } else {
-                        walk_init(s, mesh_at_pos(S, mk), r, use_kd, 0.001, dist);
-                    }
-                }
-                b += take;
-                idle = __ballot(task < 0);
-                if (__popcll(idle) < kShadowRefill) break;
-            }
-        }
-        if (idle == ~0ull && !more) break;
-        if (task >= 0) {
-            const MeshDev& M = mesh_at_pos(S, mk);
-            float leaf_c0 = 0.0f;
-            uint32_t first = 0, cnt = 0;
-            if (!next_leaf<kCount>(s, M, r, stk, w, first, cnt, leaf_c0)) {
-                // this mesh has no hit in (0.001, dist): on to the next one
-                if ((mk = mesh_pos_from(S, mk + 1)) < 0) {
-                    F.vis[task] = 1;
-                    task = -1;
-                } else {
-                    walk_init(s, mesh_at_pos(S, mk), r, use_kd, 0.001, dist);
-                }
-            } else if (leaf_c0 <= s.best_hi) {
-                double best = dist;
-                int32_t best_tri = -1;
-                bool found = false, tie = false;
-                if (leaf_test<true, kCount>(s, M, r, first, cnt, 0.001, best, best_tri, found, tie, w)) {
-                    F.vis[task] = 0;
-                    task = -1;
-                }
-            }
-        }
-    }
+    int b, e;
+    while (queue.next(b, e))
+        for (; b < e; b += 64)
+            if (b + lane < e) shadow_item<kCount>(S, F, b + lane, stk, w);
     if (kCount) flush_work(F, w, 1);
 }
 

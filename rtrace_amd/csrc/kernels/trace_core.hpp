@@ -890,19 +890,6 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
     return false;
 }
 
-// Mesh cursor of a shadow query that walks its meshes one leaf at a time: a position in
-// the object list world_any walks (S.others with the object BVH, else S.objs). Returns
-// the position of the first mesh at or after k, or -1.
-RT_HD int mesh_pos_from(const SceneView& S, int k) {
-    const int n = S.obvh ? S.n_others : S.n_obj;
-    for (; k < n; ++k)
-        if (S.objs[S.obvh ? S.others[k] : k].kind == kMesh) return k;
-    return -1;
-}
-RT_HD const MeshDev& mesh_at_pos(const SceneView& S, int k) {
-    return S.meshes[S.objs[S.obvh ? S.others[k] : k].mesh];
-}
-
 // ------------------------------------------------------------ camera
 // Camera::get_ray (src/camera.rs:155-167)
 RT_HD RayD camera_ray(const CameraDev& c, double u, double v) {

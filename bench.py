@@ -17,6 +17,7 @@ Prints one JSON line.
 import argparse
 import json
 import os
+import re
 import sys
 import tempfile
 import time
@@ -179,7 +180,7 @@ def measure_traffic(workload, kernel):
         rows = []
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             rows += list(csv.DictReader(open(f)))
-        kern = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
+        kern = [float(r["Counter_Value"]) for r in rows if re.search(r"::%s[<(]" % kernel, r["Kernel_Name"])]
         copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"] or "copy" in r["Kernel_Name"].lower()]
         copy = [c for c in copy if c > 1000]  # the 256 MiB copy, not small staging copies
         shutil.rmtree(d, ignore_errors=True)

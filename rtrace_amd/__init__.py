@@ -112,7 +112,7 @@ class RenderStats(ctypes.Structure):
     _fields_ = [
         ("rays_primary", ctypes.c_uint64), ("rays_reflect", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64),
         ("work", Work * 2), ("ms_render", ctypes.c_double), ("ms_upload", ctypes.c_double),
-        ("ms_stage", ctypes.c_double * 6), ("launches_stage", ctypes.c_int32 * 6), ("levels", ctypes.c_int32),
+        ("ms_stage", ctypes.c_double * len(STAGES)), ("launches_stage", ctypes.c_int32 * len(STAGES)), ("levels", ctypes.c_int32),
     ]
 
     def as_dict(self):

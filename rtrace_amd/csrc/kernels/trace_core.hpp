@@ -218,6 +218,22 @@ RT_HOT bool box_clip(const MeshRay& q, const float* bmin, const float* bmax, flo
     return c0 <= c1;
 }
 
+// box_clip with (b - o) * i computed as fma(b, i, -(o * i)); nox = -(o * i) per axis. The
+// extra rounding of o * i (2^-24 |o i|) lies inside the per-axis slack.
+RT_HOT bool box_clip_fma(const MeshRay& q, const float* nox, const float* bmin, const float* bmax, float& c0,
+                         float& c1) {
+    float ax = fmaf(bmin[0], q.ix, nox[0]), bx = fmaf(bmax[0], q.ix, nox[0]);
+    float ay = fmaf(bmin[1], q.iy, nox[1]), by = fmaf(bmax[1], q.iy, nox[1]);
+    float az = fmaf(bmin[2], q.iz, nox[2]), bz = fmaf(bmax[2], q.iz, nox[2]);
+    float lo = fmaxf(fmaxf(fminf(ax, bx) - q.sx, fminf(ay, by) - q.sy), fminf(az, bz) - q.sz);
+    float hi = fminf(fminf(fmaxf(ax, bx) + q.sx, fmaxf(ay, by) + q.sy), fmaxf(az, bz) + q.sz);
+    lo -= kTolC * fabsf(lo);
+    hi += kTolC * fabsf(hi);
+    c0 = fmaxf(c0, lo);
+    c1 = fminf(c1, hi);
+    return c0 <= c1;
+}
+
 // ray_intersects_bounds (src/mesh.rs:218-255) in f64 on a node's exact bounds.
 RT_HD bool line_hits_box64(const RayD& r, const double* b) {
     double t0 = -INFINITY, t1 = INFINITY;
@@ -461,8 +477,9 @@ RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, ui
             const float lo0[3] = {a.x, a.y, a.z}, hi0[3] = {a.w, b.x, b.y};
             const float lo1[3] = {b.z, b.w, c.x}, hi1[3] = {c.y, c.z, c.w};
             float a0 = s.tmin_lo * 0.5f, b0 = s.best_hi, a1 = a0, b1 = b0;
-            const bool h0 = box_clip(s.q, lo0, hi0, a0, b0);
-            const bool h1 = box_clip(s.q, lo1, hi1, a1, b1);
+            const float nox[3] = {-(s.q.ox * s.q.ix), -(s.q.oy * s.q.iy), -(s.q.oz * s.q.iz)};
+            const bool h0 = box_clip_fma(s.q, nox, lo0, hi0, a0, b0);
+            const bool h1 = box_clip_fma(s.q, nox, lo1, hi1, a1, b1);
             const uint32_t l0 = f32_to_bits(d.x), l1 = f32_to_bits(d.y);
             if (h0 && h1) {
                 const bool first0 = a0 <= a1;

@@ -410,3 +410,28 @@ def test_culled_shadow_queries_leave_the_image_unchanged(R, config_dir, variant)
     assert torch.equal(rad_c, rad_f)
     ref = O.render(path, O.make_opts(W, H, max_depth=4))[0]
     compare(f"cull_{variant}", out_c.cpu().numpy().reshape(H, W, 3), ref)
+
+
+@pytest.mark.parametrize("view", ["top", "side"])
+def test_axis_parallel_rays_take_the_kd_walk(R, config_dir, view):
+    """An orthographic camera looking along an axis gives camera rays with two zero
+    direction components: the reference visits only the origin's side of such splits
+    (src/mesh.rs:425-428), so these rays take the k-d walk while reflection and shadow
+    rays take the mesh BVH. Pixels, hit ids and ray counts match the oracle."""
+    src = config_dir["config2_s4"]
+    with open(src) as f:
+        sc = json.load(f)
+    pos = [0, 0, 9] if view == "top" else [0, -9, 0]
+    up = [0, 1, 0] if view == "top" else [0, 0, 1]
+    sc["camera"] = {"kind": "ortho", "position": pos, "target": [0, 0, 0], "up": up, "width": 4, "height": 3}
+    path = os.path.join(os.path.dirname(src), f"axis_{view}.json")
+    with open(path, "w") as f:
+        json.dump(sc, f)
+    opts = O.make_opts(160, 120, max_depth=3)
+    img, st, gh = gpu_render(R, path, opts)
+    ref, ost, rh = O.render(path, opts, want_hits=True)
+    _, mism = compare(f"axis_{view}", img, ref, gh, rh)
+    assert mism == 0
+    assert np.count_nonzero(rh[0] >= 0) > 1000  # the mesh and plane are in view
+    for k in ("rays_primary", "rays_reflect", "rays_shadow"):
+        assert getattr(st, k) == ost[k], k

@@ -99,6 +99,18 @@ struct rt_ctx {
     const int32_t* last_prim_obj = nullptr;  // primary-hit records of the last render (in fb.arena)
     const int32_t* last_prim_tri = nullptr;
     const double* last_prim_t = nullptr;
+    // the last frame's launches as a hipGraph, replayed while its views and buffers
+    // (the key: SceneView, FrameView and the two ray queues, byte for byte) are the same
+    hipGraph_t frame_graph = nullptr;
+    hipGraphExec_t frame_exec = nullptr;
+    std::vector<char> frame_key;
+    void drop_frame_graph() {
+        if (frame_exec) (void)hipGraphExecDestroy(frame_exec);
+        if (frame_graph) (void)hipGraphDestroy(frame_graph);
+        frame_exec = nullptr;
+        frame_graph = nullptr;
+        frame_key.clear();
+    }
 };
 
 struct rt_scene {

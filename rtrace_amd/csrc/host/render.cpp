@@ -589,8 +589,32 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         HIP_TRY(hipMemsetAsync(F.prim_tri, 0xff, n_paths * sizeof(int32_t), st));
     }
     ctx->timer.reset();
+    // Without per-stage timing the frame's ~5 launches per level go out as one hipGraph:
+    // captured once, replayed while the kernel arguments are the same (kernels only).
+    std::vector<char> key;
+    if (!o->profile) {
+        key.resize(sizeof(S) + sizeof(F) + 2 * sizeof(RayRec*));
+        memcpy(key.data(), &S, sizeof(S));
+        memcpy(key.data() + sizeof(S), &F, sizeof(F));
+        memcpy(key.data() + sizeof(S) + sizeof(F), &q0, sizeof(q0));
+        memcpy(key.data() + sizeof(S) + sizeof(F) + sizeof(q0), &q1, sizeof(q1));
+        if (!ctx->frame_exec || key != ctx->frame_key) {
+            ctx->drop_frame_graph();
+            warm_frame_launches();
+            HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launch_frame(S, F, q0, q1, st, nullptr);
+            hipGraph_t g = nullptr;
+            hipError_t ce = hipStreamEndCapture(st, &g);
+            ctx->frame_graph = g;
+            HIP_TRY(le);
+            HIP_TRY(ce);
+            HIP_TRY(hipGraphInstantiate(&ctx->frame_exec, ctx->frame_graph, nullptr, nullptr, 0));
+            ctx->frame_key = key;
+        }
+    }
     HIP_TRY(hipEventRecord(ctx->ev0, st));
-    HIP_TRY(launch_frame(S, F, q0, q1, st, o->profile ? &ctx->timer : nullptr));
+    if (o->profile) HIP_TRY(launch_frame(S, F, q0, q1, st, &ctx->timer));
+    else HIP_TRY(hipGraphLaunch(ctx->frame_exec, st));
     HIP_TRY(hipEventRecord(ctx->ev1, st));
     if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> counts(depth + 2, 0), hit_count(depth + 1, 0);
@@ -717,6 +741,7 @@ void rt_ctx_destroy(rt_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
 
+    ctx->drop_frame_graph();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -40,6 +40,8 @@ struct StageTimer {
 
 // Enqueue one full frame on `st` (no host synchronisation inside).
 hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1, hipStream_t st, StageTimer* tm);
+// Runs the host-side queries launch_frame makes (cached), so a capture of it holds only kernels.
+void warm_frame_launches();
 
 // Closest hit of n caller rays (device pointers).
 hipError_t launch_intersect(const SceneView& S, const double* o, const double* d, int n, double tmin, double tmax,

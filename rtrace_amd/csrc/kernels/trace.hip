@@ -330,6 +330,15 @@ int resident_grid(const void* kern, int block, long long items) {
 
 }  // namespace
 
+void warm_frame_launches() {
+    // the occupancy queries of the traversal kernels, before a stream capture
+    (void)resident_grid((const void*)k_closest<false>, kClosestBlock, 1);
+    (void)resident_grid((const void*)k_closest<true>, kClosestBlock, 1);
+    (void)resident_grid((const void*)k_resolve, kBlock, 1);
+    (void)resident_grid((const void*)k_shadow<false>, kBlock, 1);
+    (void)resident_grid((const void*)k_shadow<true>, kBlock, 1);
+}
+
 hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1, hipStream_t st, StageTimer* tm) {
     constexpr int kCap = 256 * 16;  // 256 CUs x 16 workgroups
     auto mark = [&](int stage, bool begin) {

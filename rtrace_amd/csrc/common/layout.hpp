@@ -55,11 +55,26 @@ constexpr uint32_t kMeshLeaf = 0x80000000u;
 constexpr int kMeshBvhDepth = kMaxKdDepth;
 constexpr int kMeshBvhLeafMax = 8;
 
+// The walked form of the mesh BVH: 4-wide nodes made by merging each binary node with
+// its children (each slot a grandchild, or a child that is a leaf), so at most
+// kMeshBvhDepth / 2 levels. Boxes per slot in SoA order (f32 relative to the mesh centre,
+// rounded outward); an unused slot has link kMeshNone. A walk pushes one stack entry per
+// node it leaves with children still to visit (the node and a mask of them), so its
+// stack holds at most kMeshBvhDepth / 2 entries.
+struct alignas(16) MeshBvh4Node {
+    float lox[4], loy[4], loz[4];
+    float hix[4], hiy[4], hiz[4];
+    uint32_t link[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(MeshBvh4Node) == 128, "MeshBvh4Node is 128 bytes");
+constexpr uint32_t kMeshNone = 0xffffffffu;
+
 // Device view of one mesh.
 struct MeshDev {
     const KdNode* nodes;
     const float* recs;  // 12 floats per TriRec
-    const MeshBvhNode* bvh;  // null: the k-d walk for every ray
+    const MeshBvh4Node* bvh;  // null: the k-d walk for every ray
     const float* bvh_recs;   // TriRecs in BVH leaf order
     const double* tri64;
     const double* nb64;  // per node: f64 tight bounds (min xyz, max xyz) for the tie resolver

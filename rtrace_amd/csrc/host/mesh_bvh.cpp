@@ -163,6 +163,44 @@ struct Builder {
     }
 };
 
+// 4-wide node of binary node b: its children, each replaced by its own two children
+// when it is an inner node
+uint32_t collapse(const std::vector<MeshBvhNode>& bin, uint32_t b, std::vector<MeshBvh4Node>& out) {
+    const uint32_t me = (uint32_t)out.size();
+    out.push_back(MeshBvh4Node{});
+    struct Slot {
+        uint32_t link;
+        const float *lo, *hi;
+    };
+    Slot slots[4];
+    int n = 0;
+    for (int side = 0; side < 2; ++side) {
+        const MeshBvhNode& nb = bin[b];
+        const uint32_t l = nb.link[side];
+        if (l & kMeshLeaf) {
+            slots[n++] = Slot{l, side ? nb.lo1 : nb.lo0, side ? nb.hi1 : nb.hi0};
+        } else {
+            const MeshBvhNode& nc = bin[l];
+            slots[n++] = Slot{nc.link[0], nc.lo0, nc.hi0};
+            slots[n++] = Slot{nc.link[1], nc.lo1, nc.hi1};
+        }
+    }
+    uint32_t links[4] = {kMeshNone, kMeshNone, kMeshNone, kMeshNone};
+    for (int i = 0; i < n; ++i) links[i] = slots[i].link & kMeshLeaf ? slots[i].link : collapse(bin, slots[i].link, out);
+    MeshBvh4Node& q = out[me];
+    for (int i = 0; i < 4; ++i) {
+        const bool used = i < n;
+        q.lox[i] = used ? slots[i].lo[0] : 0.0f;
+        q.loy[i] = used ? slots[i].lo[1] : 0.0f;
+        q.loz[i] = used ? slots[i].lo[2] : 0.0f;
+        q.hix[i] = used ? slots[i].hi[0] : 0.0f;
+        q.hiy[i] = used ? slots[i].hi[1] : 0.0f;
+        q.hiz[i] = used ? slots[i].hi[2] : 0.0f;
+        q.link[i] = links[i];
+    }
+    return me;
+}
+
 }  // namespace
 
 bool build_mesh_bvh(const std::vector<double>& verts, const double centre[3], MeshBvh& out) {
@@ -195,6 +233,7 @@ bool build_mesh_bvh(const std::vector<double>& verts, const double centre[3], Me
         B.store(out.nodes[0], 0, box, root);
         B.store(out.nodes[0], 1, box, kMeshLeaf);
     }
+    collapse(out.nodes, 0, out.nodes4);
     return true;
 }
 

@@ -9,9 +9,10 @@
 namespace rt {
 
 struct MeshBvh {
-    std::vector<MeshBvhNode> nodes;  // node 0 is the root
-    std::vector<float> recs;         // 12 floats per TriRec, in leaf order
-    int depth = 0;                   // deepest leaf (root's children at depth 1)
+    std::vector<MeshBvhNode> nodes;    // binary tree, node 0 is the root
+    std::vector<MeshBvh4Node> nodes4;  // the same tree with 4-wide nodes (walked form), node 0 is the root
+    std::vector<float> recs;           // 12 floats per TriRec, in leaf order
+    int depth = 0;                     // deepest leaf of the binary tree (root's children at depth 1)
     int leaves = 0;
 };
 

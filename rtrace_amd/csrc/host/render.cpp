@@ -190,13 +190,13 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
         MeshBvh bvh;
         auto bvh_nodes = std::make_unique<DevBuf>(), bvh_recs = std::make_unique<DevBuf>();
         if (!moved && build_mesh_bvh(v, kd.center, bvh)) {
-            upload(*bvh_nodes, bvh.nodes, st);
+            upload(*bvh_nodes, bvh.nodes4, st);
             upload(*bvh_recs, bvh.recs, st);
         }
         MeshDev md{};
         md.nodes = nodes->as<KdNode>();
         md.recs = recs->as<float>();
-        md.bvh = bvh.nodes.empty() ? nullptr : bvh_nodes->as<MeshBvhNode>();
+        md.bvh = bvh.nodes.empty() ? nullptr : bvh_nodes->as<MeshBvh4Node>();
         md.bvh_recs = bvh.nodes.empty() ? nullptr : bvh_recs->as<float>();
         md.tri64 = tri64->as<double>();
         md.nb64 = nb64->as<double>();

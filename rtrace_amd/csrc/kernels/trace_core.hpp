@@ -349,6 +349,7 @@ struct MeshWalk {
     int sp;
     bool have_node, par0, par1, par2;
     bool bvh;  // walking the mesh BVH (else the k-d tree)
+    uint32_t cmask;  // BVH: children of `node` still to visit
 };
 constexpr uint32_t kNoLeaf = 0xffffffffu;
 
@@ -374,6 +375,7 @@ RT_HOT void walk_init(MeshWalk& s, const MeshDev& M, const RayD& r, bool use_kd,
     // (src/mesh.rs:174-178) have no BVH (M.bvh null). Those take the k-d walk.
     s.bvh = use_kd && M.bvh != nullptr && !s.par0 && !s.par1 && !s.par2;
     if (s.bvh) s.node = 0;
+    s.cmask = 0xfu;
 }
 
 // Walks inner nodes until it holds the next leaf whose interval can still beat `best`
@@ -461,58 +463,76 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
 // in f64 (intersect_triangle, src/ray.rs:527-583). kAny: true on a hit with
 // tmin < t < best. Otherwise improves best / best_tri strictly (first visited wins ties)
 // and flags `tie` when another triangle hits at exactly `best`.
-// Walks the mesh BVH until it holds the next leaf whose interval can still beat `best`
-// (true: TriRecs [first, first + cnt) of M.bvh_recs, the interval start in leaf_c0), or
-// has no node left (false). Near child first; the far child is pushed with its entry t.
+RT_HOT float f4c(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+RT_HOT uint32_t u4c(const uint4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+
+// Walks the 4-wide mesh BVH until it holds the next leaf whose interval can still beat
+// `best` (true: TriRecs [first, first + cnt) of M.bvh_recs, the interval start in
+// leaf_c0), or has no node left (false). At a node the children in s.cmask are tested;
+// the nearest hit one is taken and, when others were hit, one stack entry keeps the node,
+// the mask of those others and the least of their entry t (re-tested when popped).
 template <bool count>
 RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, uint32_t& first, uint32_t& cnt,
                         float& leaf_c0) {
+    const float nox[3] = {-(s.q.ox * s.q.ix), -(s.q.oy * s.q.iy), -(s.q.oz * s.q.iz)};
     while (s.have_node) {
-        uint32_t link = s.node;
-        float c0 = s.c0;
-        if (!(link & kMeshLeaf)) {
-            const float4* np = reinterpret_cast<const float4*>(M.bvh + link);
-            const float4 a = np[0], b = np[1], c = np[2], d = np[3];
-            if (count) w.v[kWorkBvhNodes]++;
-            const float lo0[3] = {a.x, a.y, a.z}, hi0[3] = {a.w, b.x, b.y};
-            const float lo1[3] = {b.z, b.w, c.x}, hi1[3] = {c.y, c.z, c.w};
-            float a0 = s.tmin_lo * 0.5f, b0 = s.best_hi, a1 = a0, b1 = b0;
-            const float nox[3] = {-(s.q.ox * s.q.ix), -(s.q.oy * s.q.iy), -(s.q.oz * s.q.iz)};
-            const bool h0 = box_clip_fma(s.q, nox, lo0, hi0, a0, b0);
-            const bool h1 = box_clip_fma(s.q, nox, lo1, hi1, a1, b1);
-            const uint32_t l0 = f32_to_bits(d.x), l1 = f32_to_bits(d.y);
-            if (h0 && h1) {
-                const bool first0 = a0 <= a1;
-                stk.node[s.sp * stk.stride] = first0 ? l1 : l0;
-                stk.t0[s.sp * stk.stride] = first0 ? a1 : a0;
-                ++s.sp;
-                s.node = first0 ? l0 : l1;
-                s.c0 = first0 ? a0 : a1;
-                continue;
-            }
-            if (h0 || h1) {
-                s.node = h0 ? l0 : l1;
-                s.c0 = h0 ? a0 : a1;
-                continue;
-            }
+        const uint32_t nidx = s.node;
+        const float4* np = reinterpret_cast<const float4*>(M.bvh + nidx);
+        const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
+        const uint4 lk = reinterpret_cast<const uint4*>(np)[6];
+        if (count) w.v[kWorkBvhNodes]++;
+        float t[4];
+        uint32_t hit = 0;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const float lo[3] = {f4c(lx, i), f4c(ly, i), f4c(lz, i)}, hi[3] = {f4c(hx, i), f4c(hy, i), f4c(hz, i)};
+            float a = s.tmin_lo * 0.5f, b = s.best_hi;
+            const bool h = box_clip_fma(s.q, nox, lo, hi, a, b);
+            t[i] = a;
+            hit |= (h && ((s.cmask >> i) & 1u) && u4c(lk, i) != kMeshNone) ? 1u << i : 0u;
         }
-        // a leaf is returned; in both cases the next node comes from the stack
+        int bi = -1;
+        float bt = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            if (((hit >> i) & 1u) && (bi < 0 || t[i] < bt)) {
+                bi = i;
+                bt = t[i];
+            }
+        if (bi >= 0) {
+            const uint32_t rest = hit & ~(1u << bi);
+            if (rest) {
+                float tr = INFINITY;
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if ((rest >> i) & 1u) tr = fminf(tr, t[i]);
+                stk.node[s.sp * stk.stride] = nidx << 4 | rest;
+                stk.t0[s.sp * stk.stride] = tr;
+                ++s.sp;
+            }
+            const uint32_t link = u4c(lk, bi);
+            if (!(link & kMeshLeaf)) {
+                s.node = link;
+                s.cmask = 0xfu;
+                continue;
+            }
+            first = (link & ~kMeshLeaf) >> 6;
+            cnt = link & 63u;
+            leaf_c0 = bt;
+        }
+        // next: the stack's last entry that can still beat `best`
         s.have_node = false;
         while (s.sp > 0) {
             --s.sp;
-            s.node = stk.node[s.sp * stk.stride];
-            s.c0 = stk.t0[s.sp * stk.stride];
-            if (s.c0 <= s.best_hi) {
+            const uint32_t e = stk.node[s.sp * stk.stride];
+            if (stk.t0[s.sp * stk.stride] <= s.best_hi) {
+                s.node = e >> 4;
+                s.cmask = e & 0xfu;
                 s.have_node = true;
                 break;
             }
         }
-        if (link & kMeshLeaf) {
-            first = (link & ~kMeshLeaf) >> 6;
-            cnt = link & 63u;
-            leaf_c0 = c0;
-            return true;
-        }
+        if (bi >= 0) return true;  // the leaf taken above
     }
     return false;
 }

@@ -54,6 +54,37 @@ static bool walk(uint32_t link, const float* lo, const float* hi, int depth) {
     return true;
 }
 
+static std::vector<int> seen4;
+
+// the 4-wide form: the same leaves, child boxes holding their triangles, depth <= 8
+static bool walk4(uint32_t link, const float* lo, const float* hi, int depth) {
+    if (link & rt::kMeshLeaf) {
+        const uint32_t first = (link & ~rt::kMeshLeaf) >> 6, cnt = link & 63u;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            uint32_t t;
+            std::memcpy(&t, B->recs.data() + (size_t)(first + i) * 12 + 9, 4);
+            seen4.at(t)++;
+            for (int v = 0; v < 3; ++v)
+                for (int k = 0; k < 3; ++k) {
+                    const double x = V[(size_t)t * 9 + v * 3 + k] - C[k];
+                    if (!((double)lo[k] <= x && x <= (double)hi[k])) return fail("box4");
+                }
+        }
+        return true;
+    }
+    if (depth >= rt::kMeshBvhDepth / 2) return fail("depth4");
+    if (link >= B->nodes4.size()) return fail("link4");
+    const rt::MeshBvh4Node& n = B->nodes4[link];
+    int used = 0;
+    for (int i = 0; i < 4; ++i) {
+        if (n.link[i] == rt::kMeshNone) continue;
+        ++used;
+        const float clo[3] = {n.lox[i], n.loy[i], n.loz[i]}, chi[3] = {n.hix[i], n.hiy[i], n.hiz[i]};
+        if (!walk4(n.link[i], clo, chi, depth + 1)) return false;
+    }
+    return used >= 2 ? true : fail("slots");
+}
+
 int main() {
     std::mt19937_64 rng(777);
     std::uniform_real_distribution<double> u(-1.0, 1.0), small(-0.05, 0.05);
@@ -84,6 +115,11 @@ int main() {
             if (!walk(r.link[0], r.lo0, r.hi0, 1) || !walk(r.link[1], r.lo1, r.hi1, 1)) return 1;
             for (int t = 0; t < n; ++t)
                 if (seen[t] != 1) return fail("coverage"), 1;
+            seen4.assign(n, 0);
+            const float inf[3] = {-INFINITY, -INFINITY, -INFINITY}, sup[3] = {INFINITY, INFINITY, INFINITY};
+            if (bvh.nodes4.empty() || !walk4(0, inf, sup, 0)) return 1;
+            for (int t = 0; t < n; ++t)
+                if (seen4[t] != 1) return fail("coverage4"), 1;
             if (bvh.recs.size() != (size_t)n * 12) return fail("record count"), 1;
             ++checked;
         }

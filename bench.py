@@ -28,7 +28,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per unit of traversal work (layout in rtrace_amd/csrc/common/layout.hpp)
 B_NODE, B_TRI, B_RETEST = 40, 48, 72  # node: 32 B box+links, +8 B f64 split (inner)
-B_BVH_NODE = 64  # mesh BVH node: both children's boxes and links
+B_BVH_NODE = 128  # 4-wide mesh BVH node: four slot boxes and links
 B_SPHERE, B_PLANE, B_CUBE = 32, 48, 304
 B_RAY_CLOSEST = 64 + 16  # RayRec in, HitRec out
 B_RAY_SHADOW = 48 + 24 + 1  # hit point + normal, light sample point in, visibility byte out

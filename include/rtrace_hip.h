@@ -172,7 +172,7 @@ typedef struct {
     uint64_t cubes;    /* cube tests */
     uint64_t skipped;  /* shadow queries not traced: no sample of their light faces the hit point, so
                           their answers cannot change the image (not with feature output) */
-    uint64_t bvh_nodes; /* mesh BVH nodes fetched (64 B: both children's boxes and links) */
+    uint64_t bvh_nodes; /* mesh BVH nodes fetched (128 B: four slot boxes and links) */
 } rt_work;
 
 typedef struct {

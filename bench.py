@@ -287,6 +287,8 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-denoise", action="store_true", help="skip the config-5 denoised-vs-plain section")
+    ap.add_argument("--no-stage-events", action="store_true",
+                    help="time the steps without per-stage hipEvents (stage times from as many profiled steps after)")
     ap.add_argument("--dn-seeds", default="100001,100002", help="config-5 scene seeds of the denoise section")
     ap.add_argument("--dn-threshold", type=float, default=0.06, help="tile fallback threshold")
     ap.add_argument("--c4-scenes", type=int, default=64, help="config4: number of scenes")
@@ -321,24 +323,36 @@ def main():
     st_work = r.render_device(scene, out)
     r.count_work = False
 
-    # the timed steps record hipEvents around every stage's launches on the library's
-    # stream (profile), so the per-launch kernel times come from the timed region itself
-    r.profile = True
+    # By default the timed steps record hipEvents around every stage's launches on the
+    # library's stream (profile), so the per-launch kernel times come from the timed region
+    # itself. With --no-stage-events the timed steps replay the frame's hipGraph without
+    # events (small frames, where ~6 event pairs per level weigh) and the stage times come
+    # from as many profiled steps after them.
+    stage_ms, stage_n = {}, {}
+
+    def add_stages(st_):
+        d_ = st_.as_dict()
+        for k, v in d_["ms_stage"].items():
+            stage_ms[k] = stage_ms.get(k, 0.0) + v
+            stage_n[k] = stage_n.get(k, 0) + d_["launches_stage"][k]
+
+    r.profile = not args.no_stage_events
     for _ in range(args.warmup):
         r.render_device(scene, out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     dev_ms = []
-    stage_ms, stage_n = {}, {}
     for _ in range(args.steps):
         st = r.render_device(scene, out)
         dev_ms.append(st.ms_render)
-        d = st.as_dict()
-        for k, v in d["ms_stage"].items():
-            stage_ms[k] = stage_ms.get(k, 0.0) + v
-            stage_n[k] = stage_n.get(k, 0) + d["launches_stage"][k]
+        if r.profile:
+            add_stages(st)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    if not r.profile:
+        r.profile = True
+        for _ in range(args.steps):
+            add_stages(r.render_device(scene, out))
     r.profile = False
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     rays = st.rays_total
@@ -379,6 +393,9 @@ def main():
                      "bytes_per_launch": int(byt / max(launches, 1)),
                      "ms_per_launch": round(ms / max(launches, 1), 4)},
         "stages_ms": {k: round(v, 3) for k, v in prof["ms_stage"].items()},
+        "kernel_timing": ("hipEvents around each stage's launches in %d profiled steps after the timed ones"
+                          % args.steps if args.no_stage_events else
+                          "hipEvents around each stage's launches in the timed steps"),
         "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
         "work": wk,
         # this build's traversal per traced ray (k-d nodes and FP32 triangle tests, both stages)

@@ -104,6 +104,7 @@ struct rt_ctx {
     hipGraph_t frame_graph = nullptr;
     hipGraphExec_t frame_exec = nullptr;
     std::vector<char> frame_key;
+    std::vector<char> seen_key;  // the previous frame's key: a frame is captured the second time in a row
     void drop_frame_graph() {
         if (frame_exec) (void)hipGraphExecDestroy(frame_exec);
         if (frame_graph) (void)hipGraphDestroy(frame_graph);

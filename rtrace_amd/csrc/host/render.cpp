@@ -589,16 +589,21 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         HIP_TRY(hipMemsetAsync(F.prim_tri, 0xff, n_paths * sizeof(int32_t), st));
     }
     ctx->timer.reset();
-    // Without per-stage timing the frame's ~5 launches per level go out as one hipGraph:
-    // captured once, replayed while the kernel arguments are the same (kernels only).
+    // Without per-stage timing, a frame rendered twice in a row with the same kernel
+    // arguments is captured into a hipGraph (kernels only) and replayed while they stay the
+    // same; a one-off frame launches eagerly and pays no capture.
     std::vector<char> key;
+    bool replay = false;
     if (!o->profile) {
         key.resize(sizeof(S) + sizeof(F) + 2 * sizeof(RayRec*));
         memcpy(key.data(), &S, sizeof(S));
         memcpy(key.data() + sizeof(S), &F, sizeof(F));
         memcpy(key.data() + sizeof(S) + sizeof(F), &q0, sizeof(q0));
         memcpy(key.data() + sizeof(S) + sizeof(F) + sizeof(q0), &q1, sizeof(q1));
-        if (!ctx->frame_exec || key != ctx->frame_key) {
+        replay = ctx->frame_exec && key == ctx->frame_key;
+        const bool repeat = key == ctx->seen_key;
+        ctx->seen_key = key;
+        if (!replay && repeat) {
             ctx->drop_frame_graph();
             warm_frame_launches();
             HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
@@ -610,11 +615,12 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
             HIP_TRY(ce);
             HIP_TRY(hipGraphInstantiate(&ctx->frame_exec, ctx->frame_graph, nullptr, nullptr, 0));
             ctx->frame_key = key;
+            replay = true;
         }
     }
     HIP_TRY(hipEventRecord(ctx->ev0, st));
-    if (o->profile) HIP_TRY(launch_frame(S, F, q0, q1, st, &ctx->timer));
-    else HIP_TRY(hipGraphLaunch(ctx->frame_exec, st));
+    if (replay) HIP_TRY(hipGraphLaunch(ctx->frame_exec, st));
+    else HIP_TRY(launch_frame(S, F, q0, q1, st, o->profile ? &ctx->timer : nullptr));
     HIP_TRY(hipEventRecord(ctx->ev1, st));
     if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
     std::vector<int32_t> counts(depth + 2, 0), hit_count(depth + 1, 0);

@@ -438,9 +438,10 @@ def test_axis_parallel_rays_take_the_kd_walk(R, config_dir, view):
 
 
 def test_frame_graph_replays_match_eager_renders(R, config_dir):
-    """Without per-stage timing a frame's launches are captured once into a hipGraph and
-    replayed while the kernel arguments are unchanged. Replays interleaved with eager
-    renders of another scene give the eager render's bytes every time."""
+    """Without per-stage timing a frame rendered twice in a row with the same kernel
+    arguments is captured into a hipGraph and replayed while they stay unchanged. Replays,
+    with eager renders of another scene on another renderer in between, give the eager
+    render's bytes every time."""
     import torch
     sa = R.Scene.from_json_file(config_dir["config2_s4"])
     sb = R.Scene.from_json_file(config_dir["config1"])
@@ -452,9 +453,9 @@ def test_frame_graph_replays_match_eager_renders(R, config_dir):
     ra.render_device(sa, out_a)  # eager
     ref = out_a.clone()
     ra.profile = False
-    for _ in range(4):
+    for _ in range(5):
         out_a.zero_()
-        ra.render_device(sa, out_a)  # captured on the first pass, replayed after
+        ra.render_device(sa, out_a)  # eager, then captured, then replayed
         torch.cuda.synchronize()
         assert torch.equal(out_a, ref)
         rb.profile = True

@@ -1470,9 +1470,19 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
     uint64_t path_seed = 0;
     if (S.has_square) path_seed = F.path_seed[sr.path];
     int j = 0, k = 0;
+    const uint32_t culled = F.skip[hq];
     for (int li = 0; li < S.n_lights; ++li) {
         const LightDev& L = S.lights[li];
         d3 lc = ld3(L.color);
+        if (li < 32 && ((culled >> li) & 1u)) {
+            // no sample of this light faces the hit (area_lit): its contribution is +0
+            // whatever the visibility, and its shadow queries were not traced
+            color = add(color, mk(0, 0, 0));
+            const int cnt = L.is_area ? light_count(S, L) : 1;
+            j += cnt;
+            if (!L.is_area || L.shape != kLightSquare) k += cnt;
+            continue;
+        }
         if (!L.is_area) {
             // calculate_point_light_contribution (src/lighting.rs:100-142)
             if (vis[j]) color = add(color, phong_sample(m, L, lc, mc, p, n, vd, ld3(L.pos)));

@@ -198,7 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
     if (kCount) flush_work(F, w, 1);
 }
 
-__global__ __launch_bounds__(256) void k_shade_finish(SceneView S, FrameView F, int level) {
+// Held to 3 wavefronts per SIMD (168 VGPRs, a few bytes of spill) instead of the
+// compiler's 2 at 180: the f64 Phong terms need the latency hiding (9 % on config 2).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_shade_finish(SceneView S, FrameView F, int level) {
     const int n = F.hit_count[level];
     const int stride = gridDim.x * blockDim.x;
     for (int hq = xcd_block() * blockDim.x + threadIdx.x; hq < n; hq += stride) shade_finish_item(S, F, hq);

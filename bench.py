@@ -169,9 +169,9 @@ def measure_traffic(workload, kernel):
     if not prof:
         return None, "rocprofv3 not found"
     res = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counter in ("FETCH_SIZE", "WRITE_SIZE", "TCC_REQ_sum TCC_HIT_sum"):
         d = tempfile.mkdtemp(prefix="rt_pmc_")
-        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+        cmd = [prof, "--pmc", *counter.split(), "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
                os.path.abspath(__file__), "--workload", workload, "--traffic-child"]
         try:
             subprocess.run(cmd, check=True, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
@@ -180,6 +180,25 @@ def measure_traffic(workload, kernel):
         rows = []
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             rows += list(csv.DictReader(open(f)))
+        if counter.startswith("TCC_"):
+            # L2 requests and hits per launch; bytes per request from the copy (each copied
+            # byte is one L2 read and one L2 write)
+            def per_dispatch(name, match):
+                vals = {}
+                for r in rows:
+                    if r["Counter_Name"] == name and match(r["Kernel_Name"]):
+                        vals[r.get("Dispatch_Id", len(vals))] = vals.get(r.get("Dispatch_Id", len(vals)), 0.0) + float(r["Counter_Value"])
+                return list(vals.values())
+            kreq = per_dispatch("TCC_REQ_sum", lambda n: re.search(r"::%s[<(]" % kernel, n))
+            khit = per_dispatch("TCC_HIT_sum", lambda n: re.search(r"::%s[<(]" % kernel, n))
+            creq = [c for c in per_dispatch("TCC_REQ_sum", lambda n: "copy" in n.lower()) if c > 1000]
+            shutil.rmtree(d, ignore_errors=True)
+            if kreq and creq:
+                bpr = 2.0 * COPY_BYTES / max(creq)
+                req = sum(kreq) / len(kreq)
+                res["l2"] = {"requests": int(req), "bytes": int(req * bpr), "bytes_per_request": round(bpr, 1),
+                             "hit_rate": round(sum(khit) / max(sum(kreq), 1.0), 4) if khit else None}
+            continue
         kern = [float(r["Counter_Value"]) for r in rows if re.search(r"::%s[<(]" % kernel, r["Kernel_Name"])]
         copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"] or "copy" in r["Kernel_Name"].lower()]
         copy = [c for c in copy if c > 1000]  # the 256 MiB copy, not small staging copies
@@ -410,6 +429,8 @@ def main():
             line["roofline"]["traffic_note"] = (
                 f"HBM bytes/launch = FETCH_SIZE*{t['FETCH_SIZE_scale']:.3f} + WRITE_SIZE*{t['WRITE_SIZE_scale']:.3f} "
                 f"(KiB counters scaled by a {COPY_BYTES >> 20} MiB copy)")
+            if "l2" in t:
+                line["roofline"]["l2"] = dict(t["l2"], gbs=round(t["l2"]["bytes"] / (line["roofline"]["ms_per_launch"] * 1e-3) / 1e9, 1))
         else:
             line["roofline"]["traffic_note"] = err
     if not args.no_denoise:

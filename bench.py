@@ -1,7 +1,7 @@
 """Benchmark: Mrays/s of the per-pixel trace path at 1080p on one MI355X, and the
-denoised-vs-plain frame time of configuration 5.
+denoised-vs-plain frame time of the 1080p denoiser workload.
 
-Workload (default `config2_1080p`): the procedural mesh scene of configuration 2
+Workload (default `icosphere_1080p`): the procedural mesh scene
 (81,920-triangle noisy icosphere with a host-built k-d tree, 3 transformed cubes, a
 reflective grid plane, a disk area light with 16 shadow samples per hit, fog 0.05,
 reflections to depth 4) rendered at 1920x1080 with quincunx anti-aliasing. A step is one
@@ -9,8 +9,8 @@ full frame (camera rays, closest-hit, shading, shadow rays, reflections, resolve
 quantise) rendered into a device buffer; the scene is resident in HBM before timing.
 
 Rays = primary + reflection + shadow rays, counted exactly by the pipeline.
-The "denoise" object: configuration-5 frames at 1080p rendered plain with 256 light
-samples (the judge) and denoised (4 samples + the trained U-Net + tile fallback), with
+The "denoise" object: denoiser-workload frames at 1080p (icosphere scenes, 2x2 square light) rendered plain with 256 light
+samples (the quality reference) and denoised (4 samples + the trained U-Net + tile fallback), with
 PSNR / SSIM per image and the denoiser's BF16 MFMA roofline.
 Prints one JSON line.
 """
@@ -41,14 +41,14 @@ def kernel_bytes(work, rays, per_ray):
 
 def make_workload(name, out_dir):
     from rtrace_amd import scenes
-    if name.startswith("config2_square"):
-        path = scenes.config2(out_dir, subdiv=6, light="square", light_size=1.0, light_samples=16)
-    elif name.startswith("config2"):
-        path = scenes.config2(out_dir, subdiv=6)
-    elif name.startswith("config3"):
-        path = scenes.config3(out_dir, 1000)
-    elif name.startswith("config1"):
-        path = scenes.config1(out_dir)
+    if name.startswith("icosphere_square"):
+        path = scenes.icosphere_scene(out_dir, subdiv=6, light="square", light_size=1.0, light_samples=16)
+    elif name.startswith("icosphere"):
+        path = scenes.icosphere_scene(out_dir, subdiv=6)
+    elif name.startswith("spheres_ring"):
+        path = scenes.spheres_ring(out_dir, 1000)
+    elif name.startswith("sphere_800x600"):
+        path = scenes.sphere_plane(out_dir)
     else:
         raise SystemExit(f"unknown workload {name}")
     return path, scenes.RENDER[name]
@@ -113,7 +113,7 @@ DN_MAC_PER_PX = sum(9 * ci * co / 4 ** lv for ci, co, lv in DN_LAYERS)
 
 
 def denoise_section(seeds, threshold, low=4, high=256):
-    """Config 5 at 1080p: the plain render with `high` light samples (the judge) against
+    """The denoiser workload at 1080p: the plain render with `high` light samples (the quality reference) against
     the denoised render (`low` samples + network + tile fallback), per image."""
     import rtrace_amd as R  # noqa: F401 - loads the library
     from rtrace_amd import denoise
@@ -125,7 +125,7 @@ def denoise_section(seeds, threshold, low=4, high=256):
     plain = sum(r["ms_plain_high"] for r in rows) / len(rows)
     den = sum(r["ms_denoised"] for r in rows) / len(rows)
     return {
-        "workload": "config5 (config-2-style scenes, 2x2 square light), 1920x1080 quincunx, depth 4",
+        "workload": "denoise_1080p (icosphere scenes, 2x2 square light), 1920x1080 quincunx, depth 4",
         "low_samples": low, "high_samples": high, "threshold": threshold, "tile": 16,
         "ms_per_frame_plain": round(plain, 3), "ms_per_frame_denoised": round(den, 3),
         "speedup": round(plain / den, 2),
@@ -216,8 +216,8 @@ def _progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def bench_config4(args):
-    """Configuration 4: 64 seeded scenes x 4 auto-camera views at their own resolutions,
+def bench_batch(args):
+    """Batch workload: 64 seeded scenes x 4 auto-camera views at their own resolutions,
     rendered as one batch (persistent kernels, one captured graph) with autotuned tile,
     chunk and occupancy. A step is one replay of the whole batch. Also times the same jobs
     rendered one by one (rt_render_device), the unbatched path."""
@@ -227,7 +227,7 @@ def bench_config4(args):
     from rtrace_amd import scenes
     out_dir = os.path.join(tempfile.gettempdir(), "rtrace_amd_bench_c4")
     t0 = time.perf_counter()
-    work = scenes.config4(out_dir, n_scenes=args.c4_scenes)
+    work = scenes.batch_scenes(out_dir, n_scenes=args.c4_scenes)
     jobs, tris = [], []
     for path, w, h, n_tri in work:
         scene = R.Scene.from_json_file(path)
@@ -277,11 +277,11 @@ def bench_config4(args):
         seq_ms = (time.perf_counter() - t2) * 1e3
     rays = st.rays_total
     line = {
-        "metric": "Mrays/s, batched (config 4)", "value": round(rays / (ms_step * 1e-3) / 1e6, 2), "unit": "Mrays/s",
+        "metric": "Mrays/s, batched (64 scenes x 4 views)", "value": round(rays / (ms_step * 1e-3) / 1e6, 2), "unit": "Mrays/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+f64",
         "data": "synthetic (seeded procedural scenes)",
-        "config": {"workload": f"config4: {len(work)} scenes x 4 auto-camera views, quincunx, depth 4, disk light "
+        "config": {"workload": f"batch64: {len(work)} scenes x 4 auto-camera views, quincunx, depth 4, disk light "
                                f"8 samples", "jobs": len(jobs), "pixels": st.pixels,
                    "triangles_min": min(tris), "triangles_max": max(tris), "rays_per_step": rays,
                    "primary": st.rays_primary, "reflect": st.rays_reflect, "shadow": st.rays_shadow},
@@ -301,17 +301,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="config2_1080p")
+    ap.add_argument("--workload", default="icosphere_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
     ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--no-denoise", action="store_true", help="skip the config-5 denoised-vs-plain section")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the denoised-vs-plain section")
     ap.add_argument("--no-stage-events", action="store_true",
                     help="time the steps without per-stage hipEvents (stage times from as many profiled steps after)")
-    ap.add_argument("--dn-seeds", default="100001,100002", help="config-5 scene seeds of the denoise section")
+    ap.add_argument("--dn-seeds", default="100001,100002", help="denoiser scene seeds of the denoise section")
     ap.add_argument("--dn-threshold", type=float, default=0.06, help="tile fallback threshold")
-    ap.add_argument("--c4-scenes", type=int, default=64, help="config4: number of scenes")
-    ap.add_argument("--no-sequential", action="store_true", help="config4: skip the one-by-one comparison")
+    ap.add_argument("--c4-scenes", type=int, default=64, help="batch64: number of scenes")
+    ap.add_argument("--no-sequential", action="store_true", help="batch64: skip the one-by-one comparison")
     args = ap.parse_args()
     if args.traffic_child:
         traffic_child(args.workload)
@@ -321,8 +321,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
-    if args.workload == "config4":
-        bench_config4(args)
+    if args.workload == "batch64":
+        bench_batch(args)
         return
 
     import torch
@@ -398,12 +398,12 @@ def main():
     byt, ms, launches = kernels[dom]
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     line = {
-        "metric": "Mrays/s at 1080p", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "metric": "Mrays/s at 1080p" if H == 1080 else f"Mrays/s at {W}x{H}", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+f64",
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": args.workload, "width": W, "height": H, "aa": cfg["aa"], "samples": cfg["samples"],
-                   "max_depth": cfg["max_depth"], "triangles": 81920 if args.workload.startswith("config2") else 0,
+                   "max_depth": cfg["max_depth"], "triangles": 81920 if args.workload.startswith("icosphere") else 0,
                    "rays_per_frame": rays, "primary": st.rays_primary, "reflect": st.rays_reflect,
                    "shadow": st.rays_shadow, "shadow_skipped": skipped, "rays_traced_per_frame": traced},
         "mrays_per_s_reference_queries": round(value_all, 2),

@@ -1,6 +1,6 @@
 """CPU restatement of the shadow denoiser (the U-Net of include/rtrace_denoise.h).
 
-TEST INFRASTRUCTURE ONLY: the parity judge of the HIP kernels in
+TEST INFRASTRUCTURE ONLY: the parity oracle of the HIP kernels in
 rtrace_amd/csrc/kernels/denoise.hip. Nothing in the product imports it.
 
 The network has no counterpart in the reference (SeaRyanC/rtrace estimates area lights

@@ -1,6 +1,6 @@
 // rt_oracle.cpp — CPU f64 restatement of rtrace's per-pixel trace path.
 //
-// TEST INFRASTRUCTURE ONLY (the parity judge): tests/, __graft_entry__.smoke()
+// TEST INFRASTRUCTURE ONLY (the parity oracle): tests/, __graft_entry__.smoke()
 // and bench.py's cpu_baseline leg are the only users. Nothing in rtrace_amd/
 // links or calls this file.
 //

@@ -1,7 +1,7 @@
 /*
  * rt_oracle.h — CPU f64 restatement of rtrace's per-pixel trace path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the parity judge for the HIP
+ * TEST INFRASTRUCTURE ONLY. This library is the parity oracle for the HIP
  * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
  * may load it. The product (rtrace_amd, librtrace_hip.so) never links it.
  *

@@ -19,7 +19,7 @@ namespace {
 
 constexpr int kBins = 16;
 // SAH cost of a node visit, in triangle tests (a node tests two boxes; 3 measured best
-// of 1, 2 and 3 on configs 2, 4 and 5)
+// of 1, 2 and 3 on the 1080p icosphere scene, the 64-scene batch and the denoiser scenes)
 constexpr double kTraverse = 3.0;
 
 struct Box {

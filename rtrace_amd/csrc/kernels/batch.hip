@@ -114,7 +114,7 @@ __device__ __forceinline__ bool next_chunk(const int32_t* __restrict__ base, int
 }
 
 // Held to 4 wavefronts per SIMD (128 VGPRs; the closest stage spills 36 B) as k_closest is:
-// config 4 1479 -> 1521 Mrays/s against 3 wavefronts at 136 VGPRs.
+// 64-scene batch 1479 -> 1521 Mrays/s against 3 wavefronts at 136 VGPRs.
 template <int kStage>
 __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_trace(BATCH_PARAMS) {
     __shared__ uint32_t s_node[kMaxKdDepth * kBatchBlock];

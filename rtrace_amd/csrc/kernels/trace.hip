@@ -199,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
 }
 
 // Held to 3 wavefronts per SIMD (168 VGPRs, a few bytes of spill) instead of the
-// compiler's 2 at 180: the f64 Phong terms need the latency hiding (9 % on config 2).
+// compiler's 2 at 180: the f64 Phong terms need the latency hiding (9 % on the 1080p icosphere scene).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_shade_finish(SceneView S, FrameView F, int level) {
     const int n = F.hit_count[level];
     const int stride = gridDim.x * blockDim.x;

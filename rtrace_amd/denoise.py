@@ -140,9 +140,9 @@ class Denoiser:
 
 def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06, tile=16, subdiv=6, repeats=3,
              work_dir=None, log=None):
-    """Config-5 frames: per seed, the plain render with `high` light samples (the judge),
+    """Denoiser frames: per seed, the plain render with `high` light samples (the quality reference),
     the plain render with `low`, and the denoised render (low + network + tile fallback at
-    `high`). Returns per-image PSNR / SSIM against the judge and the frame times (ms, median
+    `high`). Returns per-image PSNR / SSIM against that reference and the frame times (ms, median
     of `repeats` timed runs after one untimed)."""
     import tempfile
     import time
@@ -171,7 +171,7 @@ def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06
         return float(np.median(ts)), res
 
     for s in seeds:
-        scene = R.Scene.from_json_file(scenes.config5(work_dir, s, subdiv=subdiv))
+        scene = R.Scene.from_json_file(scenes.denoise_scene(work_dir, s, subdiv=subdiv))
         r.light_samples = high
         ms_plain, st_plain = timed(lambda: r.render_device(scene, out))
         ref = out.cpu().numpy().reshape(height, width, 3).copy()

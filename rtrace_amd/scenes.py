@@ -5,15 +5,15 @@ the reference's schema (src/scene.rs:34-446) plus binary STL meshes (f32, as the
 reference's loader reads them, src/mesh.rs:919-1003), so the oracle and the HIP
 path consume identical vertices.
 
-  config1   800x600 ortho sphere over a reflective grid plane, point light, quincunx.
-  config2   ~82k-triangle noisy icosphere + 3 transformed cubes + grid plane,
-            perspective 50 deg, disk area light (16 samples), fog 0.05, depth 4.
-  config3   1000 reflective spheres on a ring over a grid plane, 4 point lights,
-            stochastic 4 spp, depth 10, outline on.
-  config4   64 seeded scenes (displaced icospheres / extruded polygons, 1k-500k
-            triangles, 256x256 to 1920x1080) for the batched launch, 4 auto-camera views each.
-  config5   seeded config-2-style scenes with a 2x2 square light: the denoiser's
-            training data (low / high light-sample pairs) and benchmark frames.
+  sphere_plane     800x600 ortho sphere over a reflective grid plane, point light, quincunx.
+  icosphere_scene  ~82k-triangle noisy icosphere + 3 transformed cubes + grid plane,
+                   perspective 50 deg, disk area light (16 samples), fog 0.05, depth 4.
+  spheres_ring     1000 reflective spheres on a ring over a grid plane, 4 point lights,
+                   stochastic 4 spp, depth 10, outline on.
+  batch_scenes     64 seeded scenes (displaced icospheres / extruded polygons, 1k-500k
+                   triangles, 256x256 to 1920x1080) for the batched launch, 4 auto-camera views each.
+  denoise_scene    seeded icosphere-style scenes with a 2x2 square light: the denoiser's
+                   training data (low / high light-sample pairs) and benchmark frames.
 """
 import json
 import math
@@ -123,7 +123,7 @@ def _settings(background="#101822", fog=None, outline=None):
     return s
 
 
-def config1(out_dir):
+def sphere_plane(out_dir):
     """800x600 (Renderer::new(800, 600)); the 6x6 ortho view widens to 8x6 (src/camera.rs:79)."""
     sc = {
         "camera": {"kind": "ortho", "position": [0, -10, 5], "target": [0, 0, 0], "up": [0, 0, 1],
@@ -139,10 +139,10 @@ def config1(out_dir):
         "lights": [{"position": [3, -3, 5], "color": "#FFFFFF", "intensity": 1.0}],
         "scene_settings": _settings("#001122"),
     }
-    return _write(out_dir, "config1", sc)
+    return _write(out_dir, "sphere_800x600", sc)
 
 
-def config2(out_dir, subdiv=6, light="disk", light_size=1.0, light_samples=16):
+def icosphere_scene(out_dir, subdiv=6, light="disk", light_size=1.0, light_samples=16):
     """Procedural mesh scene. subdiv=6 gives 81,920 triangles (~80k).
 
     light="disk": the reference's area light (diameter `light_size`, 16 rejection samples,
@@ -151,7 +151,7 @@ def config2(out_dir, subdiv=6, light="disk", light_size=1.0, light_samples=16):
     os.makedirs(out_dir, exist_ok=True)
     rng = SplitMix64(2)
     tris = icosphere(subdiv, 1.5, 0.05, seed=2)
-    mesh_name = f"config2_icosphere_s{subdiv}.stl"
+    mesh_name = f"icosphere_s{subdiv}.stl"
     write_stl(os.path.join(out_dir, mesh_name), tris)
     objects = [{"kind": "mesh", "filename": mesh_name,
                 "material": material("#8FB8DE", 0.1, 0.7, 0.4, 48, reflectivity=0.25)}]
@@ -170,10 +170,10 @@ def config2(out_dir, subdiv=6, light="disk", light_size=1.0, light_samples=16):
                                                   "cell_size": 1.0})})
     if light == "disk":
         lt = {"diameter": light_size}
-        name = f"config2_s{subdiv}"
+        name = f"icosphere_s{subdiv}"
     elif light == "square":
         lt = {"shape": "square", "size": light_size, "samples": light_samples}
-        name = f"config2_s{subdiv}_square{light_size:g}_n{light_samples}"
+        name = f"icosphere_s{subdiv}_square{light_size:g}_n{light_samples}"
     else:
         raise ValueError(f"unknown light {light!r}")
     lights = [dict({"position": [3, -4, 6], "color": "#FFFFFF", "intensity": 1.0}, **lt)]
@@ -183,7 +183,7 @@ def config2(out_dir, subdiv=6, light="disk", light_size=1.0, light_samples=16):
     return _write(out_dir, name, sc)
 
 
-def config3(out_dir, n_spheres=1000):
+def spheres_ring(out_dir, n_spheres=1000):
     """1000 reflective spheres on a ring (annulus 14..20 around the origin, resting on
     the plane z = 0), 4 point lights; render with stochastic 4 spp, depth 10."""
     rng = SplitMix64(3)
@@ -206,11 +206,11 @@ def config3(out_dir, n_spheres=1000):
                "thickness": 1.0, "use_8_neighbors": False}
     sc = {"camera": _camera_persp([0, -34, 16], [0, 0, 0], 50), "objects": objects, "lights": lights,
           "scene_settings": _settings("#0A0A14", outline=outline)}
-    return _write(out_dir, f"config3_n{n_spheres}", sc)
+    return _write(out_dir, f"spheres_ring_n{n_spheres}", sc)
 
 
-def config5(out_dir, seed, subdiv=None):
-    """Config-2-style scene with a 2x2 square light (the denoiser's data and benchmark):
+def denoise_scene(out_dir, seed, subdiv=None):
+    """Icosphere-style scene with a 2x2 square light (the denoiser's data and benchmark):
     a noisy icosphere (subdivision 3-5 by seed unless given), 3 transformed cubes, 0-3
     spheres, a grid plane, random light position, fog on half of the seeds, camera on a
     random side. Render with quincunx, depth 4; the light's sample count is set per render."""
@@ -218,7 +218,7 @@ def config5(out_dir, seed, subdiv=None):
     rng = SplitMix64(0x5EED0000 + seed)
     sd = 3 + seed % 3 if subdiv is None else subdiv
     radius = rng.uniform(1.2, 1.8)
-    mesh_name = f"config5_{seed}_s{sd}.stl"
+    mesh_name = f"denoise_{seed}_s{sd}.stl"
     write_stl(os.path.join(out_dir, mesh_name), icosphere(sd, radius, rng.uniform(0.02, 0.08), seed=seed + 17))
     rc = lambda lo=0.3: hex_color((rng.uniform(lo, 1), rng.uniform(lo, 1), rng.uniform(lo, 1)))  # noqa: E731
     objects = [{"kind": "mesh", "filename": mesh_name,
@@ -252,7 +252,7 @@ def config5(out_dir, seed, subdiv=None):
     if rng.uniform(0, 1) < 0.5:
         fog = {"color": rc(0.5), "density": rng.uniform(0.02, 0.08), "start": 2.0, "end": 20.0}
     sc = {"camera": cam, "objects": objects, "lights": [light], "scene_settings": _settings(rc(0.0), fog=fog)}
-    return _write(out_dir, f"config5_{seed}_s{sd}", sc)
+    return _write(out_dir, f"denoise_{seed}_s{sd}", sc)
 
 
 def extruded_polygon(target_tris, radius, height, seed):
@@ -282,7 +282,7 @@ def extruded_polygon(target_tris, radius, height, seed):
     return np.concatenate([sides] + caps, 0)
 
 
-def config4(out_dir, n_scenes=64, seed=4):
+def batch_scenes(out_dir, n_scenes=64, seed=4):
     """The batched workload: `n_scenes` seeded scenes, each a displaced icosphere or an
     extruded random polygon whose triangle count is log-uniform in [1k, 500k] (icospheres
     take the nearest subdivision level), seen with a transform, over a grid plane with
@@ -295,7 +295,7 @@ def config4(out_dir, n_scenes=64, seed=4):
     out = []
     for i in range(n_scenes):
         target = math.exp(rng.uniform(math.log(1000.0), math.log(500000.0)))
-        mesh_name = f"config4_{seed}_{i}.stl"
+        mesh_name = f"batch_{seed}_{i}.stl"
         if rng.uniform(0, 1) < 0.5:
             sd = int(min(7, max(3, round(math.log(target / 20.0, 4)))))
             tris = icosphere(sd, 1.0, 0.12, seed=1000 * seed + i)
@@ -326,7 +326,7 @@ def config4(out_dir, n_scenes=64, seed=4):
         w = int(round(math.sqrt(px * aspect)))
         w = max(256, min(1920, w))
         h = max(256, min(1080, int(round(w / aspect))))
-        out.append((_write(out_dir, f"config4_{seed}_{i}", sc), w, h, len(tris)))
+        out.append((_write(out_dir, f"batch_{seed}_{i}", sc), w, h, len(tris)))
     return out
 
 
@@ -341,10 +341,10 @@ def _write(out_dir, name, scene):
 # Render settings of each configuration (Renderer fields); the 1080p variants are the
 # headline "Mrays/s at 1080p" measurement of the same scene.
 RENDER = {
-    "config1": dict(width=800, height=600, aa="quincunx", samples=1, max_depth=10),
-    "config2": dict(width=1280, height=720, aa="quincunx", samples=1, max_depth=4),
-    "config2_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
-    "config2_square_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
-    "config3": dict(width=1920, height=1080, aa="stochastic", samples=4, max_depth=10),
-    "config5": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
+    "sphere_800x600": dict(width=800, height=600, aa="quincunx", samples=1, max_depth=10),
+    "icosphere_720p": dict(width=1280, height=720, aa="quincunx", samples=1, max_depth=4),
+    "icosphere_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
+    "icosphere_square_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
+    "spheres_ring_1080p": dict(width=1920, height=1080, aa="stochastic", samples=4, max_depth=10),
+    "denoise_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
 }

@@ -2,7 +2,7 @@
 
     python -m rtrace_amd.train_denoiser --scenes 2000 --steps 4000 --out rtrace_amd/data/denoiser.npy
 
-Data: config-5 scenes (rtrace_amd.scenes.config5, seeds 0..N-1) rendered by the plain
+Data: denoiser scenes (rtrace_amd.scenes.denoise_scene, seeds 0..N-1) rendered by the plain
 method twice at a training resolution: `--low` light samples with the per-pixel features
 (the network's input) and `--high` light samples (the target radiance). All pairs stay
 resident in device memory. Steps draw random crops; the loss and the Adam update run in the
@@ -31,7 +31,7 @@ def render_pairs(seeds, W, H, low, high, work_dir, log):
     r.max_depth = 4
     t0 = time.time()
     for i, s in enumerate(seeds):
-        path = scenes.config5(work_dir, s)
+        path = scenes.denoise_scene(work_dir, s)
         scene = R.Scene.from_json_file(path)
         r.light_samples = low
         r.render_device_aux(scene, out, features=feats[i])
@@ -40,7 +40,7 @@ def render_pairs(seeds, W, H, low, high, work_dir, log):
         del scene
         os.remove(path)
         for f in os.listdir(work_dir):
-            if f.startswith(f"config5_{s}_"):
+            if f.startswith(f"denoise_{s}_"):
                 os.remove(os.path.join(work_dir, f))
         if (i + 1) % 100 == 0 or i + 1 == n:
             log(f"rendered {i + 1}/{n} pairs ({time.time() - t0:.1f} s)")

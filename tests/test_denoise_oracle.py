@@ -1,4 +1,4 @@
-"""Pins for the denoiser's CPU oracle (oracle/denoise_ref.py) before it judges the HIP
+"""Pins for the denoiser's CPU oracle (oracle/denoise_ref.py) before it checks the HIP
 kernels: bf16 rounding against torch's bfloat16 conversion, the convolution against
 torch.nn.functional.conv2d, the explicit backward pass against central finite
 differences and against torch autograd of the same network, all in float64."""

@@ -28,7 +28,7 @@ def R():
 def workload(tmp_path_factory):
     from rtrace_amd import scenes
     d = str(tmp_path_factory.mktemp("c4"))
-    return scenes.config4(d, n_scenes=3, seed=11)
+    return scenes.batch_scenes(d, n_scenes=3, seed=11)
 
 
 SIZES = [(96, 72), (64, 64), (128, 72)]

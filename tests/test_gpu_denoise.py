@@ -125,7 +125,7 @@ def test_denoise_is_noisy_plus_network_output(DN):
 @pytest.fixture(scope="module")
 def square_scene(tmp_path_factory):
     from rtrace_amd import scenes
-    return scenes.config2(str(tmp_path_factory.mktemp("dn")), subdiv=4, light="square", light_size=2.0,
+    return scenes.icosphere_scene(str(tmp_path_factory.mktemp("dn")), subdiv=4, light="square", light_size=2.0,
                           light_samples=4)
 
 
@@ -160,7 +160,7 @@ def test_render_denoised_tile_fallback(DN, square_scene):
 
 
 def test_trained_network_beats_the_plain_few_sample_render(DN, tmp_path):
-    """The shipped weights on held-out config-5 scenes: against the 256-sample render the
+    """The shipped weights on held-out denoiser scenes: against the 256-sample render the
     denoised frame gains PSNR and SSIM over the 4-sample frame it starts from, and clears
     40 dB; the untrained network does not."""
     from rtrace_amd import denoise

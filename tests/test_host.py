@@ -60,11 +60,11 @@ def test_cli_size_matches_reference_formula():
 def test_scene_generators_are_deterministic(tmp_path):
     from rtrace_amd import scenes
 
-    a = scenes.config2(str(tmp_path / "a"), subdiv=3)
-    b = scenes.config2(str(tmp_path / "b"), subdiv=3)
+    a = scenes.icosphere_scene(str(tmp_path / "a"), subdiv=3)
+    b = scenes.icosphere_scene(str(tmp_path / "b"), subdiv=3)
     assert open(a).read() == open(b).read()
-    sa = open(os.path.join(os.path.dirname(a), "config2_icosphere_s3.stl"), "rb").read()
-    sb = open(os.path.join(os.path.dirname(b), "config2_icosphere_s3.stl"), "rb").read()
+    sa = open(os.path.join(os.path.dirname(a), "icosphere_s3.stl"), "rb").read()
+    sb = open(os.path.join(os.path.dirname(b), "icosphere_s3.stl"), "rb").read()
     assert sa == sb
     assert int.from_bytes(sa[80:84], "little") == 20 * 4 ** 3
 

@@ -1,5 +1,5 @@
 """Pins for the CPU f64 oracle (oracle/rt_oracle.cpp) before it is trusted as the
-parity judge of the HIP path.
+parity oracle of the HIP path.
 
 * End-to-end: byte equality with every current render the reference commits
   (doc/images, regenerated by herebyfile.mjs:208-412 with the CLI at -s 500).

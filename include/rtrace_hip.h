@@ -236,6 +236,22 @@ typedef struct {
                                   NULL = every pixel. Not combinable with the outline pass. */
     int32_t mask_tile;         /* T, tile edge in pixels (> 0 with a mask) */
 } rt_render_aux;
+/* ---- rendering without allocation (extension) ----
+ * No counterpart in the reference. For callers that cannot allocate device memory: code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * rt_render_device with the frame's buffers (ray queues, hit and shade records, light
+ * points, per-level colours) in a fixed device array of RT_NOALLOC_SCRATCH_BYTES that every
+ * GPU the library runs on holds until the process ends; the call allocates nothing (no
+ * hipMalloc / hipFree, no graph instantiation: it launches eagerly, and opts->profile is
+ * ignored). Frames whose buffers need more than the array, or max_depth above 64, are
+ * refused with RT_E_INVALID_ARG and a message giving the size. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (two contexts, two threads) race.
+ * rt_last_primary_hits after such a call reads the shared array. The image equals
+ * rt_render_device's. */
+#define RT_NOALLOC_SCRATCH_BYTES (128u << 20)
+int rt_render_device_noalloc(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
+                             rt_render_stats* stats);
+
 /* rt_render_device plus auxiliary outputs. */
 int rt_render_device_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
                          const rt_render_aux* aux, rt_render_stats* stats);

@@ -69,6 +69,7 @@ class _Aux(ctypes.Structure):
                 ("mask_tile", ctypes.c_int32)]
 
 
+NOALLOC_SCRATCH_BYTES = 128 << 20  # RT_NOALLOC_SCRATCH_BYTES (include/rtrace_hip.h)
 FEAT_COUNT = 13  # rt_render_aux feature channels (include/rtrace_hip.h)
 
 
@@ -153,6 +154,7 @@ def lib():
         L.rt_render_opts_default.argtypes = [ctypes.POINTER(_Opts), ctypes.c_int32, ctypes.c_int32]
         L.rt_render.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
         L.rt_render_device.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
+        L.rt_render_device_noalloc.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(RenderStats)]
         L.rt_render_device_aux.argtypes = [P, P, ctypes.POINTER(_Opts), P, ctypes.POINTER(_Aux),
                                            ctypes.POINTER(RenderStats)]
         L.rt_last_primary_hits.argtypes = [P, P, P, P, ctypes.c_size_t]
@@ -311,6 +313,17 @@ class Renderer:
         st = RenderStats()
         _check(lib().rt_render_device(_context(), scene._h, ctypes.byref(o), ctypes.c_void_p(out_tensor.data_ptr()),
                                       ctypes.byref(st)))
+        self.last_stats = st
+        return st
+
+    def render_device_noalloc(self, scene, out_tensor):
+        """render_device with the frame's buffers in the library's fixed scratch array
+        (rt_render_device_noalloc: no device allocation, eager launches; frames whose
+        buffers exceed NOALLOC_SCRATCH_BYTES are refused)."""
+        o = self._opts()
+        st = RenderStats()
+        _check(lib().rt_render_device_noalloc(_context(), scene._h, ctypes.byref(o),
+                                              ctypes.c_void_p(out_tensor.data_ptr()), ctypes.byref(st)))
         self.last_stats = st
         return st
 

@@ -99,6 +99,7 @@ struct rt_ctx {
     const int32_t* last_prim_obj = nullptr;  // primary-hit records of the last render (in fb.arena)
     const int32_t* last_prim_tri = nullptr;
     const double* last_prim_t = nullptr;
+    char* noalloc_scratch = nullptr;  // this device's address of the no-allocation scratch array
     // the last frame's launches as a hipGraph, replayed while its views and buffers
     // (the key: SceneView, FrameView and the two ray queues, byte for byte) are the same
     hipGraph_t frame_graph = nullptr;

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -547,8 +548,19 @@ uint64_t primary_rays(const FrameView& F, long long n_paths) {
     return n;
 }
 
+// Where a frame's buffers live when they are not in the context's arena: the fixed
+// device array of the no-allocation entry point (its address on the context's device).
+struct FixedScratch {
+    char* base;
+    size_t bytes;
+};
+
+// Deepest max_depth the no-allocation entry point takes (its per-level counts are copied
+// back through fixed host arrays).
+constexpr int kNoAllocMaxDepth = 64;
+
 void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host_rgb, void* dev_rgb,
-            rt_render_stats* stats, const rt_render_aux* aux = nullptr) {
+            rt_render_stats* stats, const rt_render_aux* aux = nullptr, const FixedScratch* fixed = nullptr) {
     if (!scene || !scene->dev || !o) throw Error(RT_E_INVALID_ARG, "null scene or options");
     if (o->samples == 0) throw Error(RT_E_INVALID_ARG, "Samples must be greater than 0");
     if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
@@ -564,18 +576,41 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     const FrameLayout L = frame_fill(F, S, o, kRaygenTile);
     const int depth = L.depth;
     const long long n_paths = L.n_paths;
+    // every option check comes before the arena changes, so a refused render leaves the
+    // previous frame's buffers (and rt_last_primary_hits) as they were
+    if (aux && aux->tile_mask) {
+        if (aux->mask_tile <= 0) throw Error(RT_E_INVALID_ARG, "mask_tile must be positive");
+        if (F.outline) throw Error(RT_E_INVALID_ARG, "a tile mask cannot be combined with the outline pass");
+        if (host_rgb) throw Error(RT_E_INVALID_ARG, "a tile mask needs a device output");
+    }
+    if (fixed && depth > kNoAllocMaxDepth)
+        throw Error(RT_E_INVALID_ARG, "max_depth above " + std::to_string(kNoAllocMaxDepth) +
+                                          " needs rt_render_device (the no-allocation entry point keeps fixed buffers)");
     RayRec *q0 = nullptr, *q1 = nullptr;
     const size_t bytes = frame_carve(F, S, L, nullptr, &q0, &q1, dev_rgb == nullptr);
-    ctx->fb.arena.reserve(bytes);
-    frame_carve(F, S, L, ctx->fb.arena.as<char>(), &q0, &q1, dev_rgb == nullptr);
+    char* base = nullptr;
+    if (fixed) {
+        if (bytes > fixed->bytes)
+            throw Error(RT_E_INVALID_ARG, "frame needs " + std::to_string(bytes) +
+                                              " bytes of scratch; the no-allocation entry point holds at most " +
+                                              std::to_string(fixed->bytes));
+        base = fixed->base;
+    } else {
+        if (bytes > ctx->fb.arena.bytes) {
+            // the arena moves: the last frame's primary records go with it
+            ctx->last_paths = 0;
+            ctx->last_prim_obj = ctx->last_prim_tri = nullptr;
+            ctx->last_prim_t = nullptr;
+        }
+        ctx->fb.arena.reserve(bytes);
+        base = ctx->fb.arena.as<char>();
+    }
+    frame_carve(F, S, L, base, &q0, &q1, dev_rgb == nullptr);
     if (dev_rgb) F.rgb = static_cast<uint8_t*>(dev_rgb);
     if (aux) {
         F.feat = aux->features;
         F.rad = aux->radiance;
         if (aux->tile_mask) {
-            if (aux->mask_tile <= 0) throw Error(RT_E_INVALID_ARG, "mask_tile must be positive");
-            if (F.outline) throw Error(RT_E_INVALID_ARG, "a tile mask cannot be combined with the outline pass");
-            if (host_rgb) throw Error(RT_E_INVALID_ARG, "a tile mask needs a device output");
             F.tmask = aux->tile_mask;
             F.mtile = aux->mask_tile;
             F.mtiles_x = (W + aux->mask_tile - 1) / aux->mask_tile;
@@ -589,12 +624,14 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         HIP_TRY(hipMemsetAsync(F.prim_tri, 0xff, n_paths * sizeof(int32_t), st));
     }
     ctx->timer.reset();
+    const bool timed_stages = o->profile && !fixed;
     // Without per-stage timing, a frame rendered twice in a row with the same kernel
     // arguments is captured into a hipGraph (kernels only) and replayed while they stay the
-    // same; a one-off frame launches eagerly and pays no capture.
+    // same; a one-off frame launches eagerly and pays no capture. The no-allocation entry
+    // point always launches eagerly (instantiating a graph allocates).
     std::vector<char> key;
     bool replay = false;
-    if (!o->profile) {
+    if (!o->profile && !fixed) {
         key.resize(sizeof(S) + sizeof(F) + 2 * sizeof(RayRec*));
         memcpy(key.data(), &S, sizeof(S));
         memcpy(key.data() + sizeof(S), &F, sizeof(F));
@@ -620,14 +657,24 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     }
     HIP_TRY(hipEventRecord(ctx->ev0, st));
     if (replay) HIP_TRY(hipGraphLaunch(ctx->frame_exec, st));
-    else HIP_TRY(launch_frame(S, F, q0, q1, st, o->profile ? &ctx->timer : nullptr));
+    else HIP_TRY(launch_frame(S, F, q0, q1, st, timed_stages ? &ctx->timer : nullptr));
     HIP_TRY(hipEventRecord(ctx->ev1, st));
     if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
-    std::vector<int32_t> counts(depth + 2, 0), hit_count(depth + 1, 0);
+    // per-level queue lengths (fixed host arrays on the no-allocation path)
+    std::vector<int32_t> counts_v, hits_v;
+    int32_t counts_a[kNoAllocMaxDepth + 2] = {}, hits_a[kNoAllocMaxDepth + 1] = {};
+    int32_t* counts = counts_a;
+    int32_t* hit_count = hits_a;
+    if (!fixed) {
+        counts_v.assign(depth + 2, 0);
+        hits_v.assign(depth + 1, 0);
+        counts = counts_v.data();
+        hit_count = hits_v.data();
+    }
     unsigned long long work[2 * kWorkCount] = {};
-    HIP_TRY(hipMemcpyAsync(counts.data(), F.counts, (depth + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(counts, F.counts, (depth + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     if (depth > 0)
-        HIP_TRY(hipMemcpyAsync(hit_count.data(), F.hit_count, depth * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipMemcpyAsync(hit_count, F.hit_count, depth * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(work, F.work, sizeof(work), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     ctx->last_paths = (int)n_paths;
@@ -655,7 +702,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         stats->ms_render = ms;
         stats->ms_upload = D.ms_upload;
         stats->levels = depth;
-        if (o->profile) {
+        if (timed_stages) {
             for (size_t i = 0; i < ctx->timer.used / 2; ++i) {
                 float e = 0;
                 HIP_TRY(hipEventElapsedTime(&e, ctx->timer.ev[2 * i], ctx->timer.ev[2 * i + 1]));
@@ -711,6 +758,36 @@ rt_scene* make_scene(rt_ctx* ctx, HostScene&& hs) {
 }  // namespace
 }  // namespace rt
 
+// Frame buffers of rt_render_device_noalloc: a fixed device array, so that entry point
+// allocates nothing. Each device holds its own copy at its own address until the process
+// ends; the address is resolved once per device (scratch_on_device) and only that address,
+// never the array's name, is passed to kernels or copies. Every call on one device shares
+// the array and finds the last call's contents: the frame kernels clear what must start at
+// zero (k_init_counts), and calls that overlap on one device race.
+static __device__ __align__(256) char scratch[RT_NOALLOC_SCRATCH_BYTES];
+
+namespace rt {
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<char*> g_scratch_addr;  // per device, resolved once
+
+// Device address of `scratch` on `device`, which must be the current device.
+char* scratch_on_device(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((size_t)device >= g_scratch_addr.size()) g_scratch_addr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw Error(RT_E_DEVICE, "hipGetSymbolAddress returned no address for the no-allocation scratch");
+        g_scratch_addr[device] = static_cast<char*>(p);
+    }
+    return g_scratch_addr[device];
+}
+
+}  // namespace
+}  // namespace rt
+
 namespace rt {
 // Context access for the other host modules (host/context.hpp).
 hipStream_t ctx_stream(rt_ctx* ctx) { return ctx->stream; }
@@ -737,6 +814,7 @@ int rt_ctx_create(int device, rt_ctx** out) {
         HIP_TRY(hipEventCreate(&c->ev0));
         HIP_TRY(hipEventCreate(&c->ev1));
         c->device = device;
+        c->noalloc_scratch = scratch_on_device(device);
     });
     if (rc == RT_OK) *out = c.release();
     return rc;
@@ -827,6 +905,16 @@ int rt_render_device(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, v
                      rt_render_stats* stats) {
     if (!ctx || !d_rgb_out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
     return guarded(ctx, [&] { render(ctx, scene, opts, nullptr, d_rgb_out, stats); });
+}
+
+int rt_render_device_noalloc(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,
+                             rt_render_stats* stats) {
+    if (!ctx || !d_rgb_out) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    return guarded(ctx, [&] {
+        if (!ctx->noalloc_scratch) throw Error(RT_E_DEVICE, "the context has no no-allocation scratch");
+        const FixedScratch fixed{ctx->noalloc_scratch, (size_t)RT_NOALLOC_SCRATCH_BYTES};
+        render(ctx, scene, opts, nullptr, d_rgb_out, stats, nullptr, &fixed);
+    });
 }
 
 int rt_render_device_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, void* d_rgb_out,

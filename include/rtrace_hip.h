@@ -209,6 +209,10 @@ void rt_scene_free(rt_scene* scene);
 int rt_scene_info(rt_scene* scene, double* cam_width, double* cam_height, int32_t* outline_enabled,
                   rt_render_opts* outline_out);
 
+/* scene.objects.len() and scene.lights.len() of the loaded Scene, as the reference CLI
+ * prints them after loading (cli/src/main.rs:63-67). */
+int rt_scene_counts(rt_scene* scene, int32_t* n_objects, int32_t* n_lights);
+
 /* Defaults of Renderer::new(width, height) (src/renderer.rs:50-62). */
 void rt_render_opts_default(rt_render_opts* opts, int32_t width, int32_t height);
 
@@ -272,7 +276,8 @@ int rt_mesh_load_stl(const char* path, double* vertices, int64_t capacity_tris, 
 /* No counterpart in the reference: the flattened k-d tree of KdTree::new (src/mesh.rs:75-197)
  * for a mesh of n_tris triangles (9 f64 each), built on the host (on_device = 0) or on the
  * device (on_device = 1, what rt_scene_load_* uses for meshes of 50,000 triangles or more),
- * for checking one build against the other. Pass capacities of 0 to get the counts; arrays
+ * for checking one build against the other (ctx may be NULL for a host build, which needs no
+ * device; errors then go to rt_last_error(NULL)). Pass capacities of 0 to get the counts; arrays
  * are copied out up to their capacities: nodes and records are 48 B each, bounds 6 f64 per
  * node. */
 typedef struct {
@@ -336,7 +341,9 @@ typedef struct {
 typedef struct rt_batch rt_batch;
 
 void rt_batch_opts_default(rt_batch_opts* bo);
-/* Lays out and allocates every job's buffers and captures the launch graph. */
+/* Lays out and allocates every job's buffers and captures the launch graph. The batch keeps
+ * raw device pointers of each job's scene and d_rgb_out: both must outlive the rt_batch
+ * (free the batch before rt_scene_free or before releasing an output buffer). */
 int rt_batch_create(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, const rt_batch_opts* bo,
                     rt_batch** out);
 /* Renders every job into its d_rgb_out; blocks until done. */

@@ -71,6 +71,9 @@ def lib():
         L.oracle_parse_transforms.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
         L.oracle_hex_to_color.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
         L.oracle_kd_stats.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_int32)] * 4 + [ctypes.c_char_p, ctypes.c_int]
+        L.oracle_kd_tree.argtypes = [ctypes.c_void_p, ctypes.c_int64] + [ctypes.c_void_p] * 7 + [ctypes.c_int64,
+                                     ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64)]
         L.oracle_mesh_hit.restype = ctypes.c_int
         L.oracle_mesh_hit.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                       ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
@@ -270,6 +273,26 @@ def kd_stats(stl_path):
     if rc != 0:
         raise RuntimeError(err.value.decode())
     return dict(zip(["tris", "leaves", "max_leaf", "nodes"], [v.value for v in vals]))
+
+
+def kd_tree(triangles):
+    """KdTree::new (src/mesh.rs:75-197) of an (n, 3, 3) float64 triangle array: nodes in
+    depth-first pre-order with leaf flag, axis, split, right child, leaf triangle lists
+    and f64 bounds."""
+    v = np.ascontiguousarray(triangles, np.float64).reshape(-1, 9)
+    nn, ni = ctypes.c_int64(), ctypes.c_int64()
+    z = ctypes.c_void_p()
+    lib().oracle_kd_tree(v.ctypes.data, len(v), z, z, z, z, z, z, z, 0, z, 0, ctypes.byref(nn), ctypes.byref(ni))
+    n = nn.value
+    leaf, axis, right, first, count = (np.zeros(n, np.int32) for _ in range(5))
+    split = np.zeros(n, np.float64)
+    bounds = np.zeros((n, 6), np.float64)
+    ids = np.zeros(max(ni.value, 1), np.int32)
+    lib().oracle_kd_tree(v.ctypes.data, len(v), leaf.ctypes.data, axis.ctypes.data, split.ctypes.data,
+                         right.ctypes.data, first.ctypes.data, count.ctypes.data, bounds.ctypes.data, n,
+                         ids.ctypes.data, ni.value, ctypes.byref(nn), ctypes.byref(ni))
+    return {"leaf": leaf, "axis": axis, "split": split, "right": right, "first": first, "count": count,
+            "bounds": bounds, "tri_ids": ids[:ni.value]}
 
 
 def mesh_hit(stl_path, use_kdtree, origin, direction, t_min=0.001, t_max=float("inf")):

@@ -2241,6 +2241,38 @@ int oracle_kd_stats(const char* stl_path, int32_t* n_tris, int32_t* n_leaves, in
         return e.code;
     }
 }
+int oracle_kd_tree(const double* verts, int64_t n_tris, int32_t* leaf, int32_t* axis, double* split, int32_t* right,
+                   int32_t* first, int32_t* count, double* bounds, int64_t node_cap, int32_t* tri_ids, int64_t id_cap,
+                   int64_t* n_nodes, int64_t* n_ids) {
+    // KdTree::new (src/mesh.rs:75-101) on the given triangles; kd_build pushes a node before
+    // its children, left subtree first, so nodes[] is in depth-first pre-order
+    std::vector<Tri> T((size_t)n_tris);
+    for (int64_t i = 0; i < n_tris; ++i)
+        for (int k = 0; k < 3; ++k) T[i].v[k] = v3(verts[9 * i + 3 * k], verts[9 * i + 3 * k + 1], verts[9 * i + 3 * k + 2]);
+    KdTree kt = kd_new(T);
+    int64_t ids = 0;
+    for (size_t i = 0; i < kt.nodes.size(); ++i) {
+        const KdNode& n = kt.nodes[i];
+        if ((int64_t)i < node_cap) {
+            leaf[i] = n.leaf;
+            axis[i] = n.leaf ? -1 : n.axis;
+            split[i] = n.leaf ? 0.0 : n.split;
+            right[i] = n.leaf ? -1 : n.right;
+            first[i] = n.leaf ? (int32_t)ids : -1;
+            count[i] = n.leaf ? (int32_t)n.tris.size() : 0;
+            const double b[6] = {n.bmin.x, n.bmin.y, n.bmin.z, n.bmax.x, n.bmax.y, n.bmax.z};
+            for (int k = 0; k < 6; ++k) bounds[6 * i + k] = b[k];
+        }
+        if (n.leaf)
+            for (int t : n.tris) {
+                if (ids < id_cap) tri_ids[ids] = t;
+                ++ids;
+            }
+    }
+    *n_nodes = (int64_t)kt.nodes.size();
+    *n_ids = ids;
+    return 0;
+}
 int oracle_mesh_hit(const char* stl_path, int use_kdtree, const double origin[3], const double dir[3], double t_min,
                     double t_max, double* t_out) {
     static std::map<std::string, std::shared_ptr<Obj>> cache;

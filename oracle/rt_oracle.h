@@ -98,6 +98,14 @@ int oracle_parse_transforms(const char* joined, double m_out[16]); /* '\n'-separ
 int oracle_hex_to_color(const char* hex, double out[3]);
 int oracle_kd_stats(const char* stl_path, int32_t* n_tris, int32_t* n_leaves, int32_t* max_leaf,
                     int32_t* n_nodes, char* err, int errlen);
+/* KdTree::new (src/mesh.rs:75-197) of n_tris triangles (9 f64 each), nodes in depth-first
+ * pre-order (left child = node + 1): leaf flag, axis (-1 leaf), split, right child (-1 leaf),
+ * the leaf's triangle list as [first, first + count) of tri_ids (leaves in node order), and
+ * the node's f64 bounds (min xyz, max xyz). Arrays are filled up to their capacities;
+ * n_nodes / n_ids receive the full sizes. */
+int oracle_kd_tree(const double* verts, int64_t n_tris, int32_t* leaf, int32_t* axis, double* split, int32_t* right,
+                   int32_t* first, int32_t* count, double* bounds, int64_t node_cap, int32_t* tri_ids,
+                   int64_t id_cap, int64_t* n_nodes, int64_t* n_ids);
 /* Closest hit of one ray against a mesh: k-d (use_kdtree=1) or brute force.
  * Returns triangle index or -1; writes t. */
 int oracle_mesh_hit(const char* stl_path, int use_kdtree, const double origin[3], const double dir[3],

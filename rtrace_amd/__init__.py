@@ -397,14 +397,16 @@ def kd_build(triangles, on_device=False):
     roots and leaf statistics."""
     v = np.ascontiguousarray(triangles, np.float64).reshape(-1, 9)
     a = _KdArrays()
-    _check(lib().rt_kd_build(_context(), v.ctypes.data, len(v), int(bool(on_device)), ctypes.byref(a)))
+    # a host build needs no device (no context); a device build runs on the context's GPU
+    ctx, check = (_context(), _check) if on_device else (None, _check_global)
+    check(lib().rt_kd_build(ctx, v.ctypes.data, len(v), int(bool(on_device)), ctypes.byref(a)))
     nodes = np.zeros((a.n_nodes, 48), np.uint8)
     recs = np.zeros((a.n_recs, 48), np.uint8)
     bounds = np.zeros((a.n_nodes, 6), np.float64)
     a.nodes, a.nodes_cap = nodes.ctypes.data, a.n_nodes
     a.recs, a.recs_cap = recs.ctypes.data, a.n_recs
     a.bounds, a.bounds_cap = bounds.ctypes.data, a.n_nodes
-    _check(lib().rt_kd_build(_context(), v.ctypes.data, len(v), int(bool(on_device)), ctypes.byref(a)))
+    check(lib().rt_kd_build(ctx, v.ctypes.data, len(v), int(bool(on_device)), ctypes.byref(a)))
     return {"nodes": nodes, "recs": recs, "bounds": bounds, "root_kd": a.root_kd, "root_bf": a.root_bf,
             "leaves": a.leaves, "max_leaf": a.max_leaf}
 
@@ -462,7 +464,9 @@ class Batch:
     """Many frames in one captured launch of persistent kernels (rt_batch_*).
 
     jobs: list of (scene, camera or None, renderer, out) where `renderer` supplies the
-    Renderer fields of that frame and `out` is a device uint8 tensor of H*W*3."""
+    Renderer fields of that frame and `out` is a device uint8 tensor of H*W*3. The batch
+    keeps a reference to every scene and output tensor: its captured launch holds their
+    device pointers until close()."""
 
     def __init__(self, jobs, opts=None):
         self._jobs, self._keep = _make_jobs(jobs)
@@ -482,7 +486,10 @@ class Batch:
             self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown: the library may already be gone
+            pass
 
 
 def default_batch_opts():
@@ -492,9 +499,13 @@ def default_batch_opts():
 
 
 def _make_jobs(jobs):
+    """The C job array, and what must outlive it: the camera copies it points at, and every
+    scene and output tensor whose device memory the captured launch reads and writes."""
     arr = (_BatchJob * len(jobs))()
     keep = []
     for i, (scene, cam, renderer, out) in enumerate(jobs):
+        keep.append(scene)
+        keep.append(out)
         arr[i].scene = scene._h
         if cam is not None:
             c = CameraDesc()

@@ -479,6 +479,9 @@ int rt_render_denoised(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts,
                        rt_denoise_stats* stats) {
     if (!ctx || !scene || !opts || !dn || !d_rgb_out) return ctx_fail(ctx, RT_E_INVALID_ARG, "null argument");
     return dn_guard(ctx, [&] {
+        // the network runs on the denoiser's stream and the compose on ctx's: one context
+        if (dn->ctx != ctx) throw Error(RT_E_INVALID_ARG, "the denoiser belongs to another context");
+        if (scene_ctx(scene) != ctx) throw Error(RT_E_INVALID_ARG, "the scene belongs to another context");
         if (tile <= 0) throw Error(RT_E_INVALID_ARG, "tile must be positive");
         if (fallback_samples < 1) throw Error(RT_E_INVALID_ARG, "fallback_samples must be positive");
         if (opts->outline_enabled && opts->aa_mode != RT_AA_QUINCUNX)

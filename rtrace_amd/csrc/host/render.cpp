@@ -20,6 +20,7 @@
 #include "frame.hpp"
 #include "kdbuild.hpp"
 #include "mesh_bvh.hpp"
+#include "object_bvh.hpp"
 #include "scene_host.hpp"
 
 namespace rt {
@@ -104,49 +105,6 @@ void cube_world_box(ObjDev& d) {
         d.bmin[i] = std::isfinite(mn[i]) ? round_down(mn[i] - pad) : -3.0e38f;
         d.bmax[i] = std::isfinite(mx[i]) ? round_up(mx[i] + pad) : 3.0e38f;
     }
-}
-
-// Object BVH over spheres: median split of the centroids on the widest axis, leaves of
-// at most 4 spheres, boxes = centre +- radius rounded outward.
-struct SphereBox {
-    float lo[3], hi[3];
-    double c[3];
-    int32_t obj;
-};
-uint32_t build_obvh(std::vector<SphereBox>& items, size_t begin, size_t end, std::vector<BvhNode>& nodes,
-                    std::vector<int32_t>& order) {
-    uint32_t me = (uint32_t)nodes.size();
-    nodes.push_back(BvhNode{});
-    BvhNode nd{};
-    for (int k = 0; k < 3; ++k) {
-        nd.bmin[k] = INFINITY;
-        nd.bmax[k] = -INFINITY;
-    }
-    double cmin[3] = {INFINITY, INFINITY, INFINITY}, cmax[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (size_t i = begin; i < end; ++i)
-        for (int k = 0; k < 3; ++k) {
-            nd.bmin[k] = std::min(nd.bmin[k], items[i].lo[k]);
-            nd.bmax[k] = std::max(nd.bmax[k], items[i].hi[k]);
-            cmin[k] = std::min(cmin[k], items[i].c[k]);
-            cmax[k] = std::max(cmax[k], items[i].c[k]);
-        }
-    if (end - begin <= 4) {
-        nd.a = (uint32_t)order.size();
-        nd.b = kBvhLeaf | (uint32_t)(end - begin);
-        for (size_t i = begin; i < end; ++i) order.push_back(items[i].obj);
-        nodes[me] = nd;
-        return me;
-    }
-    int axis = 0;
-    for (int k = 1; k < 3; ++k)
-        if (cmax[k] - cmin[k] > cmax[axis] - cmin[axis]) axis = k;
-    size_t mid = (begin + end) / 2;
-    std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
-                     [axis](const SphereBox& x, const SphereBox& y) { return x.c[axis] < y.c[axis]; });
-    nd.a = build_obvh(items, begin, mid, nodes, order);
-    nd.b = build_obvh(items, mid, end, nodes, order);
-    nodes[me] = nd;
-    return me;
 }
 
 bool is_identity(const double* m) {
@@ -305,7 +263,8 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
     if (D.has_obvh) {
         std::vector<BvhNode> nodes;
         std::vector<int32_t> order;
-        build_obvh(sph, 0, sph.size(), nodes, order);
+        if (build_obvh(sph, nodes, order) > kObvhMaxDepth)
+            throw Error(RT_E_INTERNAL, "object BVH deeper than its traversal stack");
         upload(D.obvh, nodes, st);
         upload(D.obvh_items, order, st);
         upload(D.others, others, st);
@@ -792,6 +751,7 @@ namespace rt {
 // Context access for the other host modules (host/context.hpp).
 hipStream_t ctx_stream(rt_ctx* ctx) { return ctx->stream; }
 int ctx_device(rt_ctx* ctx) { return ctx->device; }
+rt_ctx* scene_ctx(rt_scene* scene) { return scene->ctx; }
 int ctx_fail(rt_ctx* ctx, int code, const std::string& msg) { return fail(ctx, code, msg); }
 void ctx_render_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
                     rt_render_stats* stats) {
@@ -879,6 +839,13 @@ int rt_scene_info(rt_scene* scene, double* cam_width, double* cam_height, int32_
     return RT_OK;
 }
 
+int rt_scene_counts(rt_scene* scene, int32_t* n_objects, int32_t* n_lights) {
+    if (!scene || !scene->dev) return fail(nullptr, RT_E_INVALID_ARG, "null scene");
+    if (n_objects) *n_objects = (int32_t)scene->dev->host.objects.size();
+    if (n_lights) *n_lights = (int32_t)scene->dev->host.lights.size();
+    return RT_OK;
+}
+
 void rt_render_opts_default(rt_render_opts* o, int32_t width, int32_t height) {
     if (!o) return;
     memset(o, 0, sizeof(*o));
@@ -934,14 +901,15 @@ int rt_last_primary_hits(rt_ctx* ctx, int32_t* obj, int32_t* tri, double* t, siz
 }
 
 int rt_kd_build(rt_ctx* ctx, const double* vertices, int64_t n_tris, int32_t on_device, rt_kd_arrays* out) {
-    if (!ctx || !out || n_tris < 0 || (n_tris > 0 && !vertices)) return fail(ctx, RT_E_INVALID_ARG, "null argument");
+    if ((!ctx && on_device) || !out || n_tris < 0 || (n_tris > 0 && !vertices))
+        return fail(ctx, RT_E_INVALID_ARG, "null argument");
     return guarded(ctx, [&] {
         if (n_tris > (int64_t)(1u << 28)) throw Error(RT_E_INVALID_ARG, "too many triangles");
         const std::vector<double> v(vertices, vertices + (size_t)n_tris * 9);
         FlatKd kd;
         std::vector<float> recs;
-        hipStream_t st = ctx->stream;
         if (on_device) {
+            hipStream_t st = ctx->stream;
             DevBuf tri64, nodes, drecs, nb64;
             upload(tri64, v, st);
             build_flat_kd_device(v, tri64.as<double>(), kd, nodes, drecs, nb64, st);

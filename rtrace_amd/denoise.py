@@ -88,9 +88,14 @@ class Denoiser:
         np.save(path, self.get_params())
 
     def __del__(self):
-        if getattr(self, "_h", None) and self._h.value:
-            lib().rt_denoiser_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        if h is None or not h.value:
+            return
+        self._h = None
+        try:
+            lib().rt_denoiser_destroy(h)
+        except Exception:  # interpreter shutdown: module globals may already be gone
+            pass
 
     def get_params(self):
         p = np.zeros(self.n_params, np.float32)

@@ -59,19 +59,29 @@ def gpu_render(R, path, opts, with_hits=True):
     return img, r.last_stats, hits
 
 
-def compare(name, gpu, ref, ghits=None, rhits=None):
+def compare(name, gpu, ref, ghits=None, rhits=None, exact=True):
+    """The stated contract (hit ids outside the epsilon-band, +-1 LSB on >= 99.9 % of
+    pixels), then, with `exact`, what the build achieves on every case: 0 hit-id
+    mismatches and 0 differing pixels, and hit distances equal to 1e-12 relative (the
+    stochastic multi-sample jitter takes cos/sin from the device math library, which can
+    differ from glibc by an ulp and moves a camera ray by that much)."""
     d = np.abs(gpu.astype(int) - ref.astype(int)).max(-1)
     frac = float(np.mean(d <= PIX_LSB))
     msg = f"{name}: pixels exact {np.mean(d == 0) * 100:.4f}% within 1 LSB {frac * 100:.4f}% max {d.max()}"
     mism = 0
+    tmax = 0.0
     if ghits is not None:
         mism = int(np.count_nonzero((ghits[0] != rhits[0]) | (ghits[1] != rhits[1])))
-        same = (ghits[0] == rhits[0]) & (ghits[1] == rhits[1])
-        tdiff = np.abs(ghits[2] - rhits[2])[same]
-        msg += f" | hit-id mismatches {mism}/{len(ghits[0])} max |dt| on equal ids {tdiff.max() if tdiff.size else 0:.3g}"
+        same = (ghits[0] == rhits[0]) & (ghits[1] == rhits[1]) & (rhits[0] >= 0)
+        tdiff = (np.abs(ghits[2] - rhits[2]) / np.maximum(np.abs(rhits[2]), 1e-300))[same]
+        tmax = float(tdiff.max()) if tdiff.size else 0.0
+        msg += f" | hit-id mismatches {mism}/{len(ghits[0])} max relative |dt| on equal hits {tmax:.3g}"
         assert mism <= max(1, EPS_BAND_FRAC * len(ghits[0])), msg
     print(msg)
     assert frac >= PIX_FRAC, msg
+    if exact:
+        assert mism == 0 and tmax <= 1e-12, "not bit-exact: " + msg
+        assert d.max() == 0, "not bit-exact: " + msg
     return d, mism
 
 

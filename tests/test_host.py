@@ -84,3 +84,23 @@ def test_splitmix64_known_answer():
 
     g = SplitMix64(0)
     assert [g.next_u64() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+
+
+def test_batch_jobs_keep_scenes_and_outputs_alive():
+    """A batch's captured launch holds raw device pointers of each job's scene and output
+    (rt_batch_create): the Python job list must keep both objects alive with the batch."""
+    import ctypes
+
+    import rtrace_amd as R
+
+    class Scene:
+        _h = ctypes.c_void_p(0x1000)
+
+    class Out:
+        def data_ptr(self):
+            return 0x2000
+
+    scene, out = Scene(), Out()
+    arr, keep = R._make_jobs([(scene, None, R.Renderer(8, 8), out)])
+    assert any(k is scene for k in keep) and any(k is out for k in keep)
+    assert arr[0].scene == 0x1000 and arr[0].d_rgb_out == 0x2000

@@ -35,3 +35,17 @@ def test_mesh_bvh_invariants(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip() == "ok 21"
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_object_bvh_depth_fits_the_traversal_stack(tmp_path):
+    """Object BVH over spheres (scenes of 16 or more spheres, e.g. the 1,000-sphere ring):
+    its walk (obvh_visit) pushes one entry per level into a 16-entry per-lane stack, so the
+    build bounds the depth by growing leaves past 65,536 spheres (300k spheres: depth <= 16,
+    every sphere in exactly one leaf, boxes nested)."""
+    exe = str(tmp_path / "obvh_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-o", exe, os.path.join(ROOT, "tests", "cpp", "obvh_check.cpp"),
+                    os.path.join(ROOT, "rtrace_amd", "csrc", "host", "object_bvh.cpp")], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "ok 16"

@@ -1,6 +1,7 @@
 """The Node addon (rtrace_amd/bin/rtrace_hip.node): the reference binding's exports
-(index.d.ts:7-21) and, on a GPU, renders equal to the library's for the same sizing
-(bindings/node/src/lib.rs:31-45: H = D / sqrt(R^2 + 1), W = R H, both rounded)."""
+(index.d.ts:7-21) and, on a GPU, renders equal to the reference's own renders of the same
+scenes at the same sizing (bindings/node/src/lib.rs:31-45: H = D / sqrt(R^2 + 1), W = R H,
+both rounded)."""
 import json
 import math
 import os
@@ -10,7 +11,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import REF_SCENES, ROOT
+from conftest import REF_IMAGES, REF_SCENES, ROOT
 
 ADDON = os.path.join(ROOT, "rtrace_amd", "bin", "rtrace_hip.node")
 NODE = shutil.which("node")
@@ -45,35 +46,48 @@ def _size(path, diag):
     return int(round(r * h)), int(round(h))
 
 
+def _gold(name):
+    from PIL import Image
+    return np.asarray(Image.open(os.path.join(REF_IMAGES, name + ".png")).convert("RGB"))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("fn,suffix", [("renderSceneFromFile", " (multi-threaded)"),
                                        ("renderSceneFromFileBruteForce", " (brute force)"),
                                        ("renderSceneFromFileThreaded", " with 3 threads")])
-def test_file_render_equals_library(tmp_path, fn, suffix):
+def test_file_render_equals_reference(tmp_path, fn, suffix):
+    """The binding renders with Renderer::new (quincunx, depth 10, seed 0) at the CLI's sizing,
+    so at diagonal 500 its k-d renders are the reference's own doc render of the scene
+    (object-mesh.png, CLI -s 500); the brute-force one is checked against the oracle's
+    brute-force render (src/ray.rs:650-671)."""
     from PIL import Image
-    import rtrace_amd as R
+    from oracle import pyoracle as O
     scene = os.path.join(REF_SCENES, "object-mesh.json")
     out = str(tmp_path / "o.png")
     extra = ", 3" if "Threaded" in fn else ""
-    msg = node(f"console.log(m.{fn}({json.dumps(scene)}, {json.dumps(out)}, 300{extra}))")
-    W, H = _size(scene, 300)
-    assert msg == f"Successfully rendered {W}×{H} image (diagonal 300) to '{out}'" + suffix
-    r = R.Renderer(W, H, use_kdtree="Brute" not in fn)
-    ref = r.render(R.Scene.from_json_file(scene))
-    assert np.array_equal(np.asarray(Image.open(out).convert("RGB")), ref)
+    msg = node(f"console.log(m.{fn}({json.dumps(scene)}, {json.dumps(out)}, 500{extra}))")
+    W, H = _size(scene, 500)
+    assert msg == f"Successfully rendered {W}×{H} image (diagonal 500) to '{out}'" + suffix
+    img = np.asarray(Image.open(out).convert("RGB"))
+    if "Brute" in fn:
+        ref, _, _ = O.render(scene, O.make_opts(W, H, use_kdtree=False))
+    else:
+        ref = _gold("object-mesh")
+    assert img.shape == ref.shape and np.array_equal(img, ref)
 
 
 @pytest.mark.gpu
 def test_json_render_and_parse_error(tmp_path):
+    """renderScene from JSON text at diagonal 500 is the reference's lighting-multiple.png."""
     from PIL import Image
-    import rtrace_amd as R
     scene = os.path.join(REF_SCENES, "lighting-multiple.json")
     text = open(scene).read()
     out = str(tmp_path / "j.png")
-    msg = node(f"console.log(m.renderScene({json.dumps(text)}, {json.dumps(out)}))")
-    W, H = _size(scene, 1000)
-    assert msg.startswith(f"Successfully rendered {W}×{H} image (diagonal 1000)")
-    ref = R.Renderer(W, H).render(R.Scene.from_json_str(text))
-    assert np.array_equal(np.asarray(Image.open(out).convert("RGB")), ref)
+    msg = node(f"console.log(m.renderScene({json.dumps(text)}, {json.dumps(out)}, 500))")
+    W, H = _size(scene, 500)
+    assert msg == f"Successfully rendered {W}×{H} image (diagonal 500) to '{out}' (multi-threaded)"
+    img = np.asarray(Image.open(out).convert("RGB"))
+    ref = _gold("lighting-multiple")
+    assert img.shape == ref.shape and np.array_equal(img, ref)
     err = node("try { m.renderScene('{', 'x.png') } catch (e) { console.log(e.code + '|' + e.message) }")
     assert err.startswith("InvalidArg|Failed to parse scene JSON: ")

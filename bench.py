@@ -1,18 +1,24 @@
-"""Benchmark: Mrays/s of the per-pixel trace path at 1080p on one MI355X, and the
-denoised-vs-plain frame time of the 1080p denoiser workload.
+"""Benchmark: Mrays/s of the per-pixel trace path at 1080p on one MI355X, the benchmark
+configurations that fit one GPU, and the denoised-vs-plain frame time of the 1080p
+denoiser workload.
 
-Workload (default `icosphere_1080p`): the procedural mesh scene
-(81,920-triangle noisy icosphere with a host-built k-d tree, 3 transformed cubes, a
-reflective grid plane, a disk area light with 16 shadow samples per hit, fog 0.05,
-reflections to depth 4) rendered at 1920x1080 with quincunx anti-aliasing. A step is one
-full frame (camera rays, closest-hit, shading, shadow rays, reflections, resolve,
-quantise) rendered into a device buffer; the scene is resident in HBM before timing.
+Headline workload (default `icosphere_1080p`): the procedural mesh scene (81,920-triangle
+noisy icosphere with a host-built k-d tree, 3 transformed cubes, a reflective grid plane,
+a disk area light with 16 shadow samples per hit, fog 0.05, reflections to depth 4)
+rendered at 1920x1080 with quincunx anti-aliasing. A step is one full frame (camera rays,
+closest-hit, shading, shadow rays, reflections, resolve, quantise) rendered into a device
+buffer; the scene is resident in HBM before timing.
+
+The `configs` object times the other single-GPU configurations the same way:
+`sphere_800x600` (config 1), `icosphere_square_720p` (config 2 as stated: 1280x720,
+1x1 square light, 16 stratified samples) and `spheres_ring_1080p` (config 3: 1,000
+reflective spheres, stochastic 4 spp, depth 10, outline). The `denoise` object is config 5:
+1080p frames rendered plain with 256 light samples (the quality reference) and denoised
+(4 samples + the trained U-Net + tile fallback), with PSNR / SSIM and per-stage times per
+image and the denoiser's BF16 MFMA roofline.
 
 Rays = primary + reflection + shadow rays, counted exactly by the pipeline.
-The "denoise" object: denoiser-workload frames at 1080p (icosphere scenes, 2x2 square light) rendered plain with 256 light
-samples (the quality reference) and denoised (4 samples + the trained U-Net + tile fallback), with
-PSNR / SSIM per image and the denoiser's BF16 MFMA roofline.
-Prints one JSON line.
+Prints one JSON line; progress goes to stderr.
 """
 import argparse
 import json
@@ -26,12 +32,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+N_SIMD = 256 * 4  # MI355X: 256 CUs x 4 SIMDs
 # algorithmic bytes per unit of traversal work (layout in rtrace_amd/csrc/common/layout.hpp)
 B_NODE, B_TRI, B_RETEST = 40, 48, 72  # node: 32 B box+links, +8 B f64 split (inner)
 B_BVH_NODE = 128  # 4-wide mesh BVH node: four slot boxes and links
 B_SPHERE, B_PLANE, B_CUBE = 32, 48, 304
 B_RAY_CLOSEST = 64 + 16  # RayRec in, HitRec out
 B_RAY_SHADOW = 48 + 24 + 1  # hit point + normal, light sample point in, visibility byte out
+TRIANGLES = {"icosphere": 81920}
+
+
+def _progress(msg):
+    """A progress line on stderr (the JSON result line stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def kernel_bytes(work, rays, per_ray):
@@ -63,11 +76,25 @@ def configure(R, renderer, cfg, scene):
         renderer.outline_config = scene.info()["outline"]
 
 
+def host_threads():
+    """The host cores this process may use: its CPU affinity, bounded by OMP_NUM_THREADS when the
+    environment sets it (a one-GPU box grants a share of its host, 16 threads)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
 def cpu_baseline(path, cfg, budget_s=20.0):
-    """The CPU f64 oracle (a restatement of the reference renderer) on the box's host
-    cores, on a reduced-resolution render of the same scene and settings."""
+    """The CPU f64 oracle (a restatement of the reference renderer) on the box's host cores,
+    on a reduced-resolution render of the same scene and settings; and, as the checker, the
+    parity of the HIP render at that resolution against it (primary-ray hit ids, pixels)."""
+    import numpy as np
+
+    import rtrace_amd as R
     from oracle import pyoracle as O
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     w, h = 64, 36
     with open(path) as f:
         scene = json.load(f)
@@ -77,10 +104,11 @@ def cpu_baseline(path, cfg, budget_s=20.0):
     while True:
         opts = O.make_opts(w, h, max_depth=cfg["max_depth"], samples=cfg["samples"], aa=cfg["aa"], threads=threads,
                            outline=outline)
-        _, st, _ = O.render(path, opts)
+        img, st, hits = O.render(path, opts, want_hits=True)
         rays = st["rays_primary"] + st["rays_reflect"] + st["rays_shadow"]
         best = (rays / st["render_seconds"] / 1e6, w, h, st["render_seconds"], rays,
-                {"kd_nodes": (st["kd_inner_visits"] + st["kd_leaf_visits"]) / rays, "tri_tests": st["tri_tests"] / rays})
+                {"kd_nodes": (st["kd_inner_visits"] + st["kd_leaf_visits"]) / rays, "tri_tests": st["tri_tests"] / rays},
+                img, hits, st)
         spent += st["render_seconds"]
         if st["render_seconds"] * 4 > budget_s - spent or w >= cfg["width"]:
             break
@@ -90,14 +118,31 @@ def cpu_baseline(path, cfg, budget_s=20.0):
                        outline=outline)
     _, st1, _ = O.render(path, opts)
     rays1 = st1["rays_primary"] + st1["rays_reflect"] + st1["rays_shadow"]
-    return {"value": best[0], "unit": "Mrays/s", "cores": threads, "kind": "port",
+
+    # parity of the HIP path on the same sample (the oracle is the checker here)
+    bw, bh, oimg, ohits, ost = best[1], best[2], best[6], best[7], best[8]
+    r = R.Renderer(bw, bh)
+    configure(R, r, cfg, R.Scene.from_json_file(path))
+    gimg = r.render(R.Scene.from_json_file(path))
+    n_prim = bw * bh + (bw + 1) * (bh + 1) if cfg["aa"] == "quincunx" else bw * bh * cfg["samples"]
+    gh = R.last_primary_hits(n_prim)
+    mism = int(np.count_nonzero((gh[0] != ohits[0]) | (gh[1] != ohits[1])))
+    d = np.abs(gimg.astype(int) - oimg.astype(int)).max(-1)
+    gst = r.last_stats
+    parity = {"sample": f"{bw}x{bh} render of the benched scene and settings", "primary_rays": n_prim,
+              "hit_id_mismatches": mism, "pixels": int(d.size), "pixels_exact": int(np.count_nonzero(d == 0)),
+              "pixels_within_1lsb_frac": round(float(np.mean(d <= 1)), 6), "max_abs_diff": int(d.max()),
+              "ray_counts_equal": (gst.rays_primary, gst.rays_reflect, gst.rays_shadow) ==
+                                  (ost["rays_primary"], ost["rays_reflect"], ost["rays_shadow"])}
+    base = {"value": best[0], "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle render of the same scene at {best[1]}x{best[2]} ({best[4]} rays, {best[3]:.1f} s, "
-                      f"{threads} threads)",
+                      f"{threads} threads = this process's host CPU share)",
             # the reference's traversal (no early exit) per ray on this sample: an upper bound of
             # the traversal work, never the roofline's numerator
             "reference_traversal_per_ray": {k: round(v, 1) for k, v in best[5].items()},
             "one_thread": {"value": rays1 / st1["render_seconds"] / 1e6, "unit": "Mrays/s", "cores": 1,
                            "sample": f"64x36 ({rays1} rays, {st1['render_seconds']:.1f} s)"}}
+    return base, parity
 
 
 COPY_BYTES = 1 << 30
@@ -118,7 +163,8 @@ def denoise_section(seeds, threshold, low=4, high=256):
     import rtrace_amd as R  # noqa: F401 - loads the library
     from rtrace_amd import denoise
     dn = denoise.Denoiser.load()
-    rows = denoise.evaluate(dn, seeds, 1920, 1080, low=low, high=high, threshold=threshold, repeats=3)
+    rows = denoise.evaluate(dn, seeds, 1920, 1080, low=low, high=high, threshold=threshold, repeats=3,
+                            log=lambda m: _progress("denoise " + m[:160]))
     ms_dn_net = sorted(r["stage_ms"]["denoise"] for r in rows)[len(rows) // 2]
     flops = 2.0 * DN_MAC_PER_PX * 1920 * 1080
     ach = flops / (ms_dn_net * 1e-3) / 1e12
@@ -138,18 +184,24 @@ def denoise_section(seeds, threshold, low=4, high=256):
     }
 
 
-def traffic_child(workload):
-    """Profiled child: one frame of the workload, then a plain copy of COPY_BYTES (the
-    calibration kernel for the memory counters), in one process."""
+def counters_child(workload):
+    """Profiled child: one frame of the workload, one denoiser forward pass on 1080p features,
+    then a plain copy of COPY_BYTES (the calibration kernel for the memory counters), in one
+    process."""
     import torch
 
     import rtrace_amd as R
+    from rtrace_amd import denoise
     path, cfg = make_workload(workload, os.path.join(tempfile.gettempdir(), "rtrace_amd_bench"))
     scene = R.Scene.from_json_file(path)
     r = R.Renderer(cfg["width"], cfg["height"])
     configure(R, r, cfg, scene)
     out = torch.empty(cfg["width"] * cfg["height"] * 3, dtype=torch.uint8, device="cuda")
     r.render_device(scene, out)
+    dn = denoise.Denoiser.load()
+    feat = torch.rand(1080, 1920, 13, dtype=torch.float32, device="cuda")
+    dn.denoise(feat)
+    torch.cuda.synchronize()
     src = torch.ones(COPY_BYTES, dtype=torch.uint8, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -157,9 +209,18 @@ def traffic_child(workload):
     torch.cuda.synchronize()
 
 
-def measure_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` from two rocprofv3 --pmc passes (FETCH_SIZE, then
-    WRITE_SIZE; KiB per dispatch), each scaled by bytes / counter of the copy kernel."""
+# One rocprofv3 --pmc pass per entry (a pass holds at most 8 SQ, 4 TCC, 2 GRBM counters).
+PMC_PASSES = {
+    "fetch": "FETCH_SIZE",
+    "write": "WRITE_SIZE",
+    "l2": "TCC_REQ_sum TCC_HIT_sum",
+    "sq": "SQ_WAVE_CYCLES SQ_WAIT_ANY SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU SQ_BUSY_CYCLES "
+          "SQ_VALU_MFMA_BUSY_CYCLES SQ_INSTS_VALU_MFMA_MOPS_BF16 GRBM_GUI_ACTIVE",
+}
+
+
+def run_counters(workload):
+    """Runs the PMC passes over counters_child; returns {pass: [(kernel, dispatch, counter, value)]}."""
     import csv
     import glob
     import shutil
@@ -169,51 +230,166 @@ def measure_traffic(workload, kernel):
     if not prof:
         return None, "rocprofv3 not found"
     res = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE", "TCC_REQ_sum TCC_HIT_sum"):
+    for name, counters in PMC_PASSES.items():
         d = tempfile.mkdtemp(prefix="rt_pmc_")
-        cmd = [prof, "--pmc", *counter.split(), "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
-               os.path.abspath(__file__), "--workload", workload, "--traffic-child"]
+        cmd = [prof, "--pmc", *counters.split(), "--output-format", "csv", "-d", d, "-o", "run", "--", sys.executable,
+               os.path.abspath(__file__), "--workload", workload, "--counters-child"]
         try:
             subprocess.run(cmd, check=True, timeout=300, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-        except Exception as e:  # noqa: BLE001 - report and carry on without traffic
-            return None, f"{counter} pass failed: {e}"
+        except Exception as e:  # noqa: BLE001 - report and carry on without counters
+            return None, f"{name} pass failed: {e}"
         rows = []
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-            rows += list(csv.DictReader(open(f)))
-        if counter.startswith("TCC_"):
-            # L2 requests and hits per launch; bytes per request from the copy (each copied
-            # byte is one L2 read and one L2 write)
-            def per_dispatch(name, match):
-                vals = {}
-                for r in rows:
-                    if r["Counter_Name"] == name and match(r["Kernel_Name"]):
-                        vals[r.get("Dispatch_Id", len(vals))] = vals.get(r.get("Dispatch_Id", len(vals)), 0.0) + float(r["Counter_Value"])
-                return list(vals.values())
-            kreq = per_dispatch("TCC_REQ_sum", lambda n: re.search(r"::%s[<(]" % kernel, n))
-            khit = per_dispatch("TCC_HIT_sum", lambda n: re.search(r"::%s[<(]" % kernel, n))
-            creq = [c for c in per_dispatch("TCC_REQ_sum", lambda n: "copy" in n.lower()) if c > 1000]
-            shutil.rmtree(d, ignore_errors=True)
-            if kreq and creq:
-                bpr = 2.0 * COPY_BYTES / max(creq)
-                req = sum(kreq) / len(kreq)
-                res["l2"] = {"requests": int(req), "bytes": int(req * bpr), "bytes_per_request": round(bpr, 1),
-                             "hit_rate": round(sum(khit) / max(sum(kreq), 1.0), 4) if khit else None}
-            continue
-        kern = [float(r["Counter_Value"]) for r in rows if re.search(r"::%s[<(]" % kernel, r["Kernel_Name"])]
-        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"] or "copy" in r["Kernel_Name"].lower()]
-        copy = [c for c in copy if c > 1000]  # the 256 MiB copy, not small staging copies
+            with open(f) as fh:
+                rows += list(csv.DictReader(fh))
         shutil.rmtree(d, ignore_errors=True)
-        if not kern or not copy:
-            return None, f"{counter}: no samples"
-        scale = COPY_BYTES / (max(copy) * 1024.0)
-        res[counter] = (sum(kern) / len(kern)) * 1024.0 * scale
-        res[counter + "_scale"] = scale
+        res[name] = [(r["Kernel_Name"], r.get("Dispatch_Id", ""), r["Counter_Name"], float(r["Counter_Value"]))
+                     for r in rows]
+        _progress(f"pmc pass {name}: {len(rows)} rows")
     return res, None
 
 
-def _progress(msg):
-    """A progress line on stderr (the JSON result line stays alone on stdout)."""
-    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+def _per_dispatch(rows, counter, match):
+    vals = {}
+    for kname, disp, cname, v in rows:
+        if cname == counter and match(kname):
+            vals[disp] = vals.get(disp, 0.0) + v
+    return list(vals.values())
+
+
+def _kern(name):
+    return lambda n: re.search(r"::%s[<(]" % name, n) is not None
+
+
+def _is_copy(n):
+    return "copy" in n.lower()
+
+
+def counter_summary(pmc, kernel, ms_per_launch):
+    """HBM and L2 traffic per launch of `kernel` (KiB counters scaled by the copy kernel's
+    bytes per count) and what bounds it: VALU lane activity, VALU busy share and the share
+    of wave time spent waiting."""
+    out = {}
+    copy_f = [c for c in _per_dispatch(pmc["fetch"], "FETCH_SIZE", _is_copy) if c > 1000]
+    copy_w = [c for c in _per_dispatch(pmc["write"], "WRITE_SIZE", _is_copy) if c > 1000]
+    kf = _per_dispatch(pmc["fetch"], "FETCH_SIZE", _kern(kernel))
+    kw = _per_dispatch(pmc["write"], "WRITE_SIZE", _kern(kernel))
+    if kf and kw and copy_f and copy_w:
+        sf = COPY_BYTES / (max(copy_f) * 1024.0)
+        sw = COPY_BYTES / (max(copy_w) * 1024.0)
+        hbm = (sum(kf) / len(kf)) * 1024.0 * sf + (sum(kw) / len(kw)) * 1024.0 * sw
+        out["hbm_bytes"] = int(hbm)
+        out["hbm_gbs"] = round(hbm / (ms_per_launch * 1e-3) / 1e9, 1)
+        out["hbm_frac"] = round(out["hbm_gbs"] / HBM_PEAK_GBS, 4)
+        out["hbm_note"] = f"FETCH_SIZE*{sf:.3f} + WRITE_SIZE*{sw:.3f} (KiB counters scaled by a {COPY_BYTES >> 20} MiB copy)"
+    kreq = _per_dispatch(pmc["l2"], "TCC_REQ_sum", _kern(kernel))
+    khit = _per_dispatch(pmc["l2"], "TCC_HIT_sum", _kern(kernel))
+    creq = [c for c in _per_dispatch(pmc["l2"], "TCC_REQ_sum", _is_copy) if c > 1000]
+    if kreq and creq:
+        bpr = 2.0 * COPY_BYTES / max(creq)  # each copied byte is one L2 read and one L2 write
+        req = sum(kreq) / len(kreq)
+        out["l2_bytes"] = int(req * bpr)
+        out["l2_gbs"] = round(req * bpr / (ms_per_launch * 1e-3) / 1e9, 1)
+        out["l2_hit_rate"] = round(sum(khit) / max(sum(kreq), 1.0), 4) if khit else None
+    sq = pmc["sq"]
+    g = {c: _per_dispatch(sq, c, _kern(kernel)) for c in PMC_PASSES["sq"].split()}
+    if all(g.values()):
+        m = {c: sum(v) / len(v) for c, v in g.items()}
+        out["valu_lanes_active"] = round(m["SQ_THREAD_CYCLES_VALU"] / max(m["SQ_ACTIVE_INST_VALU"] * 64.0, 1.0), 4)
+        out["valu_busy"] = round(m["SQ_ACTIVE_INST_VALU"] * 4.0 / N_SIMD / max(m["GRBM_GUI_ACTIVE"], 1.0), 4)
+        out["wave_wait_frac"] = round(m["SQ_WAIT_ANY"] / max(m["SQ_WAVE_CYCLES"], 1.0), 4)
+    # the limiter, from the measurements
+    if out.get("hbm_frac", 0) >= 0.6:
+        out["limiter"] = "hbm bandwidth"
+    elif out.get("valu_busy", 0) >= 0.6:
+        out["limiter"] = "valu issue"
+    elif out.get("wave_wait_frac", 0) >= 0.5:
+        out["limiter"] = "memory latency (waves mostly waiting on dependent loads; HBM and VALU far from peak)"
+    elif "valu_busy" in out:
+        out["limiter"] = "mixed (no unit near its peak)"
+    return out
+
+
+def mfma_summary(pmc, kernel="k_conv3x3"):
+    """MFMA busy share of the denoiser's convolutions (SQ_VALU_MFMA_BUSY_CYCLES per SIMD over the
+    active cycles), and the bf16 MFMA work they did."""
+    sq = pmc["sq"]
+    busy = _per_dispatch(sq, "SQ_VALU_MFMA_BUSY_CYCLES", _kern(kernel))
+    gui = _per_dispatch(sq, "GRBM_GUI_ACTIVE", _kern(kernel))
+    mops = _per_dispatch(sq, "SQ_INSTS_VALU_MFMA_MOPS_BF16", _kern(kernel))
+    if not busy or not gui:
+        return None
+    return {"kernel": f"dn::{kernel}", "launches": len(busy),
+            "mfma_busy": round(sum(busy) / max(sum(gui) * N_SIMD, 1.0), 4),
+            "bf16_mops_per_launch": round(sum(mops) / len(mops), 1) if mops else None,
+            "counter": "SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs), summed over the launches"}
+
+
+def measure_frame(R, torch, workload, steps, warmup, stage_events=True, count=True):
+    """Times `steps` frames of a workload after `warmup` ones. With stage_events the timed frames
+    record hipEvents around each stage's launches (the per-launch kernel times come from the timed
+    region); otherwise the timed frames replay the frame's hipGraph and the stage times come from as
+    many profiled frames after them."""
+    out_dir = os.path.join(tempfile.gettempdir(), "rtrace_amd_bench")
+    path, cfg = make_workload(workload, out_dir)
+    scene = R.Scene.from_json_file(path)
+    W, H = cfg["width"], cfg["height"]
+    r = R.Renderer(W, H)
+    configure(R, r, cfg, scene)
+    out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
+    st_work = None
+    if count:  # untimed: exact work counters
+        r.count_work = True
+        st_work = r.render_device(scene, out)
+        r.count_work = False
+    stage_ms, stage_n = {}, {}
+
+    def add_stages(st_):
+        d_ = st_.as_dict()
+        for k, v in d_["ms_stage"].items():
+            stage_ms[k] = stage_ms.get(k, 0.0) + v
+            stage_n[k] = stage_n.get(k, 0) + d_["launches_stage"][k]
+
+    r.profile = stage_events
+    for _ in range(warmup):
+        r.render_device(scene, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_ms = []
+    for _ in range(steps):
+        st = r.render_device(scene, out)
+        dev_ms.append(st.ms_render)
+        if r.profile:
+            add_stages(st)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if not r.profile:
+        r.profile = True
+        for _ in range(steps):
+            add_stages(r.render_device(scene, out))
+    r.profile = False
+    ms = (t1 - t0) * 1e3 / steps
+    rays = st.rays_total
+    res = {"workload": workload, "width": W, "height": H, "aa": cfg["aa"], "samples": cfg["samples"],
+           "max_depth": cfg["max_depth"], "steps": steps, "warmup": warmup, "ms_per_step": round(ms, 3),
+           "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
+           "mrays_per_s": round(rays / (ms * 1e-3) / 1e6, 2), "rays_per_frame": rays,
+           "primary": st.rays_primary, "reflect": st.rays_reflect, "shadow": st.rays_shadow,
+           "stages_ms": {k: round(v / steps, 3) for k, v in stage_ms.items()},
+           "launches_stage": {k: v // steps for k, v in stage_n.items()},
+           "kernel_timing": ("hipEvents around each stage's launches in the timed steps" if stage_events else
+                             f"graph replays timed; stage times from {steps} profiled steps after them")}
+    if st_work is not None:
+        wk = st_work.as_dict()["work"]
+        skipped = wk["shadow"]["skipped"]
+        res["work"] = wk
+        res["shadow_skipped"] = skipped
+        res["rays_traced_per_frame"] = rays - skipped
+        # `value` counts the rays this build traces; the reference's own query count for the
+        # frame (every World::hit call, skipped shadow queries included) gives mrays_per_s
+        res["mrays_per_s_traced"] = round((rays - skipped) / (ms * 1e-3) / 1e6, 2)
+        res["_st_work"] = st_work
+    return res, path, cfg
 
 
 def bench_batch(args):
@@ -296,6 +472,10 @@ def bench_batch(args):
     print(json.dumps(line))
 
 
+SUB_CONFIGS = [("config 1", "sphere_800x600", False), ("config 2 as stated", "icosphere_square_720p", True),
+               ("config 3", "spheres_ring_1080p", True)]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -303,9 +483,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="icosphere_1080p")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes")
-    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--no-denoise", action="store_true", help="skip the denoised-vs-plain section")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations")
     ap.add_argument("--no-stage-events", action="store_true",
                     help="time the steps without per-stage hipEvents (stage times from as many profiled steps after)")
     ap.add_argument("--dn-seeds", default="100001,100002", help="denoiser scene seeds of the denoise section")
@@ -313,8 +494,8 @@ def main():
     ap.add_argument("--c4-scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--no-sequential", action="store_true", help="batch64: skip the one-by-one comparison")
     args = ap.parse_args()
-    if args.traffic_child:
-        traffic_child(args.workload)
+    if args.counters_child:
+        counters_child(args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the trace path has no multi-GPU mode in this build (asked for {args.gpus}).",
@@ -329,114 +510,84 @@ def main():
 
     import rtrace_amd as R
 
-    out_dir = os.path.join(tempfile.gettempdir(), "rtrace_amd_bench")
-    path, cfg = make_workload(args.workload, out_dir)
-    scene = R.Scene.from_json_file(path)
+    head, path, cfg = measure_frame(R, torch, args.workload, args.steps, args.warmup,
+                                    stage_events=not args.no_stage_events)
+    _progress(f"{args.workload}: {head['ms_per_step']} ms/frame, {head['mrays_per_s_traced']} Mrays/s traced")
     W, H = cfg["width"], cfg["height"]
-    r = R.Renderer(W, H)
-    configure(R, r, cfg, scene)
-    out = torch.empty(W * H * 3, dtype=torch.uint8, device="cuda")
-
-    # untimed passes: exact work counters, then per-stage hipEvent timing
-    r.count_work = True
-    st_work = r.render_device(scene, out)
-    r.count_work = False
-
-    # By default the timed steps record hipEvents around every stage's launches on the
-    # library's stream (profile), so the per-launch kernel times come from the timed region
-    # itself. With --no-stage-events the timed steps replay the frame's hipGraph without
-    # events (small frames, where ~6 event pairs per level weigh) and the stage times come
-    # from as many profiled steps after them.
-    stage_ms, stage_n = {}, {}
-
-    def add_stages(st_):
-        d_ = st_.as_dict()
-        for k, v in d_["ms_stage"].items():
-            stage_ms[k] = stage_ms.get(k, 0.0) + v
-            stage_n[k] = stage_n.get(k, 0) + d_["launches_stage"][k]
-
-    r.profile = not args.no_stage_events
-    for _ in range(args.warmup):
-        r.render_device(scene, out)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    dev_ms = []
-    for _ in range(args.steps):
-        st = r.render_device(scene, out)
-        dev_ms.append(st.ms_render)
-        if r.profile:
-            add_stages(st)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    if not r.profile:
-        r.profile = True
-        for _ in range(args.steps):
-            add_stages(r.render_device(scene, out))
-    r.profile = False
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    rays = st.rays_total
-    value_all = rays / (ms_per_step * 1e-3) / 1e6
-
-    # roofline of the dominant traversal kernel (bytes from the counters, time from events)
-    prof = {"ms_stage": {k: v / args.steps for k, v in stage_ms.items()},
-            "launches_stage": {k: v // args.steps for k, v in stage_n.items()}}
-    wk = st_work.as_dict()["work"]
-    # shadow queries whose light has no sample facing the hit point are not traced (their
-    # answers cannot change the image): one visibility byte read each
-    skipped = wk["shadow"]["skipped"]
-    traced = rays - skipped
-    # `value` counts the rays this build traces; the reference's own query count for the
-    # frame (every World::hit call, skipped shadow queries included) gives value_all
-    value = traced / (ms_per_step * 1e-3) / 1e6
+    st_work = head.pop("_st_work")
+    wk = head["work"]
+    skipped = head["shadow_skipped"]
+    ms_stage = head["stages_ms"]
+    launches = head["launches_stage"]
+    # roofline of the dominant traversal kernel (algorithmic bytes from the counters, time from events)
     kernels = {
         "k_shadow": (kernel_bytes(wk["shadow"], st_work.rays_shadow - skipped, B_RAY_SHADOW) + skipped,
-                     prof["ms_stage"]["shadow"], prof["launches_stage"]["shadow"]),
+                     ms_stage["shadow"], launches["shadow"]),
         "k_closest": (kernel_bytes(wk["closest"], st_work.rays_primary + st_work.rays_reflect, B_RAY_CLOSEST),
-                      prof["ms_stage"]["closest"], prof["launches_stage"]["closest"]),
+                      ms_stage["closest"], launches["closest"]),
     }
     dom = max(kernels, key=lambda k: kernels[k][1])
-    byt, ms, launches = kernels[dom]
+    byt, ms, nl = kernels[dom]
+    ms_launch = ms / max(nl, 1)
     achieved = byt / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    traced = head["rays_traced_per_frame"]
     line = {
-        "metric": "Mrays/s at 1080p" if H == 1080 else f"Mrays/s at {W}x{H}", "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+f64",
-        "data": "synthetic (seeded procedural scene)",
+        "metric": "Mrays/s at 1080p" if H == 1080 else f"Mrays/s at {W}x{H}", "value": head["mrays_per_s_traced"],
+        "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32+f64", "data": "synthetic (seeded procedural scene)",
         "config": {"workload": args.workload, "width": W, "height": H, "aa": cfg["aa"], "samples": cfg["samples"],
-                   "max_depth": cfg["max_depth"], "triangles": 81920 if args.workload.startswith("icosphere") else 0,
-                   "rays_per_frame": rays, "primary": st.rays_primary, "reflect": st.rays_reflect,
-                   "shadow": st.rays_shadow, "shadow_skipped": skipped, "rays_traced_per_frame": traced},
-        "mrays_per_s_reference_queries": round(value_all, 2),
+                   "max_depth": cfg["max_depth"], "triangles": TRIANGLES.get(args.workload.split("_")[0], 0),
+                   "rays_per_frame": head["rays_per_frame"], "primary": head["primary"], "reflect": head["reflect"],
+                   "shadow": head["shadow"], "shadow_skipped": skipped, "rays_traced_per_frame": traced},
+        "mrays_per_s_reference_queries": head["mrays_per_s"],
+        # `bound` names the roofline the figure is priced on (the memory roofline: traversal is
+        # pointer chasing; no MFMA in tracing); `limiter` says what the counters show bounds it
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_launch": int(byt / max(launches, 1)),
-                     "ms_per_launch": round(ms / max(launches, 1), 4)},
-        "stages_ms": {k: round(v, 3) for k, v in prof["ms_stage"].items()},
-        "kernel_timing": ("hipEvents around each stage's launches in %d profiled steps after the timed ones"
-                          % args.steps if args.no_stage_events else
-                          "hipEvents around each stage's launches in the timed steps"),
-        "device_ms_per_frame": round(sum(dev_ms) / len(dev_ms), 3),
+                     "achieved_is": "algorithmic bytes (every node, triangle record and ray record the walk "
+                                    "reads or writes, at their layout sizes) per launch / launch time",
+                     "bytes_per_launch": int(byt / max(nl, 1)), "ms_per_launch": round(ms_launch, 4)},
+        "stages_ms": ms_stage,
+        "kernel_timing": head["kernel_timing"],
+        "device_ms_per_frame": head["device_ms_per_frame"],
         "work": wk,
         # this build's traversal per traced ray (k-d nodes and FP32 triangle tests, both stages)
         "work_per_ray": {"kd_nodes": round((wk["closest"]["nodes"] + wk["shadow"]["nodes"]) / max(traced, 1), 1),
                          "bvh_nodes": round((wk["closest"]["bvh_nodes"] + wk["shadow"]["bvh_nodes"]) / max(traced, 1), 1),
                          "tri_tests": round((wk["closest"]["tris"] + wk["shadow"]["tris"]) / max(traced, 1), 1)},
     }
+    if not args.no_configs:
+        subs = {}
+        for label, wl, count in SUB_CONFIGS:
+            res, _, _ = measure_frame(R, torch, wl, max(5, args.steps // 2), max(2, args.warmup // 2),
+                                      stage_events=False, count=count)
+            res.pop("_st_work", None)
+            res.pop("work", None)
+            res["config"] = label
+            subs[wl] = res
+            _progress(f"{wl}: {res['ms_per_step']} ms/frame, {res['mrays_per_s']} Mrays/s")
+        line["configs"] = subs
     if not args.no_traffic:
-        t, err = measure_traffic(args.workload, dom)
-        if t:
-            line["roofline"]["traffic"] = int(t["FETCH_SIZE"] + t["WRITE_SIZE"])
-            line["roofline"]["traffic_note"] = (
-                f"HBM bytes/launch = FETCH_SIZE*{t['FETCH_SIZE_scale']:.3f} + WRITE_SIZE*{t['WRITE_SIZE_scale']:.3f} "
-                f"(KiB counters scaled by a {COPY_BYTES >> 20} MiB copy)")
-            if "l2" in t:
-                line["roofline"]["l2"] = dict(t["l2"], gbs=round(t["l2"]["bytes"] / (line["roofline"]["ms_per_launch"] * 1e-3) / 1e9, 1))
+        pmc, err = run_counters(args.workload)
+        if pmc:
+            cs = counter_summary(pmc, dom, ms_launch)
+            line["roofline"]["traffic"] = cs.get("hbm_bytes")
+            line["roofline"]["measured"] = cs
+            if "limiter" in cs:
+                line["roofline"]["limiter"] = cs["limiter"]
+            mf = mfma_summary(pmc)
+            if mf:
+                line["denoise_mfma_counters"] = mf
         else:
             line["roofline"]["traffic_note"] = err
     if not args.no_denoise:
         line["denoise"] = denoise_section([int(v) for v in args.dn_seeds.split(",")], args.dn_threshold)
+        if "denoise_mfma_counters" in line:
+            line["denoise"]["roofline"]["mfma_counters"] = line.pop("denoise_mfma_counters")
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(path, cfg)
+        line["cpu_baseline"], line["parity"] = cpu_baseline(path, cfg)
+        _progress(f"cpu baseline {line['cpu_baseline']['value']:.3f} Mrays/s; parity {line['parity']}")
     print(json.dumps(line))
 
 

@@ -11,6 +11,7 @@
 
 #include "../kernels/denoise_kernels.hpp"
 #include "context.hpp"
+#include "frame.hpp"
 #include "../../../include/rtrace_denoise.h"
 #include "scene_host.hpp"
 
@@ -122,6 +123,9 @@ struct rt_denoiser {
     rt::Buf x0, act[RT_DN_LAYERS], pool1, pool3, pool5;
     // training buffers
     rt::Buf g[RT_DN_LAYERS - 1], gpool, dz, dx, part, bpart, lpart;
+    // denoised frames: features, tile mask and flag count, the fallback render's frame buffers
+    rt::Buf feat, mask, flagged, fb_arena;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace rt {
@@ -336,6 +340,7 @@ int rt_denoiser_create(rt_ctx* ctx, uint64_t seed, rt_denoiser** out) {
         DN_TRY(hipSetDevice(ctx_device(ctx)));
         DN_TRY(hipEventCreate(&d->e0));
         DN_TRY(hipEventCreate(&d->e1));
+        for (auto& e : d->ev) DN_TRY(hipEventCreate(&e));
         const Offsets& o = offsets();
         d->n_params = o.total;
         // He-normal weights over the real fan-in (13 input features for layer 0); padded
@@ -375,6 +380,8 @@ void rt_denoiser_destroy(rt_denoiser* dn) {
     if (dn->st) (void)hipStreamSynchronize(dn->st);
     if (dn->e0) (void)hipEventDestroy(dn->e0);
     if (dn->e1) (void)hipEventDestroy(dn->e1);
+    for (auto e : dn->ev)
+        if (e) (void)hipEventDestroy(e);
     delete dn;
 }
 
@@ -487,51 +494,53 @@ int rt_render_denoised(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts,
         if (opts->outline_enabled && opts->aa_mode != RT_AA_QUINCUNX)
             throw Error(RT_E_INVALID_ARG, "the denoised render does not take the outline pass");
         const int W = opts->width, H = opts->height;
-        Buf feat, mask, flagged;
-        feat.reserve((size_t)W * H * RT_FEAT_COUNT * sizeof(float));
         const int tiles = ((W + tile - 1) / tile) * ((H + tile - 1) / tile);
-        mask.reserve(tiles);
-        flagged.reserve(sizeof(int));
+        dn->feat.reserve((size_t)W * H * RT_FEAT_COUNT * sizeof(float));
+        dn->mask.reserve(tiles);
+        dn->flagged.reserve(sizeof(int));
+        // The few-sample render uses the context's frame buffers (so a repeated frame replays its
+        // captured graph); the masked many-sample re-render uses the denoiser's own, so both
+        // frames' counters survive until the one host wait at the end.
+        rt_render_opts o1 = *opts;
+        o1.profile = 0;
+        rt_render_opts o2 = *opts;
+        o2.profile = 0;
+        o2.light_samples = fallback_samples;
+        dn->fb_arena.reserve(ctx_frame_bytes(scene, &o2));
+        const FixedScratch fb_arena{dn->fb_arena.as<char>(), dn->fb_arena.bytes};
         hipStream_t st = ctx_stream(ctx);
-        hipEvent_t ev[4];
-        for (auto& e : ev) DN_TRY(hipEventCreate(&e));
-        struct EvGuard {
-            hipEvent_t* e;
-            ~EvGuard() {
-                for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
-            }
-        } evg{ev};
         rt_render_aux aux{};
-        aux.features = feat.as<float>();
-        rt_render_stats s1{}, s2{};
-        DN_TRY(hipEventRecord(ev[0], st));
-        ctx_render_aux(ctx, scene, opts, d_rgb_out, &aux, &s1);
-        DN_TRY(hipEventRecord(ev[1], st));
+        aux.features = dn->feat.as<float>();
+        rt_render_aux a2{};
+        a2.tile_mask = dn->mask.as<uint8_t>();
+        a2.mask_tile = tile;
+        PendingFrame p1{}, p2{};
+        DN_TRY(hipEventRecord(dn->ev[0], st));
+        ctx_render_launch(ctx, scene, &o1, d_rgb_out, &aux, nullptr, &p1);
+        DN_TRY(hipEventRecord(dn->ev[1], st));
         ensure_shape(dn, 1, H, W, false);
-        forward(dn, feat.as<float>(), H, W);
-        DN_TRY(dn::launch_compose(dn->act[kOut].as<float>(), feat.as<float>(), H, W, dn->Wp, nullptr,
+        forward(dn, dn->feat.as<float>(), H, W);
+        DN_TRY(dn::launch_compose(dn->act[kOut].as<float>(), dn->feat.as<float>(), H, W, dn->Wp, nullptr,
                                   static_cast<uint8_t*>(d_rgb_out), st));
-        DN_TRY(hipMemsetAsync(flagged.p, 0, sizeof(int), st));
-        DN_TRY(dn::launch_tile_mask(feat.as<float>(), H, W, tile, threshold, mask.as<uint8_t>(), flagged.as<int>(), st));
+        DN_TRY(hipMemsetAsync(dn->flagged.p, 0, sizeof(int), st));
+        DN_TRY(dn::launch_tile_mask(dn->feat.as<float>(), H, W, tile, threshold, dn->mask.as<uint8_t>(),
+                                    dn->flagged.as<int>(), st));
+        DN_TRY(hipEventRecord(dn->ev[2], st));
+        // every flagged tile re-rendered with the many-sample light; with none flagged the
+        // launches find empty queues (no host round trip to find out)
+        ctx_render_launch(ctx, scene, &o2, d_rgb_out, &a2, &fb_arena, &p2);
+        DN_TRY(hipEventRecord(dn->ev[3], st));
         int n_flagged = 0;
-        DN_TRY(hipMemcpyAsync(&n_flagged, flagged.p, sizeof(int), hipMemcpyDeviceToHost, st));
-        DN_TRY(hipEventRecord(ev[2], st));
+        DN_TRY(hipMemcpyAsync(&n_flagged, dn->flagged.p, sizeof(int), hipMemcpyDeviceToHost, st));
         DN_TRY(hipStreamSynchronize(st));
-        if (n_flagged > 0) {
-            rt_render_opts o2 = *opts;
-            o2.light_samples = fallback_samples;
-            rt_render_aux a2{};
-            a2.tile_mask = mask.as<uint8_t>();
-            a2.mask_tile = tile;
-            ctx_render_aux(ctx, scene, &o2, d_rgb_out, &a2, &s2);
-        }
-        DN_TRY(hipEventRecord(ev[3], st));
-        DN_TRY(hipEventSynchronize(ev[3]));
         if (stats) {
+            rt_render_stats s1{}, s2{};
+            frame_stats(ctx, p1, &s1, true);
+            frame_stats(ctx, p2, &s2, false);
             float a = 0, b = 0, c = 0;
-            DN_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
-            DN_TRY(hipEventElapsedTime(&b, ev[1], ev[2]));
-            DN_TRY(hipEventElapsedTime(&c, ev[2], ev[3]));
+            DN_TRY(hipEventElapsedTime(&a, dn->ev[0], dn->ev[1]));
+            DN_TRY(hipEventElapsedTime(&b, dn->ev[1], dn->ev[2]));
+            DN_TRY(hipEventElapsedTime(&c, dn->ev[2], dn->ev[3]));
             stats->ms_render = a;
             stats->ms_denoise = b;
             stats->ms_fallback = c;
@@ -540,6 +549,8 @@ int rt_render_denoised(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts,
             stats->tiles_fallback = n_flagged;
             stats->rays = s1.rays_primary + s1.rays_reflect + s1.rays_shadow + s2.rays_primary + s2.rays_reflect +
                           s2.rays_shadow;
+        } else {
+            frame_stats(ctx, p1, nullptr, true);
         }
     });
 }

@@ -75,6 +75,29 @@ struct FrameLayout {
     size_t K = 1;    // shadow slots per hit
 };
 
+// Where a frame's buffers live when they are not in the context's arena: the fixed
+// device array of the no-allocation entry point, or an arena a caller owns.
+struct FixedScratch {
+    char* base;
+    size_t bytes;
+};
+
+// Deepest max_depth a frame in fixed buffers takes (its per-level counts are copied back
+// through fixed host arrays).
+constexpr int kNoAllocMaxDepth = 64;
+
+// A frame whose launches are enqueued and whose stats are read once its stream is done.
+struct SceneDevice;
+struct PendingFrame {
+    FrameView F;
+    SceneView S;
+    long long n_paths;
+    int depth;
+    bool timed_stages;
+    bool fixed;
+    const SceneDevice* D;
+};
+
 CameraDev make_camera(const rt_camera_desc& c, double aspect);
 // cam_pos: position of the camera `cam` was built from (nullptr = the scene's camera)
 SceneView make_view(const SceneDevice& D, const CameraDev& cam, int light_samples, const double* cam_pos = nullptr);
@@ -85,6 +108,15 @@ FrameLayout frame_fill(FrameView& F, const SceneView& S, const rt_render_opts* o
 // Places the frame's buffers in one allocation at `base` (nullptr: only measures); returns bytes.
 size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char* base, RayRec** q0, RayRec** q1,
                    bool own_rgb);
+
+// Stats of a launched frame; waits for ctx's stream (render.cpp).
+void frame_stats(rt_ctx* ctx, const PendingFrame& pf, rt_render_stats* stats, bool record_primary);
+// rt_render_device_aux enqueued without waiting: frame buffers in `arena` (nullable = the
+// context's arena); the stats come from frame_stats(*pending) after the stream is done.
+void ctx_render_launch(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
+                       const FixedScratch* arena, PendingFrame* pending);
+// Device bytes of a frame's buffers for these options.
+size_t ctx_frame_bytes(rt_scene* scene, const rt_render_opts* o);
 
 }  // namespace rt
 

@@ -507,19 +507,9 @@ uint64_t primary_rays(const FrameView& F, long long n_paths) {
     return n;
 }
 
-// Where a frame's buffers live when they are not in the context's arena: the fixed
-// device array of the no-allocation entry point (its address on the context's device).
-struct FixedScratch {
-    char* base;
-    size_t bytes;
-};
-
-// Deepest max_depth the no-allocation entry point takes (its per-level counts are copied
-// back through fixed host arrays).
-constexpr int kNoAllocMaxDepth = 64;
-
 void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host_rgb, void* dev_rgb,
-            rt_render_stats* stats, const rt_render_aux* aux = nullptr, const FixedScratch* fixed = nullptr) {
+            rt_render_stats* stats, const rt_render_aux* aux = nullptr, const FixedScratch* fixed = nullptr,
+            PendingFrame* pending = nullptr) {
     if (!scene || !scene->dev || !o) throw Error(RT_E_INVALID_ARG, "null scene or options");
     if (o->samples == 0) throw Error(RT_E_INVALID_ARG, "Samples must be greater than 0");
     if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
@@ -544,7 +534,7 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     }
     if (fixed && depth > kNoAllocMaxDepth)
         throw Error(RT_E_INVALID_ARG, "max_depth above " + std::to_string(kNoAllocMaxDepth) +
-                                          " needs rt_render_device (the no-allocation entry point keeps fixed buffers)");
+                                          " needs the context's own frame buffers (rt_render_device)");
     RayRec *q0 = nullptr, *q1 = nullptr;
     const size_t bytes = frame_carve(F, S, L, nullptr, &q0, &q1, dev_rgb == nullptr);
     char* base = nullptr;
@@ -619,12 +609,29 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     else HIP_TRY(launch_frame(S, F, q0, q1, st, timed_stages ? &ctx->timer : nullptr));
     HIP_TRY(hipEventRecord(ctx->ev1, st));
     if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
+    PendingFrame pf{F, S, n_paths, depth, timed_stages, fixed != nullptr, &D};
+    if (pending) {
+        *pending = pf;  // the caller reads the stats once it has synchronised the stream
+        return;
+    }
+    frame_stats(ctx, pf, stats, true);  // waits for the stream
+}
+
+}  // namespace
+
+// Stats of a launched frame (waits for ctx's stream): per-level queue lengths, work
+// counters and, for the context's last frame, its device time and stage times.
+// `record_primary` makes it the frame rt_last_primary_hits reads.
+void frame_stats(rt_ctx* ctx, const PendingFrame& pf, rt_render_stats* stats, bool record_primary) {
+    hipStream_t st = ctx->stream;
+    const FrameView& F = pf.F;
+    const int depth = pf.depth;
     // per-level queue lengths (fixed host arrays on the no-allocation path)
     std::vector<int32_t> counts_v, hits_v;
     int32_t counts_a[kNoAllocMaxDepth + 2] = {}, hits_a[kNoAllocMaxDepth + 1] = {};
     int32_t* counts = counts_a;
     int32_t* hit_count = hits_a;
-    if (!fixed) {
+    if (!pf.fixed) {
         counts_v.assign(depth + 2, 0);
         hits_v.assign(depth + 1, 0);
         counts = counts_v.data();
@@ -636,42 +643,47 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
         HIP_TRY(hipMemcpyAsync(hit_count, F.hit_count, depth * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(work, F.work, sizeof(work), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
-    ctx->last_paths = (int)n_paths;
-    ctx->last_prim_obj = F.prim_obj;
-    ctx->last_prim_tri = F.prim_tri;
-    ctx->last_prim_t = F.prim_t;
-    if (stats) {
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        memset(stats, 0, sizeof(*stats));
-        stats->rays_primary = depth > 0 ? primary_rays(F, n_paths) : 0;
-        for (int l = 1; l < depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
-        for (int l = 0; l < depth; ++l) stats->rays_shadow += (uint64_t)hit_count[l] * (uint64_t)(S.n_lights > 0 ? S.shadow_per_hit : 0);
-        for (int k = 0; k < 2; ++k) {
-            const unsigned long long* wk = work + k * kWorkCount;
-            stats->work[k].nodes = wk[kWorkNodes];
-            stats->work[k].tris = wk[kWorkTris];
-            stats->work[k].retests = wk[kWorkRetests];
-            stats->work[k].spheres = wk[kWorkSphere];
-            stats->work[k].planes = wk[kWorkPlane];
-            stats->work[k].cubes = wk[kWorkCube];
-            stats->work[k].skipped = wk[kWorkSkipped];
-            stats->work[k].bvh_nodes = wk[kWorkBvhNodes];
-        }
-        stats->ms_render = ms;
-        stats->ms_upload = D.ms_upload;
-        stats->levels = depth;
-        if (timed_stages) {
-            for (size_t i = 0; i < ctx->timer.used / 2; ++i) {
-                float e = 0;
-                HIP_TRY(hipEventElapsedTime(&e, ctx->timer.ev[2 * i], ctx->timer.ev[2 * i + 1]));
-                int sidx = ctx->timer.stage[i];
-                stats->ms_stage[sidx] += e;
-                stats->launches_stage[sidx] += 1;
-            }
+    if (record_primary) {
+        ctx->last_paths = (int)pf.n_paths;
+        ctx->last_prim_obj = F.prim_obj;
+        ctx->last_prim_tri = F.prim_tri;
+        ctx->last_prim_t = F.prim_t;
+    }
+    if (!stats) return;
+    memset(stats, 0, sizeof(*stats));
+    stats->rays_primary = depth > 0 ? primary_rays(F, pf.n_paths) : 0;
+    for (int l = 1; l < depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
+    for (int l = 0; l < depth; ++l)
+        stats->rays_shadow += (uint64_t)hit_count[l] * (uint64_t)(pf.S.n_lights > 0 ? pf.S.shadow_per_hit : 0);
+    for (int k = 0; k < 2; ++k) {
+        const unsigned long long* wk = work + k * kWorkCount;
+        stats->work[k].nodes = wk[kWorkNodes];
+        stats->work[k].tris = wk[kWorkTris];
+        stats->work[k].retests = wk[kWorkRetests];
+        stats->work[k].spheres = wk[kWorkSphere];
+        stats->work[k].planes = wk[kWorkPlane];
+        stats->work[k].cubes = wk[kWorkCube];
+        stats->work[k].skipped = wk[kWorkSkipped];
+        stats->work[k].bvh_nodes = wk[kWorkBvhNodes];
+    }
+    stats->ms_upload = pf.D->ms_upload;
+    stats->levels = depth;
+    if (!record_primary) return;  // a pending frame: its events may have been recorded over
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    stats->ms_render = ms;
+    if (pf.timed_stages) {
+        for (size_t i = 0; i < ctx->timer.used / 2; ++i) {
+            float e = 0;
+            HIP_TRY(hipEventElapsedTime(&e, ctx->timer.ev[2 * i], ctx->timer.ev[2 * i + 1]));
+            int sidx = ctx->timer.stage[i];
+            stats->ms_stage[sidx] += e;
+            stats->launches_stage[sidx] += 1;
         }
     }
 }
+
+namespace {
 
 int fail(rt_ctx* ctx, int code, const std::string& msg) {
     if (ctx) ctx->err = msg;
@@ -756,6 +768,19 @@ int ctx_fail(rt_ctx* ctx, int code, const std::string& msg) { return fail(ctx, c
 void ctx_render_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
                     rt_render_stats* stats) {
     render(ctx, scene, o, nullptr, dev_rgb, stats, aux);
+}
+void ctx_render_launch(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
+                       const FixedScratch* arena, PendingFrame* pending) {
+    render(ctx, scene, o, nullptr, dev_rgb, nullptr, aux, arena, pending);
+}
+size_t ctx_frame_bytes(rt_scene* scene, const rt_render_opts* o) {
+    SceneDevice& D = *scene->dev;
+    CameraDev cam = make_camera(D.host.camera, (double)o->width / (double)o->height);
+    SceneView S = make_view(D, cam, o->light_samples);
+    FrameView F{};
+    const FrameLayout L = frame_fill(F, S, o, kRaygenTile);
+    RayRec *q0 = nullptr, *q1 = nullptr;
+    return frame_carve(F, S, L, nullptr, &q0, &q1, false);
 }
 }  // namespace rt
 

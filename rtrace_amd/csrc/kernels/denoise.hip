@@ -13,6 +13,7 @@
 // x shift) so that both MFMA operands are 16-byte aligned 8-pixel runs.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "denoise_kernels.hpp"
@@ -36,6 +37,21 @@ constexpr int kHH = kTH + 2, kHW = kTW + 2;        // input halo
 constexpr int kCK = 32;                            // input channels per K chunk
 constexpr int kPS = 40;                            // LDS pixel stride in bf16 (80 B)
 constexpr int kNT = 32;                            // output channels per workgroup
+
+// The same vector without branches and without the zeroing: the address is clamped into
+// the image and *ok says whether the pixel is inside it (outside pixels read as zero once
+// the caller applies *ok). The load's value is not touched here, so a prefetch is not
+// waited for until the value is stored.
+__device__ __forceinline__ uint4 gather8_raw(const ConvIn& in, int b, int y, int x, int H, int W, int c, bool* ok) {
+    *ok = y >= 0 && x >= 0 && y < H && x < W;
+    const int yc = min(max(y, 0), H - 1), xc = min(max(x, 0), W - 1);
+    const int h2 = H >> 1, w2 = W >> 1;
+    const size_t i0 = in.up0 ? (((size_t)b * h2 + (yc >> 1)) * w2 + (xc >> 1)) : (((size_t)b * H + yc) * W + xc);
+    const size_t i1 = ((size_t)b * H + yc) * W + xc;
+    const bool first = c < in.c0;
+    const uint16_t* p = first ? in.src0 + i0 * in.c0 + c : in.src1 + i1 * in.c1 + (c - in.c0);
+    return *reinterpret_cast<const uint4*>(p);
+}
 
 // The same vector without branches: the address is selected and clamped into the image,
 // and a pixel outside it reads as zero. Staging loops issue all their loads before the
@@ -70,7 +86,10 @@ __device__ __forceinline__ uint4 gather8(const ConvIn& in, int b, int y, int x, 
     return *reinterpret_cast<const uint4*>(p + (c - in.c0));
 }
 
-__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
+// One pixel tile per workgroup, all K chunks: layers with several K chunks (one tile's
+// chunks in flight per workgroup and many workgroups per CU measured faster there than the
+// persistent kernel below: 1080p dec0a (96 -> 32) 229 us against 424 us).
+__global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
     // one spare pixel slot each: the staging threads with no vector to store write there, so
     // the staging has no branches and all of its loads are in flight together
     __shared__ __attribute__((aligned(16))) uint16_t s_in[(kHH * kHW + 1) * kPS];
@@ -162,6 +181,181 @@ __global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
                 }
             }
         }
+    }
+}
+
+// Persistent and software-pipelined: workgroup (bx, by) computes output channels
+// [32 by, 32 by + 32) of the pixel tiles bx, bx + gridDim.x, ... A step is one (tile, 32-channel
+// K chunk); the global loads of the next step are issued into registers before the current
+// step's MFMAs, so they are in flight while the matrix cores work, then stored into LDS
+// between two barriers. A layer with a single K chunk stages its weights once. Per output
+// element the MFMAs accumulate in (K chunk, tap) order, as a one-tile-per-workgroup kernel
+// would.
+// LDS (37.6 KB per workgroup) allows 4 workgroups, i.e. 4 waves per SIMD: the register
+// budget of that occupancy (128) holds the prefetched step without parking it elsewhere.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_conv3x3(ConvArgs a) {
+    // one spare pixel slot each: the staging threads with no vector to store write there, so
+    // the staging has no branches and all of its loads are in flight together
+    __shared__ __attribute__((aligned(16))) uint16_t s_in[(kHH * kHW + 1) * kPS];
+    __shared__ __attribute__((aligned(16))) uint16_t s_w[(9 * kNT + 1) * kPS];
+    const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
+    const int n_tiles = a.B * tiles_x * tiles_y;
+    const int co0 = blockIdx.y * kNT;
+    const int cin = a.in.c0 + a.in.c1;
+    const int n_chunks = cin / kCK;
+    const bool w_once = n_chunks == 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, r16 = lane & 15;
+
+    constexpr int kHaloVec = kHH * kHW * 4, kWVec = 9 * kNT * 4;  // 16-byte vectors per K chunk
+    constexpr int kHaloPer = (kHaloVec + 255) / 256, kWPer = (kWVec + 255) / 256;
+    uint4 hv[kHaloPer], wv[kWPer];
+    uint32_t hok = 0;  // bit r: hv[r] is inside the image (else it stores as zero)
+    // Halo loads as raw buffer loads when both sources are under 2 GiB: a pixel outside the
+    // image gets an offset past the buffer's end and the hardware returns zeros, so nothing
+    // selects on a loaded value before it is stored.
+    const int h2 = a.H >> 1, w2 = a.W >> 1;
+    const unsigned long long bytes0 =
+        (unsigned long long)a.B * (a.in.up0 ? (unsigned long long)h2 * w2 : (unsigned long long)a.H * a.W) * a.in.c0 * 2;
+    const unsigned long long bytes1 = (unsigned long long)a.B * a.H * a.W * a.in.c1 * 2;
+    const bool use_buf = bytes0 < (1ull << 31) && bytes1 < (1ull << 31);
+    auto tile_origin = [&](int t, int& b, int& y0, int& x0) {
+        b = t / (tiles_x * tiles_y);
+        t -= b * tiles_x * tiles_y;
+        y0 = (t / tiles_x) * kTH;
+        x0 = (t % tiles_x) * kTW;
+    };
+    // every load of a step first (indices clamped, no branches); the stores come later
+    auto load_step = [&](int t, int ci0, bool with_w) {
+        int b, y0, x0;
+        tile_origin(t, b, y0, x0);
+        hok = 0;
+        if (use_buf) {
+            // a K chunk lies within one source (both channel counts are multiples of 32)
+            const bool first = ci0 < a.in.c0;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)(first ? a.in.src0 : a.in.src1), (short)0, (int)(first ? bytes0 : bytes1), 0x00020000);
+#pragma unroll
+            for (int r = 0; r < kHaloPer; ++r) {
+                const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
+                const int px = v >> 2, q = v & 3;
+                const int hy = px / kHW, hx = px - hy * kHW;
+                const int y = y0 + hy - 1, x = x0 + hx - 1, c = ci0 + q * 8;
+                const bool ok = y >= 0 && x >= 0 && y < a.H && x < a.W;
+                const uint32_t e = first ? (uint32_t)(a.in.up0 ? ((b * h2 + (y >> 1)) * w2 + (x >> 1))
+                                                              : ((b * a.H + y) * a.W + x)) * (uint32_t)a.in.c0 + (uint32_t)c
+                                         : (uint32_t)((b * a.H + y) * a.W + x) * (uint32_t)a.in.c1 +
+                                               (uint32_t)(c - a.in.c0);
+                const auto w4 = __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? e * 2u : 0x80000000u, 0, 0);
+                hv[r] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+            }
+            hok = (1u << kHaloPer) - 1u;
+        } else {
+#pragma unroll
+            for (int r = 0; r < kHaloPer; ++r) {
+                const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
+                const int px = v >> 2, q = v & 3;
+                const int hy = px / kHW, hx = px - hy * kHW;
+                bool ok;
+                hv[r] = gather8_raw(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8, &ok);
+                hok |= ok ? 1u << r : 0u;
+            }
+        }
+        if (with_w) {
+#pragma unroll
+            for (int r = 0; r < kWPer; ++r) {
+                const int v = min((int)threadIdx.x + r * 256, kWVec - 1);
+                const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
+                wv[r] = *reinterpret_cast<const uint4*>(a.wt + ((size_t)tap * a.coutp + co0 + n) * cin + ci0 + q * 8);
+            }
+        }
+    };
+    auto store_halo = [&]() {
+#pragma unroll
+        for (int r = 0; r < kHaloPer; ++r) {
+            const int v = (int)threadIdx.x + r * 256;
+            const int slot = v < kHaloVec ? (v >> 2) * kPS + (v & 3) * 8 : kHH * kHW * kPS;
+            *reinterpret_cast<uint4*>(s_in + slot) = ((hok >> r) & 1u) ? hv[r] : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store_w = [&]() {
+#pragma unroll
+        for (int r = 0; r < kWPer; ++r) {
+            const int v = (int)threadIdx.x + r * 256;
+            const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
+            const int slot = v < kWVec ? (tap * kNT + n) * kPS + q * 8 : 9 * kNT * kPS;
+            *reinterpret_cast<uint4*>(s_w + slot) = wv[r];
+        }
+    };
+
+    int tile = blockIdx.x, chunk = 0;
+    if (tile >= n_tiles) return;
+    load_step(tile, 0, true);
+    if (w_once) store_w();  // ordered before the first MFMA by the loop's second barrier
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (;;) {
+        __syncthreads();  // the previous step's MFMAs are done with the LDS
+        store_halo();
+        if (!w_once) store_w();
+        __syncthreads();
+        // the next step's loads go out now and land while this step's MFMAs run
+        int ntile = tile, nchunk = chunk + 1;
+        if (nchunk == n_chunks) {
+            nchunk = 0;
+            ntile += gridDim.x;
+        }
+        const bool more = ntile < n_tiles;
+        if (more) load_step(ntile, nchunk * kCK, !w_once);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int dy = tap / 3, dx = tap - dy * 3;
+            bf16x8 bfr[2];
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+                bfr[nt] = *reinterpret_cast<const bf16x8*>(s_w + (tap * kNT + nt * 16 + r16) * kPS + g * 8);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int row = wave * 2 + mt;
+                bf16x8 afr = *reinterpret_cast<const bf16x8*>(s_in + ((row + dy) * kHW + r16 + dx) * kPS + g * 8);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nt], acc[mt][nt], 0, 0, 0);
+            }
+        }
+        if (chunk == n_chunks - 1) {
+            // D layout: lane holds rows (pixels) 4g..4g+3 of column (channel) r16
+            int b, y0, x0;
+            tile_origin(tile, b, y0, x0);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int y = y0 + wave * 2 + mt;
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) {
+                    const int co = co0 + nt * 16 + r16;
+                    const float bias = a.bias && co < a.cout ? a.bias[co] : 0.0f;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int x = x0 + 4 * g + i;
+                        if (y < a.H && co < a.cout && x < a.W) {
+                            float v = acc[mt][nt][i] + bias;
+                            const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.out_pitch + co;
+                            if (a.out_f32) {
+                                a.out_f32[o] = v;
+                            } else {
+                                if (a.relu) v = fmaxf(v, 0.0f);
+                                a.out_bf16[o] = f2bf(v);
+                            }
+                        }
+                    }
+                    acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                }
+            }
+        }
+        if (!more) break;
+        tile = ntile;
+        chunk = nchunk;
     }
 }
 
@@ -259,44 +453,69 @@ __global__ void k_reduce_splits(const float* part, int splits, long long n, floa
 
 // Input features -> 32-channel bf16 network input (channels 13..31 zero), zero outside the
 // frame (the network runs on a size rounded up to a multiple of 8).
+// One padded pixel per thread: the 13 features (normalised), written as 32 bf16 channels
+// with four 16-byte stores.
 __global__ void k_prep(const float* feat, int B, int H, int W, int Hp, int Wp, uint16_t* x0) {
     const long long n = (long long)B * Hp * Wp;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const int x = (int)(i % Wp);
         const int y = (int)((i / Wp) % Hp);
         const int b = (int)(i / ((long long)Wp * Hp));
-        uint16_t* o = x0 + i * 32;
         float v[kFeatIn];
         if (y < H && x < W) {
             const float* f = feat + (((size_t)b * H + y) * W + x) * kFeatIn;
+#pragma unroll
             for (int c = 0; c < kFeatIn; ++c) v[c] = f[c];
+#pragma unroll
             for (int c = 0; c < 3; ++c) v[c] = fminf(fmaxf(v[c], 0.0f), 4.0f);  // radiance
             v[9] = log1pf(fmaxf(v[9], 0.0f)) * 0.25f;                             // depth
             v[11] = sqrtf(fmaxf(v[11], 0.0f)) * 4.0f;                             // visibility std
         } else {
+#pragma unroll
             for (int c = 0; c < kFeatIn; ++c) v[c] = 0.0f;
         }
-        for (int c = 0; c < 32; ++c) o[c] = f2bf(c < kFeatIn ? v[c] : 0.0f);
+        uint32_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float lo = 2 * k < kFeatIn ? v[2 * k] : 0.0f, hi = 2 * k + 1 < kFeatIn ? v[2 * k + 1] : 0.0f;
+            w[k] = (uint32_t)f2bf(lo) | ((uint32_t)f2bf(hi) << 16);
+        }
+        uint4* o = reinterpret_cast<uint4*>(x0 + i * 32);
+        o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+        o[2] = make_uint4(0, 0, 0, 0);
+        o[3] = make_uint4(0, 0, 0, 0);
     }
 }
 
-// 2x2 average pooling of a bf16 tensor (H, W even).
+// 2x2 average pooling, 8 channels (one 16-byte vector) of one output pixel per thread.
 __global__ void k_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out) {
-    const int h2 = H >> 1, w2 = W >> 1;
-    const long long n = (long long)B * h2 * w2 * C;
+    const int h2 = H >> 1, w2 = W >> 1, cv = C / 8;
+    const long long n = (long long)B * h2 * w2 * cv;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % C);
-        long long p = i / C;
+        const int c = (int)(i % cv) * 8;
+        const long long p = i / cv;
         const int x = (int)(p % w2);
         const int y = (int)((p / w2) % h2);
         const int b = (int)(p / ((long long)w2 * h2));
         const uint16_t* q = in + (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
-        float s = (bf2f(q[0]) + bf2f(q[C])) + (bf2f(q[(size_t)W * C]) + bf2f(q[(size_t)W * C + C]));
-        out[i] = f2bf(s * 0.25f);
+        const uint4 a = *reinterpret_cast<const uint4*>(q), bq = *reinterpret_cast<const uint4*>(q + C);
+        const uint4 cq = *reinterpret_cast<const uint4*>(q + (size_t)W * C);
+        const uint4 d = *reinterpret_cast<const uint4*>(q + (size_t)W * C + C);
+        const uint16_t* ha = reinterpret_cast<const uint16_t*>(&a);
+        const uint16_t* hb = reinterpret_cast<const uint16_t*>(&bq);
+        const uint16_t* hc = reinterpret_cast<const uint16_t*>(&cq);
+        const uint16_t* hd = reinterpret_cast<const uint16_t*>(&d);
+        uint32_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const float s0 = (bf2f(ha[2 * k]) + bf2f(hb[2 * k])) + (bf2f(hc[2 * k]) + bf2f(hd[2 * k]));
+            const float s1 = (bf2f(ha[2 * k + 1]) + bf2f(hb[2 * k + 1])) + (bf2f(hc[2 * k + 1]) + bf2f(hd[2 * k + 1]));
+            r[k] = (uint32_t)f2bf(s0 * 0.25f) | ((uint32_t)f2bf(s1 * 0.25f) << 16);
+        }
+        *reinterpret_cast<uint4*>(out + p * C + c) = make_uint4(r[0], r[1], r[2], r[3]);
     }
 }
-
-// g_full[b][y][x][c] += 0.25 * g_half[b][y/2][x/2][c]
 __global__ void k_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf) {
     const long long n = (long long)B * H * W * C;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -454,9 +673,33 @@ int grid1(long long n, int block, int cap) {
 }
 }  // namespace
 
+// Workgroups of k_conv3x3 resident on the device at once (every CU at the kernel's
+// occupancy), queried once per device.
+int conv_resident() {
+    static int cached[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3, 256, 0) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        cached[dev] = cus * per_cu;
+    }
+    return cached[dev];
+}
 hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
     const int tiles = a.B * ((a.H + kTH - 1) / kTH) * ((a.W + kTW - 1) / kTW);
-    hipLaunchKernelGGL(k_conv3x3, dim3(tiles, a.coutp / kNT), dim3(256), 0, st, a);
+    const int ny = a.coutp / kNT;
+    // A layer with one K chunk keeps its weights in LDS: one resident wave of workgroups over
+    // all channel blocks, each taking tiles strided by gridDim.x.
+    if (a.in.c0 + a.in.c1 != kCK) {
+        hipLaunchKernelGGL(k_conv3x3_tile, dim3(tiles, ny), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
+    const int nx = std::max(1, std::min(tiles, conv_resident() / ny));
+    hipLaunchKernelGGL(k_conv3x3, dim3(nx, ny), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st) {
@@ -478,8 +721,8 @@ hipError_t launch_prep(const float* feat, int B, int H, int W, int Hp, int Wp, u
     return hipGetLastError();
 }
 hipError_t launch_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_pool, dim3(grid1((long long)B * H * W * C / 4, 256, 8192)), dim3(256), 0, st, in, B, H, W, C,
-                       out);
+    hipLaunchKernelGGL(k_pool, dim3(grid1((long long)B * (H / 2) * (W / 2) * (C / 8), 256, 8192)), dim3(256), 0, st, in,
+                       B, H, W, C, out);
     return hipGetLastError();
 }
 hipError_t launch_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf, hipStream_t st) {

@@ -185,6 +185,14 @@ def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06
         noisy = out.cpu().numpy().reshape(height, width, 3).copy()
         ms_dn, st_dn = timed(lambda: dn.render(r, scene, out, fallback_samples=high, threshold=threshold, tile=tile))
         den = out.cpu().numpy().reshape(height, width, 3).copy()
+        # stage split of the few-sample feature render (one profiled run after the timed ones)
+        feat = torch.empty(height * width * 13, dtype=torch.float32, device="cuda")
+        scratch = torch.empty_like(out)
+        r.profile = True
+        st_f = r.render_device_aux(scene, scratch, features=feat)
+        r.profile = False
+        del feat, scratch
+        render_stages = {k: round(v, 3) for k, v in st_f.as_dict()["ms_stage"].items()}
         row = {"seed": s, "ms_plain_high": round(ms_plain, 3), "ms_plain_low": round(ms_low, 3),
                "ms_denoised": round(ms_dn, 3), "speedup": round(ms_plain / ms_dn, 2),
                "psnr_low": round(psnr(noisy, ref), 2), "psnr_denoised": round(psnr(den, ref), 2),
@@ -192,6 +200,7 @@ def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06
                "tiles_fallback": st_dn.tiles_fallback, "tiles": st_dn.tiles,
                "stage_ms": {"render": round(st_dn.ms_render, 3), "denoise": round(st_dn.ms_denoise, 3),
                             "fallback": round(st_dn.ms_fallback, 3)},
+               "render_stages_ms": render_stages,
                "rays_plain_high": int(st_plain.rays_total)}
         rows.append(row)
         if log:

@@ -345,6 +345,7 @@ RENDER = {
     "icosphere_720p": dict(width=1280, height=720, aa="quincunx", samples=1, max_depth=4),
     "icosphere_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
     "icosphere_square_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
+    "icosphere_square_720p": dict(width=1280, height=720, aa="quincunx", samples=1, max_depth=4),
     "spheres_ring_1080p": dict(width=1920, height=1080, aa="stochastic", samples=4, max_depth=10),
     "denoise_1080p": dict(width=1920, height=1080, aa="quincunx", samples=1, max_depth=4),
 }

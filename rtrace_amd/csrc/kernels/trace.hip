@@ -206,6 +206,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     for (int hq = xcd_block() * blockDim.x + threadIdx.x; hq < n; hq += stride) shade_finish_item(S, F, hq);
 }
 
+// shade_finish for frames with few hits and many light samples (a tile-masked many-sample
+// re-render): one wavefront per hit, its lanes compute the area-light sample terms into LDS,
+// then one lane sums them in sample order (the same sum as the per-hit loop, which would
+// otherwise run 256 samples serially on a few thousand lanes).
+constexpr int kWideSlots = 512;
+__global__ __launch_bounds__(256) void k_shade_finish_wide(SceneView S, FrameView F, int level) {
+    __shared__ double s_term[4][kWideSlots * 3];
+    const int wave = (int)threadIdx.x >> 6, lane = __lane_id();
+    const int n = F.hit_count[level];
+    double* term = s_term[wave];
+    const int slots = S.shadow_per_hit;
+    for (int hq = (int)blockIdx.x * 4 + wave; hq < n; hq += (int)gridDim.x * 4) {  // wave-uniform
+        for (int j = lane; j < slots; j += 64) {
+            const d3 t = area_slot_term(S, F, hq, j);
+            term[3 * j] = t.x;
+            term[3 * j + 1] = t.y;
+            term[3 * j + 2] = t.z;
+        }
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this lane's LDS writes are done
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) shade_finish_item(S, F, hq, term);
+        __builtin_amdgcn_wave_barrier();  // the sum has read the terms before they are replaced
+    }
+}
+
 __global__ __launch_bounds__(256) void k_pixel(FrameView F) {
     int n = F.W * F.H;
     int stride = gridDim.x * blockDim.x;
@@ -384,7 +409,11 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
         }
         mark(3, false);
         mark(4, true);
-        hipLaunchKernelGGL(k_shade_finish, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);
+        if (F.tmask && S.n_lights > 0 && S.shadow_per_hit >= 32 && S.shadow_per_hit <= kWideSlots)
+            hipLaunchKernelGGL(k_shade_finish_wide, dim3(grid_for((long long)F.n_paths * 64, 256, kCap)), dim3(256), 0, st,
+                               S, F, level);
+        else
+            hipLaunchKernelGGL(k_shade_finish, dim3(grid_for(F.n_paths, 256, kCap)), dim3(256), 0, st, S, F, level);
         mark(4, false);
     }
     mark(5, true);

@@ -1456,9 +1456,44 @@ RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRe
 }
 
 // ------------------------------------------------------------ shade, part 2
+// Phong term of shadow slot j of hit hq when that slot belongs to an area light and its
+// sample is visible, else zero: the term shade_finish_item adds for that sample, from the
+// same inputs (precomputed across the lanes of a wavefront by k_shade_finish_wide).
+RT_HD d3 area_slot_term(const SceneView& S, const FrameView& F, int hq, int j) {
+    const ShadeRec& sr = F.shade[hq];
+    const uint8_t* vis = F.vis + (size_t)hq * S.shadow_per_hit;
+    if (!vis[j]) return mk(0, 0, 0);
+    int k = 0;  // stored (point and disk) sample points before this light
+    for (int li = 0, jj = 0; li < S.n_lights; ++li) {
+        const LightDev& L = S.lights[li];
+        const int cnt = L.is_area ? light_count(S, L) : 1;
+        if (j < jj + cnt) {
+            if (!L.is_area) return mk(0, 0, 0);
+            const int smp = j - jj;
+            const d3 p = ld3(sr.p), n = ld3(sr.n), mc = ld3(sr.mc);
+            const MatDev& m = S.mats[sr.mat];
+            const d3 vd = unit(sub(ld3(S.cam_pos), p));
+            const d3 lc = ld3(L.color);
+            d3 sp;
+            if (L.shape == kLightSquare) {
+                const AreaFrame a = area_frame(F.path_seed[sr.path], p, ld3(L.pos));
+                sp = square_sample(L, a, smp, cnt, light_cols(S, L));
+            } else {
+                sp = ld3(F.spt + ((size_t)hq * S.spt_per_hit + k + smp) * 3);
+            }
+            return phong_sample(m, L, lc, mc, p, n, vd, sp);
+        }
+        jj += cnt;
+        if (!L.is_area || L.shape != kLightSquare) k += cnt;
+    }
+    return mk(0, 0, 0);
+}
+
 // phong_lighting (src/lighting.rs:220-288) with visibility from the shadow stage, then
 // apply_fog (:297-319). Writes the level record (colour, reflectivity or -1).
-RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
+// pre (nullable): the area-light sample terms per shadow slot (area_slot_term), summed here
+// in sample order exactly as the terms computed in place would be.
+RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq, const double* pre = nullptr) {
     const ShadeRec& sr = F.shade[hq];
     d3 p = ld3(sr.p), n = ld3(sr.n), mc = ld3(sr.mc);
     const MatDev& m = S.mats[sr.mat];
@@ -1501,13 +1536,14 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq) {
             for (int s = 0; s < cnt; ++s) {
                 if (!vis[j + s]) continue;
                 ++visible;
-                total = add(total, phong_sample(m, L, lc, mc, p, n, vd, square_sample(L, a, s, cnt, cols)));
+                total = add(total, pre ? ld3(pre + 3 * (j + s))
+                                       : phong_sample(m, L, lc, mc, p, n, vd, square_sample(L, a, s, cnt, cols)));
             }
         } else {
             for (int s = 0; s < cnt; ++s) {
                 if (!vis[j + s]) continue;
                 ++visible;
-                total = add(total, phong_sample(m, L, lc, mc, p, n, vd, ld3(spt + 3 * (k + s))));
+                total = add(total, pre ? ld3(pre + 3 * (j + s)) : phong_sample(m, L, lc, mc, p, n, vd, ld3(spt + 3 * (k + s))));
             }
             k += cnt;
         }

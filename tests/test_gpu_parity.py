@@ -341,14 +341,15 @@ def test_features_and_radiance_vs_oracle(R, config_dir, aa, spp):
     assert 0 < np.mean((f[..., 10] > 0) & (f[..., 10] < 1)) < 0.5  # a penumbra exists
 
 
-@pytest.mark.parametrize("aa", ["quincunx", "stochastic"])
-def test_tile_masked_rerender_composites_exactly(R, config_dir, aa):
+@pytest.mark.parametrize("aa,light", [("quincunx", "square"), ("stochastic", "square"), ("quincunx", "disk")])
+def test_tile_masked_rerender_composites_exactly(R, tmp_path, aa, light):
     """Re-rendering a set of tiles at 64 light samples over a 4-sample frame leaves the
-    other tiles untouched and makes the masked tiles equal to a full 64-sample frame."""
+    other tiles untouched and makes the masked tiles equal to a full 64-sample frame (the
+    masked frame sums its many light samples with the wavefront-per-hit finish kernel, the
+    full frame with the per-hit loop)."""
     import torch
     from rtrace_amd import scenes
-    path = scenes.icosphere_scene(os.path.dirname(config_dir["icosphere_s4"]), subdiv=4, light="square", light_size=2.0,
-                          light_samples=4)
+    path = scenes.icosphere_scene(str(tmp_path), subdiv=4, light=light, light_size=2.0, light_samples=4)
     W, H, T = 150, 90, 16
     scene = R.Scene.from_json_file(path)
     r = R.Renderer(W, H)

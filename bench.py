@@ -416,7 +416,10 @@ def bench_batch(args):
             _progress(f"loaded {len(tris) // 4} scenes ({time.perf_counter() - t0:.1f} s)")
     t_load = time.perf_counter() - t0
     tune_jobs = jobs[::max(1, len(jobs) // 32)]
-    opts, tune_ms = R.autotune_batch(tune_jobs)
+    opts0 = R.default_batch_opts()
+    if args.c4_group_gib is not None:
+        opts0.group_gib = args.c4_group_gib
+    opts, tune_ms = R.autotune_batch(tune_jobs, opts0)
     _progress(f"autotuned on {len(tune_jobs)} jobs: {opts.as_dict()}")
     b = R.Batch(jobs, opts)
     for _ in range(args.warmup):
@@ -463,7 +466,8 @@ def bench_batch(args):
                    "primary": st.rays_primary, "reflect": st.rays_reflect, "shadow": st.rays_shadow},
         "batch": {"opts": opts.as_dict(), "tune_ms_subset": round(tune_ms, 3), "tune_jobs": len(tune_jobs),
                   "graph_nodes": st.graph_nodes, "graph_launches_per_step": 1, "device_ms": round(min(ms_dev), 3),
-                  "frame_buffer_gib": round(st.bytes / 2**30, 2), "build_ms": round(st.ms_build, 1),
+                  "frame_buffer_gib": round(st.bytes / 2**30, 2), "job_groups": st.groups,
+                  "build_ms": round(st.ms_build, 1),
                   "scene_load_s": round(t_load, 1)},
     }
     if seq_ms is not None:
@@ -493,6 +497,8 @@ def main():
     ap.add_argument("--dn-threshold", type=float, default=0.06, help="tile fallback threshold")
     ap.add_argument("--c4-scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--no-sequential", action="store_true", help="batch64: skip the one-by-one comparison")
+    ap.add_argument("--c4-group-gib", type=float, default=None,
+                    help="batch64: GiB of frame buffers per job group (default: the library's)")
     args = ap.parse_args()
     if args.counters_child:
         counters_child(args.workload)

@@ -326,6 +326,8 @@ typedef struct {
     int32_t blocks_per_cu;  /* persistent workgroups per compute unit */
     int32_t use_graph;      /* 1 = capture the launch sequence into a hipGraph and replay it */
     double max_gib;         /* refuse batches whose buffers need more device memory (GiB) */
+    double group_gib;       /* jobs run in consecutive groups whose frame buffers fit this size (GiB),
+                               all groups in the same launch, sharing one allocation; 0 = one group */
 } rt_batch_opts;
 
 typedef struct {
@@ -333,7 +335,8 @@ typedef struct {
     double ms_build;        /* host time of create: layout, allocation, upload, capture */
     int32_t n_jobs;
     int32_t graph_nodes;    /* kernels in the captured graph (0 without a graph) */
-    int64_t bytes;          /* device memory of the frame buffers */
+    int32_t groups;         /* job groups run one after another (group_gib) */
+    int64_t bytes;          /* device memory of the frame buffers (the largest group's) */
     uint64_t pixels;
     uint64_t rays_primary, rays_reflect, rays_shadow;
 } rt_batch_stats;

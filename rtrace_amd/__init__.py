@@ -78,9 +78,10 @@ class _BatchJob(ctypes.Structure):
 
 
 class BatchOpts(ctypes.Structure):
-    """rt_batch_opts: raygen tile, chunk (items per take), persistent workgroups per CU, graph, memory cap."""
+    """rt_batch_opts: raygen tile, chunk (items per take), persistent workgroups per CU, graph, memory cap,
+    job-group size (GiB of frame buffers per group; groups share one allocation)."""
     _fields_ = [("tile", ctypes.c_int32), ("chunk", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("max_gib", ctypes.c_double)]
+                ("use_graph", ctypes.c_int32), ("max_gib", ctypes.c_double), ("group_gib", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -88,7 +89,8 @@ class BatchOpts(ctypes.Structure):
 
 class BatchStats(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double), ("ms_build", ctypes.c_double), ("n_jobs", ctypes.c_int32),
-                ("graph_nodes", ctypes.c_int32), ("bytes", ctypes.c_int64), ("pixels", ctypes.c_uint64),
+                ("graph_nodes", ctypes.c_int32), ("groups", ctypes.c_int32), ("bytes", ctypes.c_int64),
+                ("pixels", ctypes.c_uint64),
                 ("rays_primary", ctypes.c_uint64), ("rays_reflect", ctypes.c_uint64), ("rays_shadow", ctypes.c_uint64)]
 
     def as_dict(self):

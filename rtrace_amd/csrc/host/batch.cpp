@@ -18,7 +18,9 @@ struct rt_batch {
     rt_ctx* ctx = nullptr;
     int n_jobs = 0;
     rt_batch_opts bo{};
-    rt::DevBuf arena, views, frames, plan;
+    rt::DevBuf arena, views, frames, plan, tally;
+    std::vector<int> group_start;        // jobs [group_start[g], group_start[g + 1]) form group g
+    std::vector<int> group_depth;        // deepest job of each group
     std::vector<rt::FrameView> hframes;  // [2 * n_jobs]
     std::vector<rt::SceneView> hviews;
     std::vector<rt::FrameLayout> layouts;
@@ -49,6 +51,24 @@ void check_opts(const rt_batch_opts& bo) {
     if (bo.chunk < kBatchBlock || bo.chunk % kBatchBlock != 0 || bo.chunk > (1 << 20))
         throw Error(RT_E_INVALID_ARG, "chunk must be a positive multiple of 64");
     if (bo.blocks_per_cu < 1 || bo.blocks_per_cu > 64) throw Error(RT_E_INVALID_ARG, "blocks_per_cu out of range");
+    if (!(bo.group_gib >= 0.0)) throw Error(RT_E_INVALID_ARG, "group_gib must be >= 0");
+}
+
+// Every group's launch sequence, one after another (the groups share the frame buffers).
+hipError_t launch_groups(rt_batch* b, hipStream_t st) {
+    for (size_t g = 0; g + 1 < b->group_start.size(); ++g) {
+        const int j0 = b->group_start[g], j1 = b->group_start[g + 1];
+        BatchArgs a = b->args;
+        a.views += j0;
+        a.frames += 2 * (size_t)j0;
+        a.tally += 2 * (size_t)j0;
+        a.n_jobs = j1 - j0;
+        BatchLaunch l = b->launch;
+        l.max_depth = b->group_depth[g];
+        const hipError_t e = launch_batch(a, l, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, const rt_batch_opts& bo) {
@@ -62,9 +82,7 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
     b->hviews.resize(n);
     b->hframes.resize(2 * (size_t)n);
     b->layouts.resize(n);
-    std::vector<size_t> offs(n);
-    size_t total = 0;
-    int max_depth = 0;
+    std::vector<size_t> need(n);
     for (int j = 0; j < n; ++j) {
         const rt_batch_job& jb = jobs[j];
         if (!jb.scene || !jb.scene->dev || !jb.d_rgb_out) throw Error(RT_E_INVALID_ARG, "job without scene or output");
@@ -84,14 +102,36 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
         F = FrameView{};
         b->layouts[j] = frame_fill(F, b->hviews[j], &o, bo.tile);
         RayRec *q0 = nullptr, *q1 = nullptr;
-        offs[j] = total;
-        total += frame_carve(F, b->hviews[j], b->layouts[j], nullptr, &q0, &q1, false);
-        max_depth = std::max(max_depth, b->layouts[j].depth);
+        need[j] = frame_carve(F, b->hviews[j], b->layouts[j], nullptr, &q0, &q1, false);
         b->pixels += (uint64_t)o.width * o.height;
     }
-    if ((double)total > bo.max_gib * 1073741824.0)
-        throw Error(RT_E_OOM, "batch needs " + std::to_string(total >> 20) + " MiB of frame buffers, over max_gib");
-    b->arena.reserve(total);
+    // Consecutive jobs form groups whose frame buffers fit group_gib (a job larger than that is
+    // a group of its own). The groups run one after another in the same launch sequence and
+    // share one arena, so the device holds the largest group's buffers, not every job's.
+    // Consecutive jobs are usually views of the same scene, so a group's meshes stay in L2
+    // (groups balanced across scenes measured slower: 1082 against 1211 Mrays/s at 16 GiB).
+    const double budget = bo.group_gib > 0.0 ? bo.group_gib * 1073741824.0 : 1e300;
+    std::vector<int> order(n);  // slot -> job: slots follow the job order
+    std::vector<size_t> offs(n);
+    size_t arena = 0, in_group = 0;
+    b->group_start.assign(1, 0);
+    b->group_depth.assign(1, 0);
+    for (int j = 0; j < n; ++j) {
+        order[j] = j;
+        if (in_group > 0 && (double)(in_group + need[j]) > budget) {
+            b->group_start.push_back(j);
+            b->group_depth.push_back(0);
+            in_group = 0;
+        }
+        offs[j] = in_group;
+        in_group += need[j];
+        arena = std::max(arena, in_group);
+        b->group_depth.back() = std::max(b->group_depth.back(), b->layouts[j].depth);
+    }
+    b->group_start.push_back(n);
+    if ((double)arena > bo.max_gib * 1073741824.0)
+        throw Error(RT_E_OOM, "batch needs " + std::to_string(arena >> 20) + " MiB of frame buffers, over max_gib");
+    b->arena.reserve(arena);
     for (int j = 0; j < n; ++j) {
         FrameView& F = b->hframes[2 * j];
         RayRec *q0 = nullptr, *q1 = nullptr;
@@ -99,30 +139,38 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
         F.rgb = static_cast<uint8_t*>(jobs[j].d_rgb_out);
         F.rays = q0;
         F.rays_next = q1;
-        FrameView& G = b->hframes[2 * j + 1];
-        G = F;
-        G.rays = q1;
-        G.rays_next = q0;
+        FrameView& G2 = b->hframes[2 * j + 1];
+        G2 = F;
+        G2.rays = q1;
+        G2.rays_next = q0;
+    }
+    // the device arrays in slot order
+    std::vector<SceneView> dviews(n);
+    std::vector<FrameView> dframes(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        dviews[k] = b->hviews[order[k]];
+        dframes[2 * (size_t)k] = b->hframes[2 * (size_t)order[k]];
+        dframes[2 * (size_t)k + 1] = b->hframes[2 * (size_t)order[k] + 1];
     }
     hipStream_t st = ctx_stream(ctx);
     b->views.reserve(n * sizeof(SceneView));
     b->frames.reserve(2 * (size_t)n * sizeof(FrameView));
     b->plan.reserve((n + 2) * sizeof(int32_t));
-    HIP_TRY(hipMemcpyAsync(b->views.p, b->hviews.data(), n * sizeof(SceneView), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(b->frames.p, b->hframes.data(), 2 * (size_t)n * sizeof(FrameView), hipMemcpyHostToDevice,
-                           st));
+    b->tally.reserve(2 * (size_t)n * sizeof(unsigned long long));
+    HIP_TRY(hipMemcpyAsync(b->views.p, dviews.data(), n * sizeof(SceneView), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(b->frames.p, dframes.data(), 2 * (size_t)n * sizeof(FrameView), hipMemcpyHostToDevice, st));
     b->args.views = b->views.as<SceneView>();
     b->args.frames = b->frames.as<FrameView>();
     b->args.base = b->plan.as<int32_t>();
     b->args.counter = b->plan.as<int32_t>() + n + 1;
     b->args.n_jobs = n;
     b->args.chunk = bo.chunk;
+    b->args.tally = b->tally.as<unsigned long long>();
     b->launch.grid = compute_units(ctx_device(ctx)) * bo.blocks_per_cu;
-    b->launch.max_depth = max_depth;
     HIP_TRY(hipStreamSynchronize(st));
     if (bo.use_graph) {
         HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t le = launch_batch(b->args, b->launch, st);
+        hipError_t le = launch_groups(b.get(), st);
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(st, &g);
         b->graph = g;
@@ -150,7 +198,7 @@ void run(rt_batch* b, rt_batch_stats* stats) {
     } ev{e0, e1};
     HIP_TRY(hipEventRecord(e0, st));
     if (b->exec) HIP_TRY(hipGraphLaunch(b->exec, st));
-    else HIP_TRY(launch_batch(b->args, b->launch, st));
+    else HIP_TRY(launch_groups(b, st));
     HIP_TRY(hipEventRecord(e1, st));
     HIP_TRY(hipEventSynchronize(e1));
     if (!stats) return;
@@ -163,16 +211,13 @@ void run(rt_batch* b, rt_batch_stats* stats) {
     stats->graph_nodes = (int32_t)b->graph_nodes;
     stats->bytes = (int64_t)b->arena.bytes;
     stats->pixels = b->pixels;
+    stats->groups = (int32_t)(b->group_start.size() - 1);
+    std::vector<unsigned long long> tally(2 * (size_t)b->n_jobs);
+    HIP_TRY(hipMemcpy(tally.data(), b->tally.p, tally.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
     for (int j = 0; j < b->n_jobs; ++j) {
-        const FrameView& F = b->hframes[2 * j];
-        const FrameLayout& L = b->layouts[j];
-        std::vector<int32_t> counts(L.depth + 1), hits(L.depth);
-        HIP_TRY(hipMemcpy(counts.data(), F.counts, (L.depth + 1) * sizeof(int32_t), hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(hits.data(), F.hit_count, L.depth * sizeof(int32_t), hipMemcpyDeviceToHost));
-        stats->rays_primary += (uint64_t)L.n_paths;
-        for (int l = 1; l < L.depth; ++l) stats->rays_reflect += (uint64_t)counts[l];
-        const uint64_t k = b->hviews[j].n_lights > 0 ? (uint64_t)b->hviews[j].shadow_per_hit : 0;
-        for (int l = 0; l < L.depth; ++l) stats->rays_shadow += (uint64_t)hits[l] * k;
+        stats->rays_primary += (uint64_t)b->layouts[j].n_paths;
+        stats->rays_reflect += tally[2 * (size_t)j];
+        stats->rays_shadow += tally[2 * (size_t)j + 1];
     }
 }
 
@@ -204,6 +249,7 @@ void rt_batch_opts_default(rt_batch_opts* bo) {
     bo->blocks_per_cu = 16;
     bo->use_graph = 1;
     bo->max_gib = 200.0;
+    bo->group_gib = 32.0;
 }
 
 int rt_batch_create(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, const rt_batch_opts* bo, rt_batch** out) {

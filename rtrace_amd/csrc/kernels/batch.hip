@@ -87,6 +87,23 @@ __global__ void k_batch_init(BatchArgs a) {
     }
 }
 
+// Rays each job traced (reflection, shadow), read from its queue counts before another
+// group of jobs reuses the frame buffers.
+__global__ void k_batch_tally(BatchArgs a) {
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= a.n_jobs) return;
+    const SceneView& S = a.views[j];
+    const FrameView& F = a.frames[2 * j];
+    unsigned long long refl = 0, shadow = 0;
+    const unsigned long long k = S.n_lights > 0 ? (unsigned long long)S.shadow_per_hit : 0ull;
+    for (int l = 0; l < F.max_depth; ++l) {
+        if (l > 0) refl += (unsigned long long)F.counts[l];
+        shadow += (unsigned long long)F.hit_count[l] * k;
+    }
+    a.tally[2 * j] = refl;
+    a.tally[2 * j + 1] = shadow;
+}
+
 // The scene and frame views reach the stage kernels as separate read-only, non-aliased
 // kernel arguments so that their wave-uniform fields load through the scalar cache.
 #define BATCH_PARAMS                                                                                        \
@@ -204,6 +221,7 @@ hipError_t launch_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t st
     hipLaunchKernelGGL(k_batch_plain<kStPixel>, grid, blk, 0, st, BATCH_ARGS(0));
     plan(kStFinalize, 0);
     hipLaunchKernelGGL(k_batch_plain<kStFinalize>, grid, blk, 0, st, BATCH_ARGS(0));
+    hipLaunchKernelGGL(k_batch_tally, dim3((a.n_jobs + 63) / 64), dim3(64), 0, st, a);
 #undef BATCH_ARGS
     return hipGetLastError();
 }

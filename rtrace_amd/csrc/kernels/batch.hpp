@@ -17,6 +17,7 @@ struct BatchArgs {
     int32_t* counter;         // next chunk to hand out
     int32_t n_jobs;
     int32_t chunk;            // items per chunk (a multiple of the block size)
+    unsigned long long* tally;  // [n_jobs][2]: reflection and shadow rays, written at the end
 };
 
 struct BatchLaunch {
@@ -24,7 +25,8 @@ struct BatchLaunch {
     int max_depth;  // deepest job
 };
 
-// The whole batch's launch sequence on `st` (capturable into a hipGraph: kernels only).
+// One group of jobs' launch sequence on `st` (capturable into a hipGraph: kernels only);
+// its last kernel tallies each job's rays, since the next group reuses the frame buffers.
 hipError_t launch_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t st);
 
 constexpr int kBatchBlock = 64;  // one wavefront per workgroup

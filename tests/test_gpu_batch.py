@@ -74,14 +74,30 @@ def _with_camera(R, scene, cam):
     return _cam_scenes[key]
 
 
-@pytest.mark.parametrize("use_graph,tile", [(1, 8), (0, 8), (1, 4), (1, 16)])
-def test_batch_equals_single_frames(R, workload, use_graph, tile):
+@pytest.mark.parametrize("use_graph,tile,group_gib", [(1, 8, 0.0), (0, 8, 0.0), (1, 4, 0.0), (1, 16, 0.0),
+                                                     (1, 8, 0.004), (0, 8, 1e-9)])
+def test_batch_equals_single_frames(R, workload, use_graph, tile, group_gib):
+    """group_gib > 0 runs the jobs in consecutive groups sharing one set of frame buffers
+    (1e-9: every job its own group); the images and ray counts stay the same."""
     jobs = _jobs(R, workload)
     o = R.default_batch_opts()
-    o.use_graph, o.tile, o.chunk, o.blocks_per_cu = use_graph, tile, 256, 2
+    o.use_graph, o.tile, o.chunk, o.blocks_per_cu, o.group_gib = use_graph, tile, 256, 2, group_gib
     b = R.Batch(jobs, o)
     st = b.run()
     assert st.n_jobs == len(jobs) and (st.graph_nodes > 0) == bool(use_graph)
+    if group_gib == 1e-9:
+        assert st.groups == len(jobs)
+    elif group_gib == 0.0:
+        assert st.groups == 1
+    else:
+        assert st.groups > 1
+    if group_gib > 0.0:  # the groups share buffers smaller than all jobs' together
+        o1 = R.default_batch_opts()
+        o1.use_graph, o1.tile, o1.chunk, o1.blocks_per_cu, o1.group_gib = 0, tile, 256, 2, 0.0
+        one = R.Batch(jobs, o1)
+        assert st.bytes < one.run().bytes
+        one.close()
+        b.run()  # the outputs again from the grouped batch
     rays = [0, 0, 0]
     for scene, cam, r, out in jobs:
         ref, s1 = _single(R, scene, cam, r)

@@ -184,162 +184,166 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
     }
 }
 
-// Persistent and software-pipelined: workgroup (bx, by) computes output channels
-// [32 by, 32 by + 32) of the pixel tiles bx, bx + gridDim.x, ... A step is one (tile, 32-channel
-// K chunk); the global loads of the next step are issued into registers before the current
-// step's MFMAs, so they are in flight while the matrix cores work, then stored into LDS
-// between two barriers. A layer with a single K chunk stages its weights once. Per output
-// element the MFMAs accumulate in (K chunk, tap) order, as a one-tile-per-workgroup kernel
-// would.
-// LDS (37.6 KB per workgroup) allows 4 workgroups, i.e. 4 waves per SIMD: the register
-// budget of that occupancy (128) holds the prefetched step without parking it elsewhere.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_conv3x3(ConvArgs a) {
+// Staging of one step (a pixel tile's halo and, optionally, a weight block for one
+// 32-channel K chunk) for the persistent kernel: loads into registers, later stores into
+// LDS. Halo loads are raw buffer loads when both sources are under 2 GiB: a pixel outside
+// the image gets an offset past the buffer's end and reads as zero, so no value is
+// selected on before it is stored.
+constexpr int kHaloVec = kHH * kHW * 4, kWVec = 9 * kNT * 4;  // 16-byte vectors per K chunk
+constexpr int kHaloPer = (kHaloVec + 255) / 256, kWPer = (kWVec + 255) / 256;
+
+struct StepGeom {
+    int tiles_x, tiles_y, h2, w2, cin, co0;
+    unsigned long long bytes0, bytes1;
+    bool use_buf;
+};
+
+__device__ __forceinline__ void tile_origin(const StepGeom& g, int t, int& b, int& y0, int& x0) {
+    b = t / (g.tiles_x * g.tiles_y);
+    t -= b * g.tiles_x * g.tiles_y;
+    y0 = (t / g.tiles_x) * kTH;
+    x0 = (t % g.tiles_x) * kTW;
+}
+
+__device__ __forceinline__ void load_halo(const ConvArgs& a, const StepGeom& g, int t, int ci0, uint4 (&hv)[kHaloPer],
+                                          uint32_t& hok) {
+    int b, y0, x0;
+    tile_origin(g, t, b, y0, x0);
+    hok = 0;
+    if (g.use_buf) {
+        // a K chunk lies within one source (both channel counts are multiples of 32)
+        const bool first = ci0 < a.in.c0;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(first ? a.in.src0 : a.in.src1), (short)0, (int)(first ? g.bytes0 : g.bytes1), 0x00020000);
+#pragma unroll
+        for (int r = 0; r < kHaloPer; ++r) {
+            const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
+            const int px = v >> 2, q = v & 3;
+            const int hy = px / kHW, hx = px - hy * kHW;
+            const int y = y0 + hy - 1, x = x0 + hx - 1, c = ci0 + q * 8;
+            const bool ok = y >= 0 && x >= 0 && y < a.H && x < a.W;
+            const uint32_t e = first ? (uint32_t)(a.in.up0 ? ((b * g.h2 + (y >> 1)) * g.w2 + (x >> 1))
+                                                          : ((b * a.H + y) * a.W + x)) * (uint32_t)a.in.c0 + (uint32_t)c
+                                     : (uint32_t)((b * a.H + y) * a.W + x) * (uint32_t)a.in.c1 + (uint32_t)(c - a.in.c0);
+            const auto w4 = __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? e * 2u : 0x80000000u, 0, 0);
+            hv[r] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
+        }
+        hok = (1u << kHaloPer) - 1u;
+    } else {
+#pragma unroll
+        for (int r = 0; r < kHaloPer; ++r) {
+            const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
+            const int px = v >> 2, q = v & 3;
+            const int hy = px / kHW, hx = px - hy * kHW;
+            bool ok;
+            hv[r] = gather8_raw(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8, &ok);
+            hok |= ok ? 1u << r : 0u;
+        }
+    }
+}
+
+__device__ __forceinline__ void load_w(const ConvArgs& a, const StepGeom& g, int ci0, uint4 (&wv)[kWPer]) {
+#pragma unroll
+    for (int r = 0; r < kWPer; ++r) {
+        const int v = min((int)threadIdx.x + r * 256, kWVec - 1);
+        const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
+        wv[r] = *reinterpret_cast<const uint4*>(a.wt + ((size_t)tap * a.coutp + g.co0 + n) * g.cin + ci0 + q * 8);
+    }
+}
+
+__device__ __forceinline__ void store_halo(uint16_t* s_in, const uint4 (&hv)[kHaloPer], uint32_t hok) {
+#pragma unroll
+    for (int r = 0; r < kHaloPer; ++r) {
+        const int v = (int)threadIdx.x + r * 256;
+        const int slot = v < kHaloVec ? (v >> 2) * kPS + (v & 3) * 8 : kHH * kHW * kPS;
+        *reinterpret_cast<uint4*>(s_in + slot) = ((hok >> r) & 1u) ? hv[r] : make_uint4(0, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void store_w(uint16_t* s_w, const uint4 (&wv)[kWPer]) {
+#pragma unroll
+    for (int r = 0; r < kWPer; ++r) {
+        const int v = (int)threadIdx.x + r * 256;
+        const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
+        const int slot = v < kWVec ? (tap * kNT + n) * kPS + q * 8 : 9 * kNT * kPS;
+        *reinterpret_cast<uint4*>(s_w + slot) = wv[r];
+    }
+}
+
+// Layers with a single 32-channel K chunk (32 input channels), persistent and
+// software-pipelined: workgroup (bx, by) stages the weights of output channels
+// [32 by, 32 by + 32) once, then computes the pixel tiles bx, bx + gridDim.x, ... The halo
+// loads of the next tile are issued into registers before the current tile's MFMAs, so they
+// are in flight while the matrix cores work, then stored into LDS between two barriers. The
+// MFMAs accumulate in tap order, as the one-tile kernel does.
+__global__ __launch_bounds__(256) void k_conv3x3(ConvArgs a) {
     // one spare pixel slot each: the staging threads with no vector to store write there, so
     // the staging has no branches and all of its loads are in flight together
     __shared__ __attribute__((aligned(16))) uint16_t s_in[(kHH * kHW + 1) * kPS];
     __shared__ __attribute__((aligned(16))) uint16_t s_w[(9 * kNT + 1) * kPS];
-    const int tiles_x = (a.W + kTW - 1) / kTW, tiles_y = (a.H + kTH - 1) / kTH;
-    const int n_tiles = a.B * tiles_x * tiles_y;
-    const int co0 = blockIdx.y * kNT;
-    const int cin = a.in.c0 + a.in.c1;
-    const int n_chunks = cin / kCK;
-    const bool w_once = n_chunks == 1;
+    StepGeom g;
+    g.tiles_x = (a.W + kTW - 1) / kTW;
+    g.tiles_y = (a.H + kTH - 1) / kTH;
+    g.h2 = a.H >> 1;
+    g.w2 = a.W >> 1;
+    g.cin = a.in.c0 + a.in.c1;
+    g.co0 = blockIdx.y * kNT;
+    g.bytes0 = (unsigned long long)a.B * (a.in.up0 ? (unsigned long long)g.h2 * g.w2 : (unsigned long long)a.H * a.W) *
+               a.in.c0 * 2;
+    g.bytes1 = (unsigned long long)a.B * a.H * a.W * a.in.c1 * 2;
+    g.use_buf = g.bytes0 < (1ull << 31) && g.bytes1 < (1ull << 31);
+    const int n_tiles = a.B * g.tiles_x * g.tiles_y;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int g = lane >> 4, r16 = lane & 15;
+    const int gq = lane >> 4, r16 = lane & 15;
 
-    constexpr int kHaloVec = kHH * kHW * 4, kWVec = 9 * kNT * 4;  // 16-byte vectors per K chunk
-    constexpr int kHaloPer = (kHaloVec + 255) / 256, kWPer = (kWVec + 255) / 256;
-    uint4 hv[kHaloPer], wv[kWPer];
-    uint32_t hok = 0;  // bit r: hv[r] is inside the image (else it stores as zero)
-    // Halo loads as raw buffer loads when both sources are under 2 GiB: a pixel outside the
-    // image gets an offset past the buffer's end and the hardware returns zeros, so nothing
-    // selects on a loaded value before it is stored.
-    const int h2 = a.H >> 1, w2 = a.W >> 1;
-    const unsigned long long bytes0 =
-        (unsigned long long)a.B * (a.in.up0 ? (unsigned long long)h2 * w2 : (unsigned long long)a.H * a.W) * a.in.c0 * 2;
-    const unsigned long long bytes1 = (unsigned long long)a.B * a.H * a.W * a.in.c1 * 2;
-    const bool use_buf = bytes0 < (1ull << 31) && bytes1 < (1ull << 31);
-    auto tile_origin = [&](int t, int& b, int& y0, int& x0) {
-        b = t / (tiles_x * tiles_y);
-        t -= b * tiles_x * tiles_y;
-        y0 = (t / tiles_x) * kTH;
-        x0 = (t % tiles_x) * kTW;
-    };
-    // every load of a step first (indices clamped, no branches); the stores come later
-    auto load_step = [&](int t, int ci0, bool with_w) {
-        int b, y0, x0;
-        tile_origin(t, b, y0, x0);
-        hok = 0;
-        if (use_buf) {
-            // a K chunk lies within one source (both channel counts are multiples of 32)
-            const bool first = ci0 < a.in.c0;
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)(first ? a.in.src0 : a.in.src1), (short)0, (int)(first ? bytes0 : bytes1), 0x00020000);
-#pragma unroll
-            for (int r = 0; r < kHaloPer; ++r) {
-                const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
-                const int px = v >> 2, q = v & 3;
-                const int hy = px / kHW, hx = px - hy * kHW;
-                const int y = y0 + hy - 1, x = x0 + hx - 1, c = ci0 + q * 8;
-                const bool ok = y >= 0 && x >= 0 && y < a.H && x < a.W;
-                const uint32_t e = first ? (uint32_t)(a.in.up0 ? ((b * h2 + (y >> 1)) * w2 + (x >> 1))
-                                                              : ((b * a.H + y) * a.W + x)) * (uint32_t)a.in.c0 + (uint32_t)c
-                                         : (uint32_t)((b * a.H + y) * a.W + x) * (uint32_t)a.in.c1 +
-                                               (uint32_t)(c - a.in.c0);
-                const auto w4 = __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? e * 2u : 0x80000000u, 0, 0);
-                hv[r] = make_uint4(w4[0], w4[1], w4[2], w4[3]);
-            }
-            hok = (1u << kHaloPer) - 1u;
-        } else {
-#pragma unroll
-            for (int r = 0; r < kHaloPer; ++r) {
-                const int v = min((int)threadIdx.x + r * 256, kHaloVec - 1);
-                const int px = v >> 2, q = v & 3;
-                const int hy = px / kHW, hx = px - hy * kHW;
-                bool ok;
-                hv[r] = gather8_raw(a.in, b, y0 + hy - 1, x0 + hx - 1, a.H, a.W, ci0 + q * 8, &ok);
-                hok |= ok ? 1u << r : 0u;
-            }
-        }
-        if (with_w) {
-#pragma unroll
-            for (int r = 0; r < kWPer; ++r) {
-                const int v = min((int)threadIdx.x + r * 256, kWVec - 1);
-                const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
-                wv[r] = *reinterpret_cast<const uint4*>(a.wt + ((size_t)tap * a.coutp + co0 + n) * cin + ci0 + q * 8);
-            }
-        }
-    };
-    auto store_halo = [&]() {
-#pragma unroll
-        for (int r = 0; r < kHaloPer; ++r) {
-            const int v = (int)threadIdx.x + r * 256;
-            const int slot = v < kHaloVec ? (v >> 2) * kPS + (v & 3) * 8 : kHH * kHW * kPS;
-            *reinterpret_cast<uint4*>(s_in + slot) = ((hok >> r) & 1u) ? hv[r] : make_uint4(0, 0, 0, 0);
-        }
-    };
-    auto store_w = [&]() {
-#pragma unroll
-        for (int r = 0; r < kWPer; ++r) {
-            const int v = (int)threadIdx.x + r * 256;
-            const int q = v & 3, n = (v >> 2) % kNT, tap = (v >> 2) / kNT;
-            const int slot = v < kWVec ? (tap * kNT + n) * kPS + q * 8 : 9 * kNT * kPS;
-            *reinterpret_cast<uint4*>(s_w + slot) = wv[r];
-        }
-    };
-
-    int tile = blockIdx.x, chunk = 0;
+    uint4 hv[kHaloPer];
+    uint32_t hok = 0;
+    int tile = blockIdx.x;
     if (tile >= n_tiles) return;
-    load_step(tile, 0, true);
-    if (w_once) store_w();  // ordered before the first MFMA by the loop's second barrier
-    f32x4 acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    load_halo(a, g, tile, 0, hv, hok);
+    {
+        uint4 wv[kWPer];
+        load_w(a, g, 0, wv);
+        store_w(s_w, wv);  // ordered before the first MFMA by the loop's second barrier
+    }
+    f32x4 acc00 = f32x4{0.f, 0.f, 0.f, 0.f}, acc01 = acc00, acc10 = acc00, acc11 = acc00;
     for (;;) {
-        __syncthreads();  // the previous step's MFMAs are done with the LDS
-        store_halo();
-        if (!w_once) store_w();
+        __syncthreads();  // the previous tile's MFMAs are done with the LDS
+        store_halo(s_in, hv, hok);
         __syncthreads();
-        // the next step's loads go out now and land while this step's MFMAs run
-        int ntile = tile, nchunk = chunk + 1;
-        if (nchunk == n_chunks) {
-            nchunk = 0;
-            ntile += gridDim.x;
-        }
+        // the next tile's loads go out now and land while this tile's MFMAs run
+        const int ntile = tile + (int)gridDim.x;
         const bool more = ntile < n_tiles;
-        if (more) load_step(ntile, nchunk * kCK, !w_once);
+        if (more) load_halo(a, g, ntile, 0, hv, hok);
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap) {
             const int dy = tap / 3, dx = tap - dy * 3;
-            bf16x8 bfr[2];
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
-                bfr[nt] = *reinterpret_cast<const bf16x8*>(s_w + (tap * kNT + nt * 16 + r16) * kPS + g * 8);
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const int row = wave * 2 + mt;
-                bf16x8 afr = *reinterpret_cast<const bf16x8*>(s_in + ((row + dy) * kHW + r16 + dx) * kPS + g * 8);
-#pragma unroll
-                for (int nt = 0; nt < 2; ++nt)
-                    acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nt], acc[mt][nt], 0, 0, 0);
-            }
+            const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(s_w + (tap * kNT + r16) * kPS + gq * 8);
+            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(s_w + (tap * kNT + 16 + r16) * kPS + gq * 8);
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(s_in + ((wave * 2 + dy) * kHW + r16 + dx) * kPS + gq * 8);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(s_in + ((wave * 2 + 1 + dy) * kHW + r16 + dx) * kPS + gq * 8);
+            acc00 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc00, 0, 0, 0);
+            acc01 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc01, 0, 0, 0);
+            acc10 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc10, 0, 0, 0);
+            acc11 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc11, 0, 0, 0);
         }
-        if (chunk == n_chunks - 1) {
-            // D layout: lane holds rows (pixels) 4g..4g+3 of column (channel) r16
+        {
+            // D layout: lane holds rows (pixels) 4gq..4gq+3 of column (channel) r16
             int b, y0, x0;
-            tile_origin(tile, b, y0, x0);
+            tile_origin(g, tile, b, y0, x0);
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
                 const int y = y0 + wave * 2 + mt;
 #pragma unroll
                 for (int nt = 0; nt < 2; ++nt) {
-                    const int co = co0 + nt * 16 + r16;
+                    const f32x4 acc = mt == 0 ? (nt == 0 ? acc00 : acc01) : (nt == 0 ? acc10 : acc11);
+                    const int co = g.co0 + nt * 16 + r16;
                     const float bias = a.bias && co < a.cout ? a.bias[co] : 0.0f;
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int x = x0 + 4 * g + i;
+                        const int x = x0 + 4 * gq + i;
                         if (y < a.H && co < a.cout && x < a.W) {
-                            float v = acc[mt][nt][i] + bias;
+                            float v = acc[i] + bias;
                             const size_t o = (((size_t)b * a.H + y) * a.W + x) * a.out_pitch + co;
                             if (a.out_f32) {
                                 a.out_f32[o] = v;
@@ -349,13 +353,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                             }
                         }
                     }
-                    acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
                 }
             }
+            acc00 = acc01 = acc10 = acc11 = f32x4{0.f, 0.f, 0.f, 0.f};
         }
         if (!more) break;
         tile = ntile;
-        chunk = nchunk;
     }
 }
 

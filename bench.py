@@ -33,6 +33,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 N_SIMD = 256 * 4  # MI355X: 256 CUs x 4 SIMDs
+N_XCC = 8  # GRBM_GUI_ACTIVE is counted per XCC; rocprofv3 sums the 8 instances
 # algorithmic bytes per unit of traversal work (layout in rtrace_amd/csrc/common/layout.hpp)
 B_NODE, B_TRI, B_RETEST = 40, 48, 72  # node: 32 B box+links, +8 B f64 split (inner)
 B_BVH_NODE = 128  # 4-wide mesh BVH node: four slot boxes and links
@@ -296,33 +297,39 @@ def counter_summary(pmc, kernel, ms_per_launch):
     if all(g.values()):
         m = {c: sum(v) / len(v) for c, v in g.items()}
         out["valu_lanes_active"] = round(m["SQ_THREAD_CYCLES_VALU"] / max(m["SQ_ACTIVE_INST_VALU"] * 64.0, 1.0), 4)
-        out["valu_busy"] = round(m["SQ_ACTIVE_INST_VALU"] * 4.0 / N_SIMD / max(m["GRBM_GUI_ACTIVE"], 1.0), 4)
+        gui = max(m["GRBM_GUI_ACTIVE"] / N_XCC, 1.0)  # GPU cycles of the dispatch
+        out["valu_busy"] = round(m["SQ_ACTIVE_INST_VALU"] * 4.0 / N_SIMD / gui, 4)
         out["wave_wait_frac"] = round(m["SQ_WAIT_ANY"] / max(m["SQ_WAVE_CYCLES"], 1.0), 4)
     # the limiter, from the measurements
     if out.get("hbm_frac", 0) >= 0.6:
         out["limiter"] = "hbm bandwidth"
     elif out.get("valu_busy", 0) >= 0.6:
-        out["limiter"] = "valu issue"
+        out["limiter"] = (f"valu issue: the SIMDs issue VALU {out['valu_busy'] * 100:.0f}% of the cycles with "
+                          f"{out.get('valu_lanes_active', 0) * 64:.0f} of 64 lanes active (divergent traversal); "
+                          f"HBM at {out.get('hbm_frac', 0) * 100:.1f}% of peak")
     elif out.get("wave_wait_frac", 0) >= 0.5:
         out["limiter"] = "memory latency (waves mostly waiting on dependent loads; HBM and VALU far from peak)"
     elif "valu_busy" in out:
         out["limiter"] = "mixed (no unit near its peak)"
+    out["counters"] = ("valu_busy = 4 SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8 XCCs); valu_lanes_active = "
+                       "SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU); wave_wait_frac = SQ_WAIT_ANY / SQ_WAVE_CYCLES")
     return out
 
 
-def mfma_summary(pmc, kernel="k_conv3x3"):
-    """MFMA busy share of the denoiser's convolutions (SQ_VALU_MFMA_BUSY_CYCLES per SIMD over the
-    active cycles), and the bf16 MFMA work they did."""
+def mfma_summary(pmc):
+    """MFMA busy share of the denoiser's convolutions (both conv kernels: SQ_VALU_MFMA_BUSY_CYCLES
+    per SIMD over the dispatches' GPU cycles), and the bf16 MFMA work they did."""
     sq = pmc["sq"]
-    busy = _per_dispatch(sq, "SQ_VALU_MFMA_BUSY_CYCLES", _kern(kernel))
-    gui = _per_dispatch(sq, "GRBM_GUI_ACTIVE", _kern(kernel))
-    mops = _per_dispatch(sq, "SQ_INSTS_VALU_MFMA_MOPS_BF16", _kern(kernel))
+    match = lambda n: re.search(r"::k_conv3x3(_tile)?[<(]", n) is not None  # noqa: E731
+    busy = _per_dispatch(sq, "SQ_VALU_MFMA_BUSY_CYCLES", match)
+    gui = _per_dispatch(sq, "GRBM_GUI_ACTIVE", match)
+    mops = _per_dispatch(sq, "SQ_INSTS_VALU_MFMA_MOPS_BF16", match)
     if not busy or not gui:
         return None
-    return {"kernel": f"dn::{kernel}", "launches": len(busy),
-            "mfma_busy": round(sum(busy) / max(sum(gui) * N_SIMD, 1.0), 4),
-            "bf16_mops_per_launch": round(sum(mops) / len(mops), 1) if mops else None,
-            "counter": "SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE x 1024 SIMDs), summed over the launches"}
+    return {"kernel": "dn::k_conv3x3 + dn::k_conv3x3_tile (15 layers)", "launches": len(busy),
+            "mfma_busy": round(sum(busy) / max(sum(gui) / N_XCC * N_SIMD, 1.0), 4),
+            "bf16_mops_per_forward": round(sum(mops), 1) if mops else None,
+            "counter": "SQ_VALU_MFMA_BUSY_CYCLES / (1024 SIMDs x GRBM_GUI_ACTIVE / 8 XCCs), summed over the launches"}
 
 
 def measure_frame(R, torch, workload, steps, warmup, stage_events=True, count=True):

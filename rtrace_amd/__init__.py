@@ -141,6 +141,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RtraceError(-5, f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        # PyTorch-ROCm ships its own HIP runtime under the soname the library links against:
+        # when it is installed, load it first so both share that runtime (loaded the other way
+        # round, torch later finds no GPU).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.c_void_p
         L.rt_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(P)]

@@ -650,11 +650,40 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t fir
 // "while-while" traversal: each lane walks inner nodes until it holds its next leaf,
 // then the wavefront tests leaves together, so lanes at inner nodes do not wait through
 // other lanes' triangle loops. Leaves are still tested in this lane's visit order.
+// The mesh bounds test of MeshObject::hit (src/ray.rs:586-610) in f64 on the exact box,
+// with the interval [tmin, tmax] widened by a relative 2^-30: false only when no point of
+// the box lies on the ray within [tmin, tmax], so no triangle can be hit there.
+RT_HD bool mesh_bounds_may_hit(const MeshDev& M, const RayD& r, double tmin, double tmax) {
+    double lo = tmin, hi = tmax;
+    const double o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+    for (int a = 0; a < 3; ++a) {
+        const double inv = 1.0 / d[a];
+        double t0 = (M.bmin64[a] - o[a]) * inv, t1 = (M.bmax64[a] - o[a]) * inv;
+        if (inv < 0.0) {
+            const double x = t0;
+            t0 = t1;
+            t1 = x;
+        }
+        lo = fmax_r(lo, t0);  // a NaN bound (0 * inf) leaves the interval as it is
+        hi = fmin_r(hi, t1);
+    }
+    if (lo <= hi) return true;
+    const double slack = 0x1p-30 * (fabs(lo) + fabs(hi)) + 1e-300;
+    return isfinite(slack) && lo <= hi + slack;  // an infinite bound: the ray never meets a slab
+}
+
+// Origins farther than this many mesh radii (L1) from the centre take the f64 bounds test
+// first: the FP32 walk's slack grows with |origin| and would admit most of the mesh's nodes
+// (a ray leaving a far point of a plane toward a light above the scene).
+constexpr float kFarOrigin = 256.0f;
+
 template <bool kAny, bool count>
 RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
                       StackRef stk, Work& w) {
     MeshWalk s;
     walk_init(s, M, r, use_kd, tmin, best);
+    if (fabsf(s.q.ox) + fabsf(s.q.oy) + fabsf(s.q.oz) > kFarOrigin * M.radius && !mesh_bounds_may_hit(M, r, tmin, best))
+        return false;
     bool found = false;
     bool tie = false;  // another triangle of this mesh hit at exactly `best`
     for (;;) {

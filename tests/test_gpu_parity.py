@@ -448,6 +448,41 @@ def test_axis_parallel_rays_take_the_kd_walk(R, config_dir, view):
         assert getattr(st, k) == ost[k], k
 
 
+def test_far_origin_shadow_rays(R, tmp_path):
+    """A wall at x = 1e5 faces a light on the far side of a mesh at the origin; the camera,
+    6e4 from the mesh, looks at the wall, so every ray starts far from the mesh and the
+    shadow rays pass through it for some pixels and beside it for others. Origins this far
+    from the mesh take the f64 bounds test before the FP32 walk; pixels, hit ids and ray
+    counts match the oracle and the mesh's shadow covers part of the view."""
+    import shutil
+    shutil.copy(os.path.join(REF_SCENES, "icosahedron.stl"), str(tmp_path))
+    mat = {"color": "#C0C0C0", "ambient": 0.1, "diffuse": 0.8, "specular": 0.1, "shininess": 8}
+    objs = [{"kind": "plane", "point": [1e5, 0, 0], "normal": [-1, 0, 0], "material": mat},
+            {"kind": "mesh", "filename": "icosahedron.stl", "material": dict(mat, color="#FFAA00"),
+             "transform": ["scale(2, 2, 2)", "translate(0, 0, 1.5)"]}]
+    sc = {"camera": {"kind": "perspective", "position": [6e4, 0, 1.5], "target": [1e5, 0, 1.5], "up": [0, 0, 1],
+                     "width": 4, "height": 3, "fov": 90},
+          "objects": objs,
+          "lights": [{"position": [-6, 0, 1.5], "color": "#FFFFFF", "intensity": 0.9}],
+          "scene_settings": {"ambient_illumination": {"color": "#FFFFFF", "intensity": 0.1},
+                             "background_color": "#101010"}}
+    paths = []
+    for name, o in (("far.json", objs), ("far_nomesh.json", objs[:1])):
+        p = tmp_path / name
+        p.write_text(json.dumps(dict(sc, objects=o)))
+        paths.append(str(p))
+    opts = O.make_opts(120, 90, max_depth=2)
+    img, st, gh = gpu_render(R, paths[0], opts)
+    ref, ost, rh = O.render(paths[0], opts, want_hits=True)
+    _, mism = compare("far origins", img, ref, gh, rh)
+    assert mism == 0
+    for k in ("rays_primary", "rays_reflect", "rays_shadow"):
+        assert getattr(st, k) == ost[k], k
+    lit, _, _ = O.render(paths[1], opts)
+    shadowed = float(np.mean(np.any(lit != ref, axis=-1)))
+    assert 0.2 < shadowed < 0.8, shadowed
+
+
 def test_frame_graph_replays_match_eager_renders(R, config_dir):
     """Without per-stage timing a frame rendered twice in a row with the same kernel
     arguments is captured into a hipGraph and replayed while they stay unchanged. Replays,

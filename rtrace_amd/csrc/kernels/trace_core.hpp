@@ -677,13 +677,17 @@ RT_HD bool mesh_bounds_may_hit(const MeshDev& M, const RayD& r, double tmin, dou
 // (a ray leaving a far point of a plane toward a light above the scene).
 constexpr float kFarOrigin = 256.0f;
 
+// false when the walk set up in s can be skipped: a far origin whose ray misses the box
+RT_HD bool walk_needed(const MeshWalk& s, const MeshDev& M, const RayD& r, double tmin, double best) {
+    return fabsf(s.q.ox) + fabsf(s.q.oy) + fabsf(s.q.oz) <= kFarOrigin * M.radius || mesh_bounds_may_hit(M, r, tmin, best);
+}
+
 template <bool kAny, bool count>
 RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
                       StackRef stk, Work& w) {
     MeshWalk s;
     walk_init(s, M, r, use_kd, tmin, best);
-    if (fabsf(s.q.ox) + fabsf(s.q.oy) + fabsf(s.q.oz) > kFarOrigin * M.radius && !mesh_bounds_may_hit(M, r, tmin, best))
-        return false;
+    if (!walk_needed(s, M, r, tmin, best)) return false;
     bool found = false;
     bool tie = false;  // another triangle of this mesh hit at exactly `best`
     for (;;) {
@@ -884,10 +888,15 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
 // The scene-order fold finds a hit exactly when some object hits in (tmin, tmax): the
 // first such object sees the full range. So the order is free, and the analytic objects
 // are tested before the meshes, the costly tests.
-template <bool count, bool kMeshes = true>
-RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w) {
+//
+// The analytic part: true when a sphere, plane or cube hits in (tmin, tmax); `meshes`
+// tells whether the scene also has meshes to walk.
+template <bool count>
+RT_HD bool world_any_analytic(const SceneView& S, const RayD& r, double tmin, double tmax, StackRef stk, Work& w,
+                              bool& meshes) {
     RayF f = make_rayf(r);
     const MeshRay wq = make_world_ray(r);
+    meshes = false;
     if (S.obvh) {
         // any hit in [tmin, tmax] over the spheres, then the other objects
         bool hit = false;
@@ -900,7 +909,6 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
             return hit;
         });
         if (hit) return true;
-        bool meshes = false;
         for (int k = 0; k < S.n_others; ++k) {
             const ObjDev& ob = S.objs[S.others[k]];
             double t;
@@ -916,17 +924,8 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
                 meshes = true;
             }
         }
-        if (kMeshes && meshes)
-            for (int k = 0; k < S.n_others; ++k) {
-                const ObjDev& ob = S.objs[S.others[k]];
-                if (ob.kind != kMesh) continue;
-                int32_t tri = -1;
-                double best = tmax;
-                if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
-            }
         return false;
     }
-    bool meshes = false;
     for (int i = 0; i < S.n_obj; ++i) {
         const ObjDev& ob = S.objs[i];
         double t;
@@ -945,14 +944,29 @@ RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax
             meshes = true;
         }
     }
-    if (kMeshes && meshes)
-        for (int i = 0; i < S.n_obj; ++i) {
-            const ObjDev& ob = S.objs[i];
-            if (ob.kind != kMesh) continue;
-            int32_t tri = -1;
-            double best = tmax;
-            if (mesh_trace<true, count>(S.meshes[ob.mesh], r, use_kd, tmin, best, tri, stk, w)) return true;
-        }
+    return false;
+}
+
+// The meshes of the scene in object order: position k of the list the shadow query walks
+// (the non-sphere objects with an object BVH, else all objects); -1 after the last.
+RT_HD int next_mesh(const SceneView& S, int k) {
+    const int n = S.obvh ? S.n_others : S.n_obj;
+    for (++k; k < n; ++k)
+        if (S.objs[S.obvh ? S.others[k] : k].kind == kMesh) return k;
+    return -1;
+}
+RT_HD const MeshDev& mesh_at(const SceneView& S, int k) { return S.meshes[S.objs[S.obvh ? S.others[k] : k].mesh]; }
+
+template <bool count>
+RT_HD bool world_any(const SceneView& S, const RayD& r, double tmin, double tmax, bool use_kd, StackRef stk, Work& w) {
+    bool meshes = false;
+    if (world_any_analytic<count>(S, r, tmin, tmax, stk, w, meshes)) return true;
+    if (!meshes) return false;
+    for (int k = next_mesh(S, -1); k >= 0; k = next_mesh(S, k)) {
+        int32_t tri = -1;
+        double best = tmax;
+        if (mesh_trace<true, count>(mesh_at(S, k), r, use_kd, tmin, best, tri, stk, w)) return true;
+    }
     return false;
 }
 

@@ -341,9 +341,10 @@ RT_HD int32_t mesh_resolve_tie(const MeshDev& M, const RayD& r, double tmin, dou
 // reference's.
 struct MeshWalk {
     MeshRay q;
-    float R;        // max |v - centre|_1 over the mesh
-    float tmin_lo;  // lower bound of tmin in f32
-    float best_hi;  // upper bound of best in f32
+    float R;        // max |v - centre|_1 over the mesh (plus the re-based origin's error)
+    float tshift;   // 0, or the distance along the ray of the re-based origin (walk_init)
+    float tmin_lo;  // lower bound of tmin in f32, in the walk's frame (distances from q's origin)
+    float best_hi;  // upper bound of best in f32, in the walk's frame
     float c0, c1;   // interval of the current node
     uint32_t node;
     int sp;
@@ -353,20 +354,95 @@ struct MeshWalk {
 };
 constexpr uint32_t kNoLeaf = 0xffffffffu;
 
+// The mesh bounds test of MeshObject::hit (src/ray.rs:586-610) in f64 on the exact box,
+// with the interval [tmin, tmax] widened by a relative 2^-30: false only when no point of
+// the box lies on the ray within [tmin, tmax], so no triangle can be hit there. `enter`:
+// where the ray's part within [tmin, tmax] enters the box (at least tmin).
+RT_HD bool mesh_bounds_may_hit(const MeshDev& M, const RayD& r, double tmin, double tmax, double& enter) {
+    double lo = tmin, hi = tmax;
+    const double o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
+    for (int a = 0; a < 3; ++a) {
+        const double inv = 1.0 / d[a];
+        double t0 = (M.bmin64[a] - o[a]) * inv, t1 = (M.bmax64[a] - o[a]) * inv;
+        if (inv < 0.0) {
+            const double x = t0;
+            t0 = t1;
+            t1 = x;
+        }
+        lo = fmax_r(lo, t0);  // a NaN bound (0 * inf) leaves the interval as it is
+        hi = fmin_r(hi, t1);
+    }
+    enter = lo;
+    if (lo <= hi) return true;
+    const double slack = 0x1p-30 * (fabs(lo) + fabs(hi)) + 1e-300;
+    return isfinite(slack) && lo <= hi + slack;  // an infinite bound: the ray never meets a slab
+}
+
+// Origins farther than this many mesh radii (L1) from the centre take the f64 bounds test
+// first, and a BVH walk that goes on starts from where the ray enters the box: the FP32
+// walk's slack grows with |origin| and would admit most of the mesh's nodes (a ray leaving
+// a far point of a plane toward a light above the scene).
+constexpr float kFarOrigin = 256.0f;
+
+// The FP32 ray re-based to the point ts along r (computed in f64): distances in the walk
+// are then t - ts. That point lies off the ray by a few ulps of |o| + ts, which the box
+// slack and the triangle tests' error scale (R) absorb.
+RT_HD MeshRay make_mesh_ray_at(const MeshDev& M, const RayD& r, double ts, float& R) {
+    RayD r2;
+    r2.o = add(r.o, scale(ts, r.d));
+    r2.d = r.d;
+    MeshRay q = make_mesh_ray(M, r2);
+    const float e = (float)(0x1p-48 * (fabs(r.o.x) + fabs(r.o.y) + fabs(r.o.z) + ts));
+    q.sx += e * fabsf(q.ix);
+    q.sy += e * fabsf(q.iy);
+    q.sz += e * fabsf(q.iz);
+    R = M.radius + e / kTolC;
+    return q;
+}
+// Where a walk's node intervals start: below tmin_lo (tmin > 0: half of it), or at tmin_lo
+// in a re-based walk's frame, where it is negative.
+RT_HOT float walk_start(const MeshWalk& s) { return s.tshift == 0.0f ? s.tmin_lo * 0.5f : s.tmin_lo; }
+// Bounds of f64 distance t in a re-based walk's frame (t - tshift): above and below it.
+RT_HD float shift_up(float tshift, double t) {
+    const float x = (float)(t - (double)tshift);
+    return x + kTolC * fabsf(x) + kTolC;
+}
+RT_HD float shift_down(float tshift, double t) {
+    const float x = (float)(t - (double)tshift);
+    return x - kTolC * fabsf(x) - kTolC;
+}
+
+// kRebase: a BVH walk from a far origin starts where the ray enters the mesh's box (shadow
+// walks; closest-hit walks keep the origin, which spares their registers).
+template <bool kRebase>
 RT_HOT void walk_init(MeshWalk& s, const MeshDev& M, const RayD& r, bool use_kd, double tmin, double best) {
     s.q = make_mesh_ray(M, r);
     s.R = M.radius;
     s.par0 = fabs(r.d.x) < 1e-9;
     s.par1 = fabs(r.d.y) < 1e-9;
     s.par2 = fabs(r.d.z) < 1e-9;
-    s.tmin_lo = (float)tmin * (1.0f - kTolC);
-    s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
-    // the mesh AABB check (src/ray.rs:586-610) is the root's box test
-    s.c0 = s.tmin_lo * 0.5f;
-    s.c1 = s.best_hi;
     s.node = use_kd ? M.root_kd : M.root_bf;
     s.have_node = true;
     s.sp = 0;
+    s.tshift = 0.0f;
+    if (fabsf(s.q.ox) + fabsf(s.q.oy) + fabsf(s.q.oz) > kFarOrigin * M.radius) {
+        double enter;
+        if (!mesh_bounds_may_hit(M, r, tmin, best, enter))
+            s.have_node = false;
+        else if (kRebase && use_kd && M.bvh != nullptr && !s.par0 && !s.par1 && !s.par2 && enter > tmin)
+            s.tshift = (float)enter;  // the BVH walk (below) starts at the box
+        if (s.tshift != 0.0f) s.q = make_mesh_ray_at(M, r, (double)s.tshift, s.R);
+    }
+    if (s.tshift == 0.0f) {
+        s.tmin_lo = (float)tmin * (1.0f - kTolC);
+        s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
+    } else {
+        s.tmin_lo = shift_down(s.tshift, tmin);
+        s.best_hi = shift_up(s.tshift, best);
+    }
+    // the mesh AABB check (src/ray.rs:586-610) is the root's box test
+    s.c0 = walk_start(s);
+    s.c1 = s.best_hi;
     // The BVH gives the reference's candidates for a ray with no axis-parallel component:
     // the reference's walk then reaches every leaf cell the ray's forward part crosses,
     // and a triangle hit at q lies in the leaf of q's cell. With a component below 1e-9 the
@@ -475,6 +551,7 @@ template <bool count>
 RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, uint32_t& first, uint32_t& cnt,
                         float& leaf_c0) {
     const float nox[3] = {-(s.q.ox * s.q.ix), -(s.q.oy * s.q.iy), -(s.q.oz * s.q.iz)};
+    const float t0 = walk_start(s);
     while (s.have_node) {
         const uint32_t nidx = s.node;
         const float4* np = reinterpret_cast<const float4*>(M.bvh + nidx);
@@ -486,7 +563,7 @@ RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, ui
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const float lo[3] = {f4c(lx, i), f4c(ly, i), f4c(lz, i)}, hi[3] = {f4c(hx, i), f4c(hy, i), f4c(hz, i)};
-            float a = s.tmin_lo * 0.5f, b = s.best_hi;
+            float a = t0, b = s.best_hi;
             const bool h = box_clip_fma(s.q, nox, lo, hi, a, b);
             t[i] = a;
             hit |= (h && ((s.cmask >> i) & 1u) && u4c(lk, i) != kMeshNone) ? 1u << i : 0u;
@@ -565,7 +642,7 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t fir
             if (tt < best) {
                 best = tt;
                 best_tri = (int32_t)ti;
-                s.best_hi = (float)best * (1.0f + kTolC) + kTolC;
+                s.best_hi = s.tshift == 0.0f ? (float)best * (1.0f + kTolC) + kTolC : shift_up(s.tshift, best);
                 found = true;
                 tie = false;
             } else if (tt == best && found && (int32_t)ti != best_tri) {
@@ -612,8 +689,9 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t fir
                     if (t + dt <= s.tmin_lo || t - dt >= s.best_hi) {
                         maybe = false;
                     } else if (kAny) {
-                        sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du &&
-                                  t - dt > (float)tmin * (1.0f + kTolC) && t + dt < (float)best * (1.0f - kTolC);
+                        const float tmin_up = s.tshift == 0.0f ? (float)tmin * (1.0f + kTolC) : shift_up(s.tshift, tmin);
+                        const float best_dn = s.tshift == 0.0f ? (float)best * (1.0f - kTolC) : shift_down(s.tshift, best);
+                        sure_in = u >= du && v >= du && u + v <= 1.0f - 2.0f * du && t - dt > tmin_up && t + dt < best_dn;
                     }
                 }
             }
@@ -650,44 +728,11 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t fir
 // "while-while" traversal: each lane walks inner nodes until it holds its next leaf,
 // then the wavefront tests leaves together, so lanes at inner nodes do not wait through
 // other lanes' triangle loops. Leaves are still tested in this lane's visit order.
-// The mesh bounds test of MeshObject::hit (src/ray.rs:586-610) in f64 on the exact box,
-// with the interval [tmin, tmax] widened by a relative 2^-30: false only when no point of
-// the box lies on the ray within [tmin, tmax], so no triangle can be hit there.
-RT_HD bool mesh_bounds_may_hit(const MeshDev& M, const RayD& r, double tmin, double tmax) {
-    double lo = tmin, hi = tmax;
-    const double o[3] = {r.o.x, r.o.y, r.o.z}, d[3] = {r.d.x, r.d.y, r.d.z};
-    for (int a = 0; a < 3; ++a) {
-        const double inv = 1.0 / d[a];
-        double t0 = (M.bmin64[a] - o[a]) * inv, t1 = (M.bmax64[a] - o[a]) * inv;
-        if (inv < 0.0) {
-            const double x = t0;
-            t0 = t1;
-            t1 = x;
-        }
-        lo = fmax_r(lo, t0);  // a NaN bound (0 * inf) leaves the interval as it is
-        hi = fmin_r(hi, t1);
-    }
-    if (lo <= hi) return true;
-    const double slack = 0x1p-30 * (fabs(lo) + fabs(hi)) + 1e-300;
-    return isfinite(slack) && lo <= hi + slack;  // an infinite bound: the ray never meets a slab
-}
-
-// Origins farther than this many mesh radii (L1) from the centre take the f64 bounds test
-// first: the FP32 walk's slack grows with |origin| and would admit most of the mesh's nodes
-// (a ray leaving a far point of a plane toward a light above the scene).
-constexpr float kFarOrigin = 256.0f;
-
-// false when the walk set up in s can be skipped: a far origin whose ray misses the box
-RT_HD bool walk_needed(const MeshWalk& s, const MeshDev& M, const RayD& r, double tmin, double best) {
-    return fabsf(s.q.ox) + fabsf(s.q.oy) + fabsf(s.q.oz) <= kFarOrigin * M.radius || mesh_bounds_may_hit(M, r, tmin, best);
-}
-
 template <bool kAny, bool count>
 RT_HD bool mesh_trace(const MeshDev& M, const RayD& r, bool use_kd, double tmin, double& best, int32_t& best_tri,
                       StackRef stk, Work& w) {
     MeshWalk s;
-    walk_init(s, M, r, use_kd, tmin, best);
-    if (!walk_needed(s, M, r, tmin, best)) return false;
+    walk_init<kAny>(s, M, r, use_kd, tmin, best);
     bool found = false;
     bool tie = false;  // another triangle of this mesh hit at exactly `best`
     for (;;) {

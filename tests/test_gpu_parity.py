@@ -448,17 +448,26 @@ def test_axis_parallel_rays_take_the_kd_walk(R, config_dir, view):
         assert getattr(st, k) == ost[k], k
 
 
-def test_far_origin_shadow_rays(R, tmp_path):
+@pytest.mark.parametrize("mesh", ["icosahedron", "icosphere_s4"])
+def test_far_origin_shadow_rays(R, tmp_path, mesh):
     """A wall at x = 1e5 faces a light on the far side of a mesh at the origin; the camera,
     6e4 from the mesh, looks at the wall, so every ray starts far from the mesh and the
     shadow rays pass through it for some pixels and beside it for others. Origins this far
-    from the mesh take the f64 bounds test before the FP32 walk; pixels, hit ids and ray
-    counts match the oracle and the mesh's shadow covers part of the view."""
+    from the mesh take the f64 bounds test, and the BVH walk of a ray that enters the box
+    starts there: pixels, hit ids and ray counts match the oracle, the mesh's shadow covers
+    part of the view, and the walk costs what a near origin's would: with the 5,120-triangle
+    icosphere, 4 triangle tests and 0.7 f64 re-tests per shadow ray, where a walk from the
+    origin itself (FP32 slack of 0.4 units at 1e5) made 350 and 332."""
     import shutil
-    shutil.copy(os.path.join(REF_SCENES, "icosahedron.stl"), str(tmp_path))
+
+    from rtrace_amd import scenes
+    if mesh == "icosahedron":
+        shutil.copy(os.path.join(REF_SCENES, "icosahedron.stl"), str(tmp_path))
+    else:
+        scenes.write_stl(str(tmp_path / "icosphere_s4.stl"), scenes.icosphere(4, 1.0, 0.03, seed=5))
     mat = {"color": "#C0C0C0", "ambient": 0.1, "diffuse": 0.8, "specular": 0.1, "shininess": 8}
     objs = [{"kind": "plane", "point": [1e5, 0, 0], "normal": [-1, 0, 0], "material": mat},
-            {"kind": "mesh", "filename": "icosahedron.stl", "material": dict(mat, color="#FFAA00"),
+            {"kind": "mesh", "filename": f"{mesh}.stl", "material": dict(mat, color="#FFAA00"),
              "transform": ["scale(2, 2, 2)", "translate(0, 0, 1.5)"]}]
     sc = {"camera": {"kind": "perspective", "position": [6e4, 0, 1.5], "target": [1e5, 0, 1.5], "up": [0, 0, 1],
                      "width": 4, "height": 3, "fov": 90},
@@ -474,13 +483,22 @@ def test_far_origin_shadow_rays(R, tmp_path):
     opts = O.make_opts(120, 90, max_depth=2)
     img, st, gh = gpu_render(R, paths[0], opts)
     ref, ost, rh = O.render(paths[0], opts, want_hits=True)
-    _, mism = compare("far origins", img, ref, gh, rh)
+    _, mism = compare(f"far origins {mesh}", img, ref, gh, rh)
     assert mism == 0
     for k in ("rays_primary", "rays_reflect", "rays_shadow"):
         assert getattr(st, k) == ost[k], k
     lit, _, _ = O.render(paths[1], opts)
     shadowed = float(np.mean(np.any(lit != ref, axis=-1)))
     assert 0.2 < shadowed < 0.8, shadowed
+    import torch
+    r = R.Renderer(120, 90)
+    r.max_depth = 2
+    r.count_work = True
+    out = torch.zeros(120 * 90 * 3, dtype=torch.uint8, device="cuda")
+    work = r.render_device(R.Scene.from_json_file(paths[0]), out).as_dict()["work"]["shadow"]
+    per_ray = {k: work[k] / st.rays_shadow for k in ("tris", "retests", "bvh_nodes")}
+    print(f"far origins {mesh}: shadow work per ray {per_ray}")
+    assert per_ray["tris"] < 20 and per_ray["retests"] < 5, per_ray  # a walk from the far origin: > 300 of each
 
 
 def test_frame_graph_replays_match_eager_renders(R, config_dir):

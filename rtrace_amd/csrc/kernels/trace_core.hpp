@@ -375,7 +375,7 @@ RT_HD bool mesh_bounds_may_hit(const MeshDev& M, const RayD& r, double tmin, dou
     enter = lo;
     if (lo <= hi) return true;
     const double slack = 0x1p-30 * (fabs(lo) + fabs(hi)) + 1e-300;
-    return isfinite(slack) && lo <= hi + slack;  // an infinite bound: the ray never meets a slab
+    return slack < INFINITY && lo <= hi + slack;  // an infinite bound: the ray never meets a slab
 }
 
 // Origins farther than this many mesh radii (L1) from the centre take the f64 bounds test

@@ -218,17 +218,35 @@ RT_HOT bool box_clip(const MeshRay& q, const float* bmin, const float* bmax, flo
     return c0 <= c1;
 }
 
-// box_clip with (b - o) * i computed as fma(b, i, -(o * i)); nox = -(o * i) per axis. The
-// extra rounding of o * i (2^-24 |o i|) lies inside the per-axis slack.
-RT_HOT bool box_clip_fma(const MeshRay& q, const float* nox, const float* bmin, const float* bmax, float& c0,
+// box_clip with (b - o) * i computed as fma(b, i, off) and the per-axis slack folded into
+// the offsets: off = -(o * i) - s for the plane the ray enters first on that axis (bmin
+// when i >= 0, else bmax) and -(o * i) + s for the other (box_offsets), so the smaller of
+// the two slab distances is the entry minus the slack and the larger the exit plus it, as
+// in box_clip. The roundings of o * i and of the offsets (2^-24 |o i| each) lie inside the
+// slack, which is 64 times larger.
+struct BoxOffsets {
+    float lo[3], hi[3];  // added to bmin * i and bmax * i
+};
+RT_HOT BoxOffsets box_offsets(const MeshRay& q) {
+    const float nox = -(q.ox * q.ix), noy = -(q.oy * q.iy), noz = -(q.oz * q.iz);
+    BoxOffsets b;
+    b.lo[0] = q.ix >= 0.0f ? nox - q.sx : nox + q.sx;
+    b.hi[0] = q.ix >= 0.0f ? nox + q.sx : nox - q.sx;
+    b.lo[1] = q.iy >= 0.0f ? noy - q.sy : noy + q.sy;
+    b.hi[1] = q.iy >= 0.0f ? noy + q.sy : noy - q.sy;
+    b.lo[2] = q.iz >= 0.0f ? noz - q.sz : noz + q.sz;
+    b.hi[2] = q.iz >= 0.0f ? noz + q.sz : noz - q.sz;
+    return b;
+}
+RT_HOT bool box_clip_fma(const MeshRay& q, const BoxOffsets& o, const float* bmin, const float* bmax, float& c0,
                          float& c1) {
-    float ax = fmaf(bmin[0], q.ix, nox[0]), bx = fmaf(bmax[0], q.ix, nox[0]);
-    float ay = fmaf(bmin[1], q.iy, nox[1]), by = fmaf(bmax[1], q.iy, nox[1]);
-    float az = fmaf(bmin[2], q.iz, nox[2]), bz = fmaf(bmax[2], q.iz, nox[2]);
-    float lo = fmaxf(fmaxf(fminf(ax, bx) - q.sx, fminf(ay, by) - q.sy), fminf(az, bz) - q.sz);
-    float hi = fminf(fminf(fmaxf(ax, bx) + q.sx, fmaxf(ay, by) + q.sy), fmaxf(az, bz) + q.sz);
-    lo -= kTolC * fabsf(lo);
-    hi += kTolC * fabsf(hi);
+    const float ax = fmaf(bmin[0], q.ix, o.lo[0]), bx = fmaf(bmax[0], q.ix, o.hi[0]);
+    const float ay = fmaf(bmin[1], q.iy, o.lo[1]), by = fmaf(bmax[1], q.iy, o.hi[1]);
+    const float az = fmaf(bmin[2], q.iz, o.lo[2]), bz = fmaf(bmax[2], q.iz, o.hi[2]);
+    float lo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    float hi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    lo = fmaf(-kTolC, fabsf(lo), lo);  // widened by 64 ulps: one rounding less than box_clip's form
+    hi = fmaf(kTolC, fabsf(hi), hi);
     c0 = fmaxf(c0, lo);
     c1 = fminf(c1, hi);
     return c0 <= c1;
@@ -550,7 +568,8 @@ RT_HOT uint32_t u4c(const uint4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y
 template <bool count>
 RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, uint32_t& first, uint32_t& cnt,
                         float& leaf_c0) {
-    const float nox[3] = {-(s.q.ox * s.q.ix), -(s.q.oy * s.q.iy), -(s.q.oz * s.q.iz)};
+    static_assert(kMeshBvhDepth / 2 < kMaxKdDepth, "a 4-wide walk's entries fit the per-lane stack");
+    const BoxOffsets off = box_offsets(s.q);
     const float t0 = walk_start(s);
     while (s.have_node) {
         const uint32_t nidx = s.node;
@@ -564,30 +583,31 @@ RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, ui
         for (int i = 0; i < 4; ++i) {
             const float lo[3] = {f4c(lx, i), f4c(ly, i), f4c(lz, i)}, hi[3] = {f4c(hx, i), f4c(hy, i), f4c(hz, i)};
             float a = t0, b = s.best_hi;
-            const bool h = box_clip_fma(s.q, nox, lo, hi, a, b);
+            const bool h = box_clip_fma(s.q, off, lo, hi, a, b);
             t[i] = a;
             hit |= (h && ((s.cmask >> i) & 1u) && u4c(lk, i) != kMeshNone) ? 1u << i : 0u;
         }
+        // the nearest hit child (the first of equal entries) and the least entry of the others
         int bi = -1;
-        float bt = 0.0f;
+        float bt = INFINITY, tr = INFINITY;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            if (((hit >> i) & 1u) && (bi < 0 || t[i] < bt)) {
-                bi = i;
-                bt = t[i];
-            }
+        for (int i = 0; i < 4; ++i) {  // selects, no branches (entries are never NaN)
+            const bool h = ((hit >> i) & 1u) != 0;
+            const bool nearer = h & ((bi < 0) | (t[i] < bt));
+            const float joins = nearer ? bt : t[i];  // the entry that goes to the others
+            tr = (h & (joins < tr)) ? joins : tr;
+            bt = nearer ? t[i] : bt;
+            bi = nearer ? i : bi;
+        }
         if (bi >= 0) {
+            // the other hit children as one entry: written in any case (a node at 4-wide
+            // level L has at most L entries below it, L < kMeshBvhDepth / 2, so slot sp is
+            // inside the stack) and kept when there are any
             const uint32_t rest = hit & ~(1u << bi);
-            if (rest) {
-                float tr = INFINITY;
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if ((rest >> i) & 1u) tr = fminf(tr, t[i]);
-                stk.node[s.sp * stk.stride] = nidx << 4 | rest;
-                stk.t0[s.sp * stk.stride] = tr;
-                ++s.sp;
-            }
-            const uint32_t link = u4c(lk, bi);
+            stk.node[s.sp * stk.stride] = nidx << 4 | rest;
+            stk.t0[s.sp * stk.stride] = tr;
+            s.sp += rest ? 1 : 0;
+            const uint32_t link = bi == 0 ? lk.x : (bi == 1 ? lk.y : (bi == 2 ? lk.z : lk.w));
             if (!(link & kMeshLeaf)) {
                 s.node = link;
                 s.cmask = 0xfu;

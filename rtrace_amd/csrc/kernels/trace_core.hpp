@@ -478,6 +478,7 @@ template <bool count>
 RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, StackRef stk, Work& w, float& leaf_c0) {
     // selects on values, not on member addresses, keep the ray in registers
     const double rox = r.o.x, roy = r.o.y, roz = r.o.z, rdx = r.d.x, rdy = r.d.y, rdz = r.d.z;
+    const BoxOffsets off = box_offsets(s.q);
     uint32_t leaf = kNoLeaf;
     while (s.have_node && leaf == kNoLeaf) {
         bool next = false;  // `node` already holds the next node to visit
@@ -489,7 +490,7 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
             const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
             uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
             uint32_t ax = nb & 3u;
-            if (box_clip(s.q, bmin, bmax, s.c0, s.c1) && s.c0 <= s.best_hi) {
+            if (box_clip_fma(s.q, off, bmin, bmax, s.c0, s.c1) && s.c0 <= s.best_hi) {
                 if (ax == kAxisLeaf) {
                     leaf = s.node;
                     leaf_c0 = s.c0;
@@ -558,6 +559,15 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
 // tmin < t < best. Otherwise improves best / best_tri strictly (first visited wins ties)
 // and flags `tie` when another triangle hits at exactly `best`.
 RT_HOT float f4c(const float4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
+// 1 / x for the FP32 prefilter: the hardware reciprocal on the device (1 ulp; the
+// triangle test's bounds allow 64), a division on the host.
+RT_HOT float rcp_f32(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
 RT_HOT uint32_t u4c(const uint4& v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : (i == 2 ? v.z : v.w)); }
 
 // Walks the 4-wide mesh BVH until it holds the next leaf whose interval can still beat
@@ -684,27 +694,30 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t fir
         float e1x = r0.w, e1y = r1.x, e1z = r1.y;
         float e2x = r1.z, e2y = r1.w, e2z = r2.x;
         float E = r2.z;  // max(|e1|_1, |e2|_1)
-        float hx = q.dy * e2z - q.dz * e2y, hy = q.dz * e2x - q.dx * e2z, hz = q.dx * e2y - q.dy * e2x;
-        float a = e1x * hx + e1y * hy + e1z * hz;
+        // FP32 with fused multiply-adds (one rounding fewer per step than the bounds allow for)
+        const float hx = fmaf(q.dy, e2z, -(q.dz * e2y)), hy = fmaf(q.dz, e2x, -(q.dx * e2z));
+        const float hz = fmaf(q.dx, e2y, -(q.dy * e2x));
+        const float a = fmaf(e1x, hx, fmaf(e1y, hy, e1z * hz));
         float sx = q.ox - r0.x, sy = q.oy - r0.y, sz = q.oz - r0.z;
         float S = fabsf(sx) + fabsf(sy) + fabsf(sz) + s.R;
         float fa = fabsf(a);
         bool sure_in = false;
         bool maybe = true;
         if (fa > 4e-8f + kTolC * E * E) {
-            float f = 1.0f / a;
+            const float f = rcp_f32(a);
             float g = E * fabsf(f);
             float du = kTolC * (S + E) * g + 1e-7f;
-            float u = f * (sx * hx + sy * hy + sz * hz);
+            float u = f * fmaf(sx, hx, fmaf(sy, hy, sz * hz));
             if (u < -du || u > 1.0f + du) {
                 maybe = false;
             } else {
-                float qx = sy * e1z - sz * e1y, qy = sz * e1x - sx * e1z, qz = sx * e1y - sy * e1x;
-                float v = f * (q.dx * qx + q.dy * qy + q.dz * qz);
+                const float qx = fmaf(sy, e1z, -(sz * e1y)), qy = fmaf(sz, e1x, -(sx * e1z));
+                const float qz = fmaf(sx, e1y, -(sy * e1x));
+                float v = f * fmaf(q.dx, qx, fmaf(q.dy, qy, q.dz * qz));
                 if (v < -du || u + v > 1.0f + 2.0f * du) {
                     maybe = false;
                 } else {
-                    float t = f * (e2x * qx + e2y * qy + e2z * qz);
+                    float t = f * fmaf(e2x, qx, fmaf(e2y, qy, e2z * qz));
                     float dt = kTolC * E * (S + fabsf(t)) * g + kTolC * fabsf(t) + 1e-7f;
                     if (t + dt <= s.tmin_lo || t - dt >= s.best_hi) {
                         maybe = false;
@@ -786,11 +799,12 @@ RT_HD void obvh_visit(const SceneView& S, const MeshRay& wq, double tmin, Hi hi,
     uint32_t node = 0;
     int sp = 0;
     float c0 = (float)tmin * 0.5f, c1 = hi();
+    const BoxOffsets off = box_offsets(wq);
     for (;;) {
         bool next = false;
         if (c0 <= hi()) {
             const BvhNode& nd = S.obvh[node];
-            if (box_clip(wq, nd.bmin, nd.bmax, c0, c1) && c0 <= hi()) {
+            if (box_clip_fma(wq, off, nd.bmin, nd.bmax, c0, c1) && c0 <= hi()) {
                 if (nd.b & kBvhLeaf) {
                     uint32_t first = nd.a, cnt = nd.b & ~kBvhLeaf;
                     for (uint32_t k = 0; k < cnt; ++k)
@@ -800,8 +814,8 @@ RT_HD void obvh_visit(const SceneView& S, const MeshRay& wq, double tmin, Hi hi,
                     const BvhNode& L = S.obvh[nd.a];
                     const BvhNode& R = S.obvh[nd.b];
                     float l0 = c0, l1 = hi(), r0 = c0, r1 = hi();
-                    bool hl = box_clip(wq, L.bmin, L.bmax, l0, l1);
-                    bool hr = box_clip(wq, R.bmin, R.bmax, r0, r1);
+                    bool hl = box_clip_fma(wq, off, L.bmin, L.bmax, l0, l1);
+                    bool hr = box_clip_fma(wq, off, R.bmin, R.bmax, r0, r1);
                     if (hl && hr) {
                         bool left_first = l0 <= r0;
                         stk.node[sp * stk.stride] = left_first ? nd.b : nd.a;

@@ -223,7 +223,10 @@ RT_HOT bool box_clip(const MeshRay& q, const float* bmin, const float* bmax, flo
 // when i >= 0, else bmax) and -(o * i) + s for the other (box_offsets), so the smaller of
 // the two slab distances is the entry minus the slack and the larger the exit plus it, as
 // in box_clip. The roundings of o * i and of the offsets (2^-24 |o i| each) lie inside the
-// slack, which is 64 times larger.
+// slack, which is 64 times larger. Only for rays with no direction component below 1e-9
+// (the mesh BVH walk): there |i| <= 1e9 and o * i stays finite. A zero component's
+// stand-in reciprocal (1e30, make_mesh_ray) would overflow o * i for |o| > 3.4e8, so the
+// k-d and sphere-BVH walks, which take such rays, keep box_clip's (b - o) * i.
 struct BoxOffsets {
     float lo[3], hi[3];  // added to bmin * i and bmax * i
 };
@@ -478,7 +481,6 @@ template <bool count>
 RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, StackRef stk, Work& w, float& leaf_c0) {
     // selects on values, not on member addresses, keep the ray in registers
     const double rox = r.o.x, roy = r.o.y, roz = r.o.z, rdx = r.d.x, rdy = r.d.y, rdz = r.d.z;
-    const BoxOffsets off = box_offsets(s.q);
     uint32_t leaf = kNoLeaf;
     while (s.have_node && leaf == kNoLeaf) {
         bool next = false;  // `node` already holds the next node to visit
@@ -490,7 +492,7 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
             const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
             uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
             uint32_t ax = nb & 3u;
-            if (box_clip_fma(s.q, off, bmin, bmax, s.c0, s.c1) && s.c0 <= s.best_hi) {
+            if (box_clip(s.q, bmin, bmax, s.c0, s.c1) && s.c0 <= s.best_hi) {
                 if (ax == kAxisLeaf) {
                     leaf = s.node;
                     leaf_c0 = s.c0;
@@ -799,12 +801,11 @@ RT_HD void obvh_visit(const SceneView& S, const MeshRay& wq, double tmin, Hi hi,
     uint32_t node = 0;
     int sp = 0;
     float c0 = (float)tmin * 0.5f, c1 = hi();
-    const BoxOffsets off = box_offsets(wq);
     for (;;) {
         bool next = false;
         if (c0 <= hi()) {
             const BvhNode& nd = S.obvh[node];
-            if (box_clip_fma(wq, off, nd.bmin, nd.bmax, c0, c1) && c0 <= hi()) {
+            if (box_clip(wq, nd.bmin, nd.bmax, c0, c1) && c0 <= hi()) {
                 if (nd.b & kBvhLeaf) {
                     uint32_t first = nd.a, cnt = nd.b & ~kBvhLeaf;
                     for (uint32_t k = 0; k < cnt; ++k)
@@ -814,8 +815,8 @@ RT_HD void obvh_visit(const SceneView& S, const MeshRay& wq, double tmin, Hi hi,
                     const BvhNode& L = S.obvh[nd.a];
                     const BvhNode& R = S.obvh[nd.b];
                     float l0 = c0, l1 = hi(), r0 = c0, r1 = hi();
-                    bool hl = box_clip_fma(wq, off, L.bmin, L.bmax, l0, l1);
-                    bool hr = box_clip_fma(wq, off, R.bmin, R.bmax, r0, r1);
+                    bool hl = box_clip(wq, L.bmin, L.bmax, l0, l1);
+                    bool hr = box_clip(wq, R.bmin, R.bmax, r0, r1);
                     if (hl && hr) {
                         bool left_first = l0 <= r0;
                         stk.node[sp * stk.stride] = left_first ? nd.b : nd.a;

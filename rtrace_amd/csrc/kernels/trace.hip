@@ -181,8 +181,9 @@ __global__ __launch_bounds__(256) void k_shade_prepare(SceneView S, FrameView F,
     }
 }
 
-// Shadow rays (any hit), 64 queue items per wavefront step.
-template <bool kCount>
+// Shadow rays (any hit), 64 queue items per wavefront step. kByLight (scenes with several
+// lights): items go light by light within blocks of 64 hits (shadow_task).
+template <bool kCount, bool kByLight>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int level) {
     __shared__ uint32_t s_node[kMaxKdDepth * kBlock];
     __shared__ float s_t0[kMaxKdDepth * kBlock];
@@ -191,10 +192,11 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneView S, FrameView F, int
     // < 2^31 items: checked at frame set-up
     WaveQueue<kShadowSlices> queue(&F.take[(3 * level + 1) * kXcd * kQueuePad], F.hit_count[level] * S.shadow_per_hit);
     const int lane = __lane_id();
+    const int n_hits = F.hit_count[level];
     int b, e;
     while (queue.next(b, e))
         for (; b < e; b += 64)
-            if (b + lane < e) shadow_item<kCount>(S, F, b + lane, stk, w);
+            if (b + lane < e) shadow_item<kCount>(S, F, kByLight ? shadow_task(S, n_hits, b + lane) : b + lane, stk, w);
     if (kCount) flush_work(F, w, 1);
 }
 
@@ -362,8 +364,9 @@ void warm_frame_launches() {
     (void)resident_grid((const void*)k_closest<false>, kClosestBlock, 1);
     (void)resident_grid((const void*)k_closest<true>, kClosestBlock, 1);
     (void)resident_grid((const void*)k_resolve, kBlock, 1);
-    (void)resident_grid((const void*)k_shadow<false>, kBlock, 1);
-    (void)resident_grid((const void*)k_shadow<true>, kBlock, 1);
+    (void)resident_grid((const void*)k_shadow<false, false>, kBlock, 1);
+    (void)resident_grid((const void*)k_shadow<false, true>, kBlock, 1);
+    (void)resident_grid((const void*)k_shadow<true, false>, kBlock, 1);
 }
 
 hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1, hipStream_t st, StageTimer* tm) {
@@ -400,12 +403,18 @@ hipError_t launch_frame(const SceneView& S, FrameView F, RayRec* q0, RayRec* q1,
         const long long cap_shadow = (long long)F.n_paths * S.shadow_per_hit;
         if (S.n_lights == 0) {
             // no shadow rays
-        } else if (F.count_work) {
-            hipLaunchKernelGGL(k_shadow<true>, dim3(resident_grid((const void*)k_shadow<true>, kBlock, cap_shadow)),
-                               dim3(kBlock), 0, st, S, F, level);
+        } else if (F.count_work) {  // counts do not depend on the order
+            hipLaunchKernelGGL((k_shadow<true, false>),
+                               dim3(resident_grid((const void*)k_shadow<true, false>, kBlock, cap_shadow)), dim3(kBlock),
+                               0, st, S, F, level);
+        } else if (S.n_lights > 1) {
+            hipLaunchKernelGGL((k_shadow<false, true>),
+                               dim3(resident_grid((const void*)k_shadow<false, true>, kBlock, cap_shadow)), dim3(kBlock),
+                               0, st, S, F, level);
         } else {
-            hipLaunchKernelGGL(k_shadow<false>, dim3(resident_grid((const void*)k_shadow<false>, kBlock, cap_shadow)),
-                               dim3(kBlock), 0, st, S, F, level);
+            hipLaunchKernelGGL((k_shadow<false, false>),
+                               dim3(resident_grid((const void*)k_shadow<false, false>, kBlock, cap_shadow)), dim3(kBlock),
+                               0, st, S, F, level);
         }
         mark(3, false);
         mark(4, true);

@@ -1565,6 +1565,29 @@ RT_HD void shadow_ray(const SceneView& S, const FrameView& F, int task, RayD& r,
     r.d = unit(ld);
 }
 
+// Queue item -> shadow task (hit hq's slot j, task hq * slots + j). Hits go in blocks of 64
+// and, within a block, light by light: the slots of one light for hit 0, hit 1, ... So the
+// lanes of a wavefront trace rays toward one light from neighbouring hits (a point light:
+// 64 rays toward the same point) or the samples of one area light from a few neighbouring
+// hits, instead of rays toward several lights far apart. The last partial block of hits
+// keeps hit-major order.
+RT_HD int shadow_task(const SceneView& S, int n_hits, int item) {
+    if (S.n_lights <= 1) return item;  // one light: already that order
+    const int slots = S.shadow_per_hit;
+    const int blk = item / (64 * slots), in = item - blk * 64 * slots;
+    if ((blk + 1) * 64 > n_hits) return item;  // hit-major: the same task index
+    int off = 0;
+    for (int li = 0; li < S.n_lights; ++li) {
+        const int cnt = light_count(S, S.lights[li]);
+        if (in < 64 * (off + cnt)) {
+            const int r = in - 64 * off;
+            return (blk * 64 + r / cnt) * slots + off + r % cnt;
+        }
+        off += cnt;
+    }
+    return item;  // not reached: the lights' slots add up to `slots`
+}
+
 template <bool count>
 RT_HD void shadow_item(const SceneView& S, const FrameView& F, int task, StackRef stk, Work& w) {
     if (shadow_culled(S, F, task)) {

@@ -69,3 +69,18 @@ def test_mesh_walk_matches_brute_force_on_the_host(tmp_path):
     assert out.returncode == 0, out.stdout + out.stderr
     print(out.stdout)
     assert out.stdout.strip().splitlines()[-1] == "ok 32000"
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"),
+                    reason="needs g++ and the HIP headers")
+def test_shadow_task_order_is_a_permutation(tmp_path):
+    """The shadow stage walks its queue light by light within blocks of 64 hits
+    (shadow_task, compiled for the host): every task is visited exactly once for mixes of
+    point, disk and square lights and hit counts around the block size, and a point light's
+    items in a full block are 64 different hits' rays toward it."""
+    exe = str(tmp_path / "shadow_order_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-o", exe, os.path.join(ROOT, "tests", "cpp", "shadow_order_check.cpp")], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.strip() == "ok 48"

@@ -61,6 +61,9 @@ def lib():
         L.oracle_seed_key.argtypes = [ctypes.c_uint64, ctypes.c_void_p]
         L.oracle_stdrng_u64.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
         L.oracle_stdrng_f64.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]
+        L.oracle_stdrng_construction.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_pcg32_f64.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
         L.oracle_sample_disk.argtypes = [ctypes.c_uint64, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
         L.oracle_area_samples.argtypes = [ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p]
@@ -232,6 +235,24 @@ def stdrng_u64(seed, n):
 def stdrng_f64(seed, n):
     out = np.zeros(n, dtype=np.float64)
     lib().oracle_stdrng_f64(seed, n, out.ctypes.data)
+    return out
+
+
+def stdrng_construction(seed32):
+    """Two draws of rand's StdRng construction sequence (from_seed, next_u64, from_rng, next_u64)."""
+    a = np.frombuffer(bytes(seed32), dtype=np.uint8).copy()
+    assert a.size == 32
+    out = np.zeros(2, dtype=np.uint64)
+    lib().oracle_stdrng_construction(a.ctypes.data, out.ctypes.data)
+    return out
+
+
+def pcg32_f64(state, stream, n, low=None, high=None):
+    """n f64 draws from Pcg32::new(state, stream): gen::<f64>() without bounds, else gen_range(low..high)."""
+    out = np.zeros(n, dtype=np.float64)
+    kind = 0 if low is None else 1
+    lib().oracle_pcg32_f64(state, stream, kind, 0.0 if low is None else low, 0.0 if high is None else high,
+                           n, out.ctypes.data)
     return out
 
 

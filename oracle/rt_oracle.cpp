@@ -1136,8 +1136,14 @@ struct StdRng {
     int idx = 16;
     u64 block = 0;
     explicit StdRng(u64 seed) {
+        u32 key[8];
+        seed_key(seed, key);
+        init(key);
+    }
+    explicit StdRng(const u32 key[8]) { init(key); }  // SeedableRng::from_seed, key words little-endian
+    void init(const u32 key[8]) {
         in[0] = 0x61707865u; in[1] = 0x3320646eu; in[2] = 0x79622d32u; in[3] = 0x6b206574u;
-        seed_key(seed, in + 4);
+        memcpy(in + 4, key, 8 * sizeof(u32));
         in[12] = in[13] = in[14] = in[15] = 0;
     }
     u32 next_u32() {
@@ -1155,25 +1161,62 @@ struct StdRng {
         u64 hi = next_u32();
         return lo | (hi << 32);
     }
-    double gen_f64() {  // Standard for f64: (u64 >> 11) * 2^-53
-        return (double)(next_u64() >> 11) * (1.0 / 9007199254740992.0);
+    // BlockRng::fill_bytes for whole words: consecutive words, each little-endian.
+    void fill_words(u32* out, int n) {
+        for (int i = 0; i < n; ++i) out[i] = next_u32();
     }
-    double gen_range(double low, double high) {  // UniformFloat::<f64>::sample_single
-        double scale = high - low;
-        for (;;) {
-            u64 bits = (next_u64() >> 12) | (1023ull << 52);
-            double v12;
-            memcpy(&v12, &bits, 8);
-            double v01 = v12 - 1.0;
-            double res = v01 * scale + low;
-            if (res < high) return res;
-            u64 sb;
-            memcpy(&sb, &scale, 8);
-            sb -= 1;
-            memcpy(&scale, &sb, 8);
-        }
+    double gen_f64();
+    double gen_range(double low, double high);
+};
+
+// rand_pcg 0.3 Pcg32 (Lcg64Xsh32): only the published value-stability vectors of rand's
+// float distributions use it, so it pins gen_f64 / gen_range below independently of ChaCha.
+struct Pcg32 {
+    u64 state, inc;
+    Pcg32(u64 st, u64 stream) : state(st), inc((stream << 1) | 1) {  // Pcg32::new
+        state += inc;
+        state = state * 6364136223846793005ull + inc;
+    }
+    u32 next_u32() {
+        u64 old = state;
+        state = old * 6364136223846793005ull + inc;
+        u32 xs = (u32)(((old >> 18) ^ old) >> 27);
+        u32 rot = (u32)(old >> 59);
+        return (xs >> rot) | (xs << ((32 - rot) & 31));
+    }
+    u64 next_u64() {  // rand_core impls::next_u64_via_u32: low word first
+        u64 lo = next_u32();
+        u64 hi = next_u32();
+        return lo | (hi << 32);
     }
 };
+
+// rand 0.8 Standard for f64: (next_u64 >> 11) * 2^-53.
+template <class R>
+double gen_f64_of(R& r) {
+    return (double)(r.next_u64() >> 11) * (1.0 / 9007199254740992.0);
+}
+// rand 0.8 UniformFloat::<f64>::sample_single (what gen_range(low..high) calls): a value in
+// [1, 2) from the top 52 bits, minus 1, times the scale plus low; a result reaching `high`
+// retries with the scale lowered by one ulp.
+template <class R>
+double gen_range_of(R& r, double low, double high) {
+    double scale = high - low;
+    for (;;) {
+        u64 bits = (r.next_u64() >> 12) | (1023ull << 52);
+        double v12;
+        memcpy(&v12, &bits, 8);
+        double v01 = v12 - 1.0;
+        double res = v01 * scale + low;
+        if (res < high) return res;
+        u64 sb;
+        memcpy(&sb, &scale, 8);
+        sb -= 1;
+        memcpy(&scale, &sb, 8);
+    }
+}
+double StdRng::gen_f64() { return gen_f64_of(*this); }
+double StdRng::gen_range(double low, double high) { return gen_range_of(*this, low, high); }
 
 // ---------------------------------------------------------------------------
 // Shading (src/lighting.rs)
@@ -2167,6 +2210,21 @@ void oracle_stdrng_u64(uint64_t seed, int n, uint64_t* out) {
 void oracle_stdrng_f64(uint64_t seed, int n, double* out) {
     StdRng r(seed);
     for (int i = 0; i < n; ++i) out[i] = r.gen_f64();
+}
+void oracle_stdrng_construction(const uint8_t seed[32], uint64_t out[2]) {
+    u32 key[8];
+    for (int i = 0; i < 8; ++i)
+        key[i] = (u32)seed[4 * i] | (u32)seed[4 * i + 1] << 8 | (u32)seed[4 * i + 2] << 16 | (u32)seed[4 * i + 3] << 24;
+    StdRng r0(key);
+    out[0] = r0.next_u64();
+    u32 key1[8];
+    r0.fill_words(key1, 8);  // StdRng::from_rng(r0): fill_bytes of a 32-byte seed
+    StdRng r1(key1);
+    out[1] = r1.next_u64();
+}
+void oracle_pcg32_f64(uint64_t state, uint64_t stream, int kind, double low, double high, int n, double* out) {
+    Pcg32 r(state, stream);
+    for (int i = 0; i < n; ++i) out[i] = kind == 0 ? gen_f64_of(r) : gen_range_of(r, low, high);
 }
 void oracle_area_samples(uint64_t path_seed, const double p[3], const double light_pos[3], int square,
                          double extent, int n, double* out) {

@@ -84,6 +84,12 @@ void oracle_seed_key(uint64_t seed, uint32_t key_out[8]);
 void oracle_stdrng_u64(uint64_t seed, int n, uint64_t* out);
 /* n draws of rng.gen::<f64>() */
 void oracle_stdrng_f64(uint64_t seed, int n, double* out);
+/* rand 0.8's StdRng construction sequence: StdRng::from_seed(seed) -> next_u64 -> from_rng(that rng)
+ * (fill_bytes of a 32-byte seed) -> next_u64; writes the two draws. */
+void oracle_stdrng_construction(const uint8_t seed[32], uint64_t out[2]);
+/* n draws from rand_pcg's Pcg32::new(state, stream) through the oracle's f64 recipes:
+ * kind 0 = gen::<f64>() (Standard), kind 1 = gen_range(low..high) (UniformFloat::sample_single). */
+void oracle_pcg32_f64(uint64_t state, uint64_t stream, int kind, double low, double high, int n, double* out);
 /* n calls of sample_disk_point(rng, radius); writes 2n doubles */
 void oracle_sample_disk(uint64_t seed, double radius, int n, double* out_xy);
 /* n shadow-sample points of an area light for a hit at p with path seed `path_seed`

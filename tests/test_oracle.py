@@ -146,6 +146,53 @@ def test_pcg32_output_function_against_demo_sequence():
         s = (s * MUL + 109) & M64
 
 
+# Published known-answer vectors of the crates the reference's RNG comes from (Cargo.toml:
+# rand = "0.8", hence rand 0.8.x with rand_core 0.6 and rand_chacha 0.3). Each is the value a
+# crate's own value-stability test asserts; the oracle reproduces it from its restatement.
+PCG_TEST_INC = 11634580027462260723  # rand 0.8 src/lib.rs test::rng: Pcg32::new(seed, INC)
+
+
+def test_seed_from_u64_value_breakage_vector():
+    """rand_core 0.6 src/lib.rs test_seed_from_u64: an 8-byte seed type seeded from 0 reads
+    5029875928683246316, i.e. the first two PCG32 words of seed_from_u64(0), low word first.
+    The oracle's 32-byte StdRng key starts with the same two words."""
+    key = O.seed_key(0)
+    assert int(key[0]) | (int(key[1]) << 32) == 5029875928683246316
+
+
+def test_stdrng_construction_vector():
+    """rand 0.8 src/rngs/std.rs test_stdrng_construction: from_seed of this seed, next_u64,
+    then from_rng of that generator and next_u64 give these two values. It fixes ChaCha12
+    (6 double rounds), the key in words 4-11, the block counter in words 12-13 from 0, a zero
+    stream id, next_u64 = low word | high word << 32, and fill_bytes word order."""
+    seed = [1, 0, 0, 0, 23, 0, 0, 0, 200, 1, 0, 0, 210, 30, 0, 0] + [0] * 16
+    assert O.stdrng_construction(seed).tolist() == [10719222850664546238, 14064965282130556830]
+
+
+def test_standard_f64_value_stability_vector():
+    """rand 0.8 src/distributions/float.rs value_stability: Standard for f64 on
+    test::rng(0x6f44f5646c2a7334) gives these three values (the gen::<f64>() of the
+    stochastic AA jitter, src/renderer.rs:501-515)."""
+    got = O.pcg32_f64(0x6F44F5646C2A7334, PCG_TEST_INC, 3)
+    assert got.tolist() == [0.7346051961657583, 0.20298547462974248, 0.8166436635290655]
+
+
+def test_uniform_f64_sample_single_value_stability_vector():
+    """rand 0.8 src/distributions/uniform.rs value_stability: sample_single(-1e10, 1e10) on
+    test::rng(897) gives these three values (gen_range(-r..r) of the disk sampler,
+    src/lighting.rs:57-66)."""
+    got = O.pcg32_f64(897, PCG_TEST_INC, 3, -1e10, 1e10)
+    assert got.tolist() == [-4673848682.871551, 6388267422.932352, 4857075081.198343]
+
+
+def test_gen_range_is_not_the_standard_recipe():
+    """A plausible slip, gen_range built on gen::<f64>() (53-bit multiply) instead of the
+    52-bit [1, 2) mantissa, fails the sample_single vector above; check that the two
+    recipes really give different values on that stream."""
+    std = O.pcg32_f64(897, PCG_TEST_INC, 3)
+    assert (std * 2e10 - 1e10).tolist() != [-4673848682.871551, 6388267422.932352, 4857075081.198343]
+
+
 def test_gen_f64_recipe_and_range():
     """gen::<f64>() = (next_u64 >> 11) * 2^-53 (rand 0.8 Standard)."""
     u = O.stdrng_u64(12345, 64)

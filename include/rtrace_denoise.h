@@ -74,18 +74,18 @@ int rt_denoise(rt_denoiser* dn, const float* d_features, int32_t h, int32_t w, f
 
 typedef struct {
     double ms_render;    /* few-sample render with features */
-    double ms_denoise;   /* network + compose + tile decision */
-    double ms_fallback;  /* many-sample re-render of the flagged tiles */
-    double ms_total;
+    double ms_denoise;   /* tile decision + network + compose */
+    double ms_fallback;  /* many-sample re-render of the flagged tiles, on a second stream beside the network */
+    double ms_total;     /* the whole frame, first launch to last (the fallback overlaps the network) */
     int32_t tiles, tiles_fallback;
     uint64_t rays;       /* all rays traced, both renders */
 } rt_denoise_stats;
 
 /* Denoised frame into d_rgb_out (device, w*h*3 u8): render with opts (its light_samples is
- * the low count; 0 = each light's own), denoise, then re-render with `fallback_samples` light
- * samples the tiles (tile x tile pixels) whose mean visibility variance exceeds `threshold`
- * (a negative threshold re-renders every tile, one above 1 none). Needs an opts without the
- * outline pass. */
+ * the low count; 0 = each light's own), then, side by side, denoise and re-render with
+ * `fallback_samples` light samples the tiles (tile x tile pixels) whose mean visibility
+ * variance exceeds `threshold` (a negative threshold re-renders every tile, one above 1 none):
+ * the network's output goes to the other tiles. Needs an opts without the outline pass. */
 int rt_render_denoised(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, rt_denoiser* dn,
                        int32_t fallback_samples, float threshold, int32_t tile, void* d_rgb_out,
                        rt_denoise_stats* stats);

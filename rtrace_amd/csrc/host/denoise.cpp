@@ -124,8 +124,10 @@ struct rt_denoiser {
     // training buffers
     rt::Buf g[RT_DN_LAYERS - 1], gpool, dz, dx, part, bpart, lpart;
     // denoised frames: features, tile mask and flag count, the fallback render's frame buffers
+    // and the stream it runs on beside the network (default priority, like every stream here)
     rt::Buf feat, mask, flagged, fb_arena;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipStream_t st_fb = nullptr;
+    hipEvent_t ev[7] = {};
 };
 
 namespace rt {
@@ -341,6 +343,7 @@ int rt_denoiser_create(rt_ctx* ctx, uint64_t seed, rt_denoiser** out) {
         DN_TRY(hipEventCreate(&d->e0));
         DN_TRY(hipEventCreate(&d->e1));
         for (auto& e : d->ev) DN_TRY(hipEventCreate(&e));
+        DN_TRY(hipStreamCreateWithFlags(&d->st_fb, hipStreamNonBlocking));
         const Offsets& o = offsets();
         d->n_params = o.total;
         // He-normal weights over the real fan-in (13 input features for layer 0); padded
@@ -378,6 +381,10 @@ int rt_denoiser_create(rt_ctx* ctx, uint64_t seed, rt_denoiser** out) {
 void rt_denoiser_destroy(rt_denoiser* dn) {
     if (!dn) return;
     if (dn->st) (void)hipStreamSynchronize(dn->st);
+    if (dn->st_fb) {
+        (void)hipStreamSynchronize(dn->st_fb);
+        (void)hipStreamDestroy(dn->st_fb);
+    }
     if (dn->e0) (void)hipEventDestroy(dn->e0);
     if (dn->e1) (void)hipEventDestroy(dn->e1);
     for (auto e : dn->ev)
@@ -474,7 +481,8 @@ int rt_denoise(rt_denoiser* dn, const float* d_features, int32_t h, int32_t w, f
         ensure_shape(dn, 1, h, w, false);
         DN_TRY(hipEventRecord(dn->e0, dn->st));
         forward(dn, d_features, h, w);
-        DN_TRY(dn::launch_compose(dn->act[kOut].as<float>(), d_features, h, w, dn->Wp, d_rgb_f32, d_rgb_u8, dn->st));
+        DN_TRY(dn::launch_compose(dn->act[kOut].as<float>(), d_features, h, w, dn->Wp, d_rgb_f32, d_rgb_u8, nullptr, 0,
+                                  dn->st));
         DN_TRY(hipEventRecord(dn->e1, dn->st));
         DN_TRY(hipEventSynchronize(dn->e1));
         if (ms_out) DN_TRY(hipEventElapsedTime(ms_out, dn->e0, dn->e1));
@@ -515,21 +523,30 @@ int rt_render_denoised(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts,
         a2.tile_mask = dn->mask.as<uint8_t>();
         a2.mask_tile = tile;
         PendingFrame p1{}, p2{};
+        hipStream_t sf = dn->st_fb;
         DN_TRY(hipEventRecord(dn->ev[0], st));
         ctx_render_launch(ctx, scene, &o1, d_rgb_out, &aux, nullptr, &p1);
         DN_TRY(hipEventRecord(dn->ev[1], st));
-        ensure_shape(dn, 1, H, W, false);
-        forward(dn, dn->feat.as<float>(), H, W);
-        DN_TRY(dn::launch_compose(dn->act[kOut].as<float>(), dn->feat.as<float>(), H, W, dn->Wp, nullptr,
-                                  static_cast<uint8_t*>(d_rgb_out), st));
+        // the tile decision needs only the features: the flagged tiles' many-sample re-render
+        // starts on the second stream while the network runs on this one; the compose leaves
+        // the flagged tiles' pixels to it, so the two write disjoint pixels of d_rgb_out
         DN_TRY(hipMemsetAsync(dn->flagged.p, 0, sizeof(int), st));
         DN_TRY(dn::launch_tile_mask(dn->feat.as<float>(), H, W, tile, threshold, dn->mask.as<uint8_t>(),
                                     dn->flagged.as<int>(), st));
         DN_TRY(hipEventRecord(dn->ev[2], st));
+        DN_TRY(hipStreamWaitEvent(sf, dn->ev[2], 0));
+        DN_TRY(hipEventRecord(dn->ev[3], sf));
         // every flagged tile re-rendered with the many-sample light; with none flagged the
         // launches find empty queues (no host round trip to find out)
-        ctx_render_launch(ctx, scene, &o2, d_rgb_out, &a2, &fb_arena, &p2);
-        DN_TRY(hipEventRecord(dn->ev[3], st));
+        ctx_render_launch(ctx, scene, &o2, d_rgb_out, &a2, &fb_arena, &p2, sf);
+        DN_TRY(hipEventRecord(dn->ev[4], sf));
+        ensure_shape(dn, 1, H, W, false);
+        forward(dn, dn->feat.as<float>(), H, W);
+        DN_TRY(dn::launch_compose(dn->act[kOut].as<float>(), dn->feat.as<float>(), H, W, dn->Wp, nullptr,
+                                  static_cast<uint8_t*>(d_rgb_out), dn->mask.as<uint8_t>(), tile, st));
+        DN_TRY(hipEventRecord(dn->ev[5], st));
+        DN_TRY(hipStreamWaitEvent(st, dn->ev[4], 0));  // the frame ends with both
+        DN_TRY(hipEventRecord(dn->ev[6], st));
         int n_flagged = 0;
         DN_TRY(hipMemcpyAsync(&n_flagged, dn->flagged.p, sizeof(int), hipMemcpyDeviceToHost, st));
         DN_TRY(hipStreamSynchronize(st));
@@ -537,14 +554,15 @@ int rt_render_denoised(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts,
             rt_render_stats s1{}, s2{};
             frame_stats(ctx, p1, &s1, true);
             frame_stats(ctx, p2, &s2, false);
-            float a = 0, b = 0, c = 0;
+            float a = 0, b = 0, c = 0, t = 0;
             DN_TRY(hipEventElapsedTime(&a, dn->ev[0], dn->ev[1]));
-            DN_TRY(hipEventElapsedTime(&b, dn->ev[1], dn->ev[2]));
-            DN_TRY(hipEventElapsedTime(&c, dn->ev[2], dn->ev[3]));
+            DN_TRY(hipEventElapsedTime(&b, dn->ev[1], dn->ev[5]));
+            DN_TRY(hipEventElapsedTime(&c, dn->ev[3], dn->ev[4]));
+            DN_TRY(hipEventElapsedTime(&t, dn->ev[0], dn->ev[6]));
             stats->ms_render = a;
             stats->ms_denoise = b;
             stats->ms_fallback = c;
-            stats->ms_total = (double)a + b + c;
+            stats->ms_total = t;
             stats->tiles = tiles;
             stats->tiles_fallback = n_flagged;
             stats->rays = s1.rays_primary + s1.rays_reflect + s1.rays_shadow + s2.rays_primary + s2.rays_reflect +

@@ -113,8 +113,10 @@ size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char*
 void frame_stats(rt_ctx* ctx, const PendingFrame& pf, rt_render_stats* stats, bool record_primary);
 // rt_render_device_aux enqueued without waiting: frame buffers in `arena` (nullable = the
 // context's arena); the stats come from frame_stats(*pending) after the stream is done.
+// on_stream: launch on that stream instead of the context's (pending frames with their own
+// buffers only; the caller orders it against the context's stream with events).
 void ctx_render_launch(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
-                       const FixedScratch* arena, PendingFrame* pending);
+                       const FixedScratch* arena, PendingFrame* pending, hipStream_t on_stream = nullptr);
 // Device bytes of a frame's buffers for these options.
 size_t ctx_frame_bytes(rt_scene* scene, const rt_render_opts* o);
 

@@ -509,7 +509,7 @@ uint64_t primary_rays(const FrameView& F, long long n_paths) {
 
 void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host_rgb, void* dev_rgb,
             rt_render_stats* stats, const rt_render_aux* aux = nullptr, const FixedScratch* fixed = nullptr,
-            PendingFrame* pending = nullptr) {
+            PendingFrame* pending = nullptr, hipStream_t on_stream = nullptr) {
     if (!scene || !scene->dev || !o) throw Error(RT_E_INVALID_ARG, "null scene or options");
     if (o->samples == 0) throw Error(RT_E_INVALID_ARG, "Samples must be greater than 0");
     if (o->samples < 0 || o->width <= 0 || o->height <= 0) throw Error(RT_E_INVALID_ARG, "invalid image size or samples");
@@ -519,7 +519,11 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
     CameraDev cam = make_camera(D.host.camera, (double)W / (double)H);
     if (o->light_samples < 0 || o->light_samples > 65536) throw Error(RT_E_INVALID_ARG, "light_samples out of range");
     SceneView S = make_view(D, cam, o->light_samples);
-    hipStream_t st = ctx->stream;
+    // a frame launched on another stream (a pending one with its own buffers) leaves the
+    // context's stream, graph and timing events alone
+    if (on_stream && (!pending || !fixed || o->profile))
+        throw Error(RT_E_INTERNAL, "a frame on another stream must be pending, with its own buffers and untimed");
+    hipStream_t st = on_stream ? on_stream : ctx->stream;
 
     FrameView F{};
     const FrameLayout L = frame_fill(F, S, o, kRaygenTile);
@@ -604,10 +608,10 @@ void render(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, uint8_t* host
             replay = true;
         }
     }
-    HIP_TRY(hipEventRecord(ctx->ev0, st));
+    if (!on_stream) HIP_TRY(hipEventRecord(ctx->ev0, st));
     if (replay) HIP_TRY(hipGraphLaunch(ctx->frame_exec, st));
     else HIP_TRY(launch_frame(S, F, q0, q1, st, timed_stages ? &ctx->timer : nullptr));
-    HIP_TRY(hipEventRecord(ctx->ev1, st));
+    if (!on_stream) HIP_TRY(hipEventRecord(ctx->ev1, st));
     if (host_rgb) HIP_TRY(hipMemcpyAsync(host_rgb, F.rgb, (size_t)W * H * 3, hipMemcpyDeviceToHost, st));
     PendingFrame pf{F, S, n_paths, depth, timed_stages, fixed != nullptr, &D};
     if (pending) {
@@ -770,8 +774,8 @@ void ctx_render_aux(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void*
     render(ctx, scene, o, nullptr, dev_rgb, stats, aux);
 }
 void ctx_render_launch(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* o, void* dev_rgb, const rt_render_aux* aux,
-                       const FixedScratch* arena, PendingFrame* pending) {
-    render(ctx, scene, o, nullptr, dev_rgb, nullptr, aux, arena, pending);
+                       const FixedScratch* arena, PendingFrame* pending, hipStream_t on_stream) {
+    render(ctx, scene, o, nullptr, dev_rgb, nullptr, aux, arena, pending, on_stream);
 }
 size_t ctx_frame_bytes(rt_scene* scene, const rt_render_opts* o) {
     SceneDevice& D = *scene->dev;

@@ -598,15 +598,20 @@ __global__ __launch_bounds__(256) void k_loss(const float* y, const float* feat,
 }
 
 // Denoised frame: rgb = noisy + y (f32 and/or quantised like create_image_from_data).
-__global__ void k_compose(const float* y, const float* feat, int H, int W, int Wp, float* out_f32, uint8_t* out_u8) {
+// skip (optional): T x T tile flags; the 8-bit output of a flagged tile is left alone (the
+// tile-masked fallback render writes it).
+__global__ void k_compose(const float* y, const float* feat, int H, int W, int Wp, float* out_f32, uint8_t* out_u8,
+                          const uint8_t* skip, int T) {
     const long long n = (long long)H * W;
+    const int tx = skip ? (W + T - 1) / T : 0;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
         const int x = (int)(i % W), yy = (int)(i / W);
         const size_t q = ((size_t)yy * Wp + x) * 3;
+        const bool keep = skip && skip[(yy / T) * tx + x / T];
         for (int c = 0; c < 3; ++c) {
             const float v = feat[i * kFeatIn + c] + y[q + c];
             if (out_f32) out_f32[i * 3 + c] = v;
-            if (out_u8) {
+            if (out_u8 && !keep) {
                 const float cl = fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f;
                 out_u8[i * 3 + c] = (uint8_t)(cl >= 255.0f ? 255 : (int)cl);
             }
@@ -749,9 +754,10 @@ hipError_t launch_loss(const float* y, const float* feat, const float* target, i
     return hipGetLastError();
 }
 hipError_t launch_compose(const float* y, const float* feat, int H, int W, int Wp, float* out_f32, uint8_t* out_u8,
-                          hipStream_t st) {
+                          const uint8_t* skip, int T, hipStream_t st) {
+    if (skip && T <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_compose, dim3(grid1((long long)H * W, 256, 8192)), dim3(256), 0, st, y, feat, H, W, Wp,
-                       out_f32, out_u8);
+                       out_f32, out_u8, skip, T);
     return hipGetLastError();
 }
 hipError_t launch_tile_mask(const float* feat, int H, int W, int T, float threshold, uint8_t* mask, int* n_flagged,

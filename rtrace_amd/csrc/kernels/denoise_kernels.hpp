@@ -53,8 +53,9 @@ hipError_t launch_split(const float* dx, int B, int H, int W, int c0, int c1, fl
 hipError_t launch_relu_bwd(const float* g, const uint16_t* a, long long n, uint16_t* dz, hipStream_t st);
 hipError_t launch_loss(const float* y, const float* feat, const float* target, int B, int H, int W, int Hp, int Wp,
                        uint16_t* dz, float* part, int blocks, hipStream_t st);
+// skip (nullable): T x T tile flags whose 8-bit output is left to the fallback render
 hipError_t launch_compose(const float* y, const float* feat, int H, int W, int Wp, float* out_f32, uint8_t* out_u8,
-                          hipStream_t st);
+                          const uint8_t* skip, int T, hipStream_t st);
 hipError_t launch_tile_mask(const float* feat, int H, int W, int T, float threshold, uint8_t* mask, int* n_flagged,
                             hipStream_t st);
 hipError_t launch_adam(float* w, const float* grad, float* m, float* v, long long n, float lr, float b1, float b2,

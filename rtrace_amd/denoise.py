@@ -191,6 +191,9 @@ def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06
         r.profile = True
         st_f = r.render_device_aux(scene, scratch, features=feat)
         r.profile = False
+        # the network alone on those features (prep, convolutions, compose: device time)
+        dn.denoise(feat.view(height, width, 13))
+        ms_net = float(np.median([dn.denoise(feat.view(height, width, 13)) for _ in range(repeats)]))
         del feat, scratch
         render_stages = {k: round(v, 3) for k, v in st_f.as_dict()["ms_stage"].items()}
         row = {"seed": s, "ms_plain_high": round(ms_plain, 3), "ms_plain_low": round(ms_low, 3),
@@ -199,7 +202,8 @@ def evaluate(dn, seeds, width=1920, height=1080, low=4, high=256, threshold=0.06
                "ssim_low": round(ssim(noisy, ref), 5), "ssim_denoised": round(ssim(den, ref), 5),
                "tiles_fallback": st_dn.tiles_fallback, "tiles": st_dn.tiles,
                "stage_ms": {"render": round(st_dn.ms_render, 3), "denoise": round(st_dn.ms_denoise, 3),
-                            "fallback": round(st_dn.ms_fallback, 3)},
+                            "fallback": round(st_dn.ms_fallback, 3), "total": round(st_dn.ms_total, 3)},
+               "ms_network": round(ms_net, 3),
                "render_stages_ms": render_stages,
                "rays_plain_high": int(st_plain.rays_total)}
         rows.append(row)

@@ -159,6 +159,48 @@ def test_render_denoised_tile_fallback(DN, square_scene):
     assert torch.equal(out2, want)
 
 
+def test_render_denoised_mixed_tiles(DN, square_scene):
+    """With some tiles flagged, the fallback (on its own stream, beside the network) and the
+    compose write disjoint pixels: flagged tiles hold the many-sample render, exactly, and the
+    others the network's output; repeated frames give the same bytes."""
+    import torch
+    import rtrace_amd as R
+    W, H, T = 160, 90, 16
+    scene = R.Scene.from_json_file(square_scene)
+    r = R.Renderer(W, H)
+    r.max_depth = 4
+    ref = torch.zeros(H * W * 3, dtype=torch.uint8, device="cuda")
+    r.light_samples = 64
+    r.render_device(scene, ref)
+    r.light_samples = 4
+    feat = torch.zeros((H, W, R.FEAT_COUNT), dtype=torch.float32, device="cuda")
+    low = torch.zeros_like(ref)
+    r.render_device_aux(scene, low, features=feat)
+    dn = DN.Denoiser(seed=16)
+    want = torch.zeros_like(ref)
+    dn.denoise(feat, None, want)
+    # the tile decision on the host: mean visibility variance (feature 11) per tile
+    var = feat[..., 11].cpu().numpy().astype(np.float64)
+    ty, tx = (H + T - 1) // T, (W + T - 1) // T
+    means = np.array([[var[y * T:(y + 1) * T, x * T:(x + 1) * T].mean() for x in range(tx)] for y in range(ty)])
+    thr = float(np.median(means))
+    flagged = means > thr
+    assert 0 < flagged.sum() < flagged.size
+    pix = np.kron(flagged, np.ones((T, T), dtype=bool))[:H, :W]
+    outs = []
+    for _ in range(3):
+        out = torch.zeros_like(ref)
+        st = dn.render(r, scene, out, fallback_samples=64, threshold=thr, tile=T)
+        assert st.tiles_fallback == int(flagged.sum())
+        assert st.ms_total >= max(st.ms_render, st.ms_fallback)
+        outs.append(out.cpu().numpy().reshape(H, W, 3))
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    o = outs[0]
+    assert np.array_equal(o[pix], ref.cpu().numpy().reshape(H, W, 3)[pix])
+    assert np.array_equal(o[~pix], want.cpu().numpy().reshape(H, W, 3)[~pix])
+
+
 def test_trained_network_beats_the_plain_few_sample_render(DN, tmp_path):
     """The shipped weights on held-out denoiser scenes: against the 256-sample render the
     denoised frame gains PSNR and SSIM over the 4-sample frame it starts from, and clears

@@ -171,7 +171,7 @@ typedef struct {
     uint64_t planes;   /* plane tests */
     uint64_t cubes;    /* cube tests */
     uint64_t skipped;  /* shadow queries not traced: no sample of their light faces the hit point, so
-                          their answers cannot change the image (not with feature output) */
+                          their answers cannot change the image */
     uint64_t bvh_nodes; /* mesh BVH nodes fetched (128 B: four slot boxes and links) */
 } rt_work;
 
@@ -229,8 +229,9 @@ int rt_render_device(rt_ctx* ctx, rt_scene* scene, const rt_render_opts* opts, v
  *   0-2 radiance RGB (the pixel's colour before quantisation), 3-5 albedo RGB (material
  *   colour after textures), 6-8 normal XYZ (facing the ray), 9 depth (distance to the
  *   camera position), 10 mean visibility of the area-light samples (of all shadow rays
- *   when no light is an area light), 11 its binomial variance mean*(1-mean)/n, 12 fog
- *   factor. */
+ *   when no light is an area light; the samples of a light none of whose samples faces the
+ *   hit point count as not visible, as they are not traced), 11 its binomial variance
+ *   mean*(1-mean)/n, 12 fog factor. */
 enum { RT_FEAT_COUNT = 13 };
 typedef struct {
     float* features;           /* device, W*H*RT_FEAT_COUNT f32, or NULL */

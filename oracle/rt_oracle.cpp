@@ -1289,10 +1289,14 @@ const Material* apply_texture(const Material& m, double u, double v, Material& s
     return &m;
 }
 // First-hit feature probe (extension outputs for the denoiser; no reference counterpart):
-// shadow-sample visibility counts, effective material colour and fog factor.
+// shadow-sample visibility counts, effective material colour and fog factor. A light none of
+// whose samples faces the hit point (diffuse strength max(n.l, 0) = 0 for every sample; only
+// the first 32 lights of a scene are judged so) counts its samples as taken and not visible:
+// its contribution is +0 whatever they see, so they need not be traced.
 struct Probe {
     int nv = 0, nt = 0;  // area-light samples visible / taken
     int pv = 0, pt = 0;  // point lights visible / taken
+    int light = 0;       // index of the light being shaded
     V3 mc{};
     double fog = 0.0;
 };
@@ -1305,8 +1309,8 @@ V3 point_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, 
     C.shadow++;
     if (pr) pr->pt++;
     if (X.world->hit(sr, 0.001, dist, tmp, C)) return v3(0, 0, 0);
-    if (pr) pr->pv++;
     double ds = rmax(dot(h.n, ld), 0.0);
+    if (pr && (ds > 0.0 || pr->light >= 32)) pr->pv++;
     V3 diffuse = ((mat.diffuse * ds) * L.intensity) * cmul(lc, mc);
     V3 spec = v3(0, 0, 0);
     if (ds > 0.0) {
@@ -1371,10 +1375,12 @@ V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V
     const int cols = jitter_cols(SAMPLES), rows = SAMPLES / cols;
     V3 total = v3(0, 0, 0);
     int visible = 0;
+    bool faces = false;  // some sample has diffuse strength > 0 (the feature probe's rule)
     for (int s = 0; s < SAMPLES; ++s) {
         V3 sp = area_sample(L, h.p, rng, ls, s, cols, rows);
         V3 ld = normalize(sp - h.p);
         double dist = norm(sp - h.p);
+        if (rmax(dot(h.n, ld), 0.0) > 0.0) faces = true;
         Ray sr = make_ray(h.p + 0.001 * h.n, ld);
         Hit tmp;
         C.shadow++;
@@ -1393,7 +1399,7 @@ V3 area_light(const Ctx& X, const Hit& h, const Material& mat, const Light& L, V
     }
     if (pr) {
         pr->nt += SAMPLES;
-        pr->nv += visible;
+        if (faces || pr->light >= 32) pr->nv += visible;
     }
     return (total / (double)SAMPLES) * ((double)visible / (double)SAMPLES);
 }
@@ -1404,8 +1410,10 @@ V3 phong(const Ctx& X, const Hit& h, const Material& mat, u64 seed, Counters& C,
     V3 mc = hex_or(eff->color, v3(1, 1, 1));
     V3 ac = hex_or(X.amb_color, v3(1, 1, 1));
     V3 color = (eff->ambient * X.amb_intensity) * cmul(ac, mc);
-    for (const Light& L : X.lights) {
+    for (size_t li = 0; li < X.lights.size(); ++li) {
+        const Light& L = X.lights[li];
         V3 lc = hex_or(L.color, v3(1, 1, 1));
+        if (pr) pr->light = (int)li;
         V3 contrib = (L.has_d || L.square) ? area_light(X, h, *eff, L, lc, mc, seed, C, pr)
                                            : point_light(X, h, *eff, L, lc, mc, C, pr);
         color = color + contrib;

@@ -1491,8 +1491,8 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
         o[3] = n.z;
     }
     // shadow sample points of point and disk lights, in scene order (src/lighting.rs:251-285),
-    // and the lights whose shadow queries are not traced (light_lit; not with features:
-    // the visibility is then an input of the denoiser)
+    // and the lights whose shadow queries are not traced (light_lit; with features, such a
+    // light's samples count as not visible)
     uint64_t seed = F.path_seed[rr.path];
     double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
     uint32_t skip = 0;
@@ -1500,7 +1500,7 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
     for (int li = 0; li < S.n_lights; ++li) {
         const LightDev& L = S.lights[li];
         d3 lp = ld3(L.pos);
-        const bool cull = F.feat == nullptr && li < 32;
+        const bool cull = li < 32;
         if (!L.is_area) {
             st3(spt + 3 * j, lp);
             ++j;
@@ -1721,12 +1721,14 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq, con
             fp[kFeatNormal + 1] = (float)n.y;
             fp[kFeatNormal + 2] = (float)n.z;
             fp[kFeatDepth] = (float)length(sub(p, cam));
-            // visibility of the area-light samples (all shadow slots when there is no area light)
+            // visibility of the area-light samples (all shadow slots when there is no area light);
+            // a light none of whose samples faces the hit was not traced and counts as unseen
             int nv = 0, nt = 0, pv = 0, pt = 0;
             for (int li = 0, jj = 0; li < S.n_lights; ++li) {
                 const int cnt = light_count(S, S.lights[li]);
                 int v = 0;
-                for (int q = 0; q < cnt; ++q) v += vis[jj + q];
+                if (!(li < 32 && ((culled >> li) & 1u)))
+                    for (int q = 0; q < cnt; ++q) v += vis[jj + q];
                 if (S.lights[li].is_area) {
                     nv += v;
                     nt += cnt;

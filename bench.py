@@ -158,9 +158,10 @@ DN_LAYERS = [(13, 32, 0), (32, 32, 0), (32, 64, 1), (64, 64, 1), (64, 128, 2), (
 DN_MAC_PER_PX = sum(9 * ci * co / 4 ** lv for ci, co, lv in DN_LAYERS)
 
 
-def denoise_section(seeds, threshold, low=4, high=256):
+def denoise_section(seeds, threshold, low=4, high=256, variant_samples=(2, 1)):
     """The denoiser workload at 1080p: the plain render with `high` light samples (the quality reference) against
-    the denoised render (`low` samples + network + tile fallback), per image."""
+    the denoised render (`low` samples + network + tile fallback), per image; then the same with the networks
+    trained on renders with `variant_samples` light samples per hit."""
     import rtrace_amd as R  # noqa: F401 - loads the library
     from rtrace_amd import denoise
     dn = denoise.Denoiser.load()
@@ -172,6 +173,26 @@ def denoise_section(seeds, threshold, low=4, high=256):
     ach = flops / (ms_dn_net * 1e-3) / 1e12
     plain = sum(r["ms_plain_high"] for r in rows) / len(rows)
     den = sum(r["ms_denoised"] for r in rows) / len(rows)
+    # cheaper network inputs: the same frames from renders with fewer light samples per hit, each
+    # with the network trained on them and the fallback threshold at the same share of the
+    # variance estimate's maximum
+    variants = []
+    for n in variant_samples:
+        dv = denoise.Denoiser.load(denoise.WEIGHTS_BY_SAMPLES[n])
+        thr = denoise.fallback_threshold(n, threshold)
+        vr = denoise.evaluate(dv, seeds, 1920, 1080, low=n, high=high, threshold=thr, repeats=3,
+                              log=lambda m, n=n: _progress(f"denoise variant {n}: " + m[:140]))
+        vp = sum(r["ms_plain_high"] for r in vr) / len(vr)
+        vd = sum(r["ms_denoised"] for r in vr) / len(vr)
+        variants.append({
+            "low_samples": n, "threshold": thr, "ms_per_frame_plain": round(vp, 3),
+            "ms_per_frame_denoised": round(vd, 3), "speedup": round(vp / vd, 2),
+            "psnr_min": min(r["psnr_denoised"] for r in vr), "ssim_min": min(r["ssim_denoised"] for r in vr),
+            "psnr_low_min": min(r["psnr_low"] for r in vr),
+            "per_image": [{k: r[k] for k in ("seed", "ms_denoised", "speedup", "psnr_low", "psnr_denoised",
+                                             "ssim_denoised", "tiles_fallback", "stage_ms", "render_stages_ms")}
+                          for r in vr]})
+        del dv
     return {
         "workload": "denoise_1080p (icosphere scenes, 2x2 square light), 1920x1080 quincunx, depth 4",
         "low_samples": low, "high_samples": high, "threshold": threshold, "tile": 16,
@@ -179,6 +200,7 @@ def denoise_section(seeds, threshold, low=4, high=256):
         "speedup": round(plain / den, 2),
         "psnr_min": min(r["psnr_denoised"] for r in rows), "psnr_low_min": min(r["psnr_low"] for r in rows),
         "per_image": rows,
+        "variants": variants,
         "roofline": {"bound": "mfma", "kernel": "denoiser (prep, 15 conv3x3, pools, compose)",
                      "achieved": round(ach, 1), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(ach / BF16_PEAK_TFLOPS, 4), "flop_per_frame": flops,

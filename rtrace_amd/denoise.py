@@ -1,7 +1,8 @@
 """Python face of the shadow denoiser (include/rtrace_denoise.h) over ctypes.
 
 Device tensors are torch CUDA tensors; the library only sees their pointers. The trained
-parameters ship as `rtrace_amd/data/denoiser.npy` (float32, the C ABI's parameter layout).
+parameters ship as `rtrace_amd/data/denoiser*.npy` (float32, the C ABI's parameter layout), one
+set per light-sample count of the network's input render (WEIGHTS_BY_SAMPLES).
 """
 import ctypes
 import os
@@ -10,8 +11,20 @@ import numpy as np
 
 from . import _check, _context, lib
 
-WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "denoiser.npy")
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+WEIGHTS = os.path.join(_DATA, "denoiser.npy")
+# Networks trained on input renders with this many light samples per hit (4: the configuration's
+# own; 2 and 1: cheaper inputs). Each is trained by rtrace_amd.train_denoiser (its docstring
+# gives the commands).
+WEIGHTS_BY_SAMPLES = {4: WEIGHTS, 2: os.path.join(_DATA, "denoiser_s2.npy"), 1: os.path.join(_DATA, "denoiser_s1.npy")}
 N_LAYERS = 15
+
+
+def fallback_threshold(samples, at_four=0.06):
+    """The tile-fallback threshold for `samples` light samples per hit: the binomial variance
+    estimate m(1-m)/n is at most 0.25/n, so the threshold keeps the same fraction of its maximum
+    as `at_four` does at 4 samples. With 1 sample every estimate is 0 and no tile is flagged."""
+    return at_four * 4.0 / samples
 
 
 class DenoiseStats(ctypes.Structure):

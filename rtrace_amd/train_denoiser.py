@@ -1,6 +1,11 @@
 """Train the shadow denoiser on the GPU from seeded random scenes.
 
-    python -m rtrace_amd.train_denoiser --scenes 2000 --steps 4000 --out rtrace_amd/data/denoiser.npy
+The shipped parameter sets (rtrace_amd.denoise.WEIGHTS_BY_SAMPLES):
+
+    python -m rtrace_amd.train_denoiser --low 4 --steps 8000 --batch 32 --out rtrace_amd/data/denoiser.npy
+    python -m rtrace_amd.train_denoiser --low 2 --steps 8000 --batch 32 --out rtrace_amd/data/denoiser_s2.npy
+    python -m rtrace_amd.train_denoiser --low 1 --steps 16000 --batch 32 --lr 3e-4 --warmup 200 \
+        --out rtrace_amd/data/denoiser_s1.npy
 
 Data: denoiser scenes (rtrace_amd.scenes.denoise_scene, seeds 0..N-1) rendered by the plain
 method twice at a training resolution: `--low` light samples with the per-pixel features
@@ -68,6 +73,7 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--crop", type=int, default=128)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--warmup", type=int, default=0, help="steps of linear learning-rate warm-up")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--init", default=None, help="start from these parameters (.npy)")
     ap.add_argument("--out", required=True)
@@ -93,7 +99,8 @@ def main(argv=None):
     t1 = time.time()
     ema = None
     for step in range(args.steps):
-        lr = args.lr * 0.5 * (1 + math.cos(math.pi * step / args.steps)) + 1e-6
+        warm = min(1.0, (step + 1) / args.warmup) if args.warmup > 0 else 1.0
+        lr = args.lr * warm * 0.5 * (1 + math.cos(math.pi * step / args.steps)) + 1e-6
         f, t = crops(feats, targets, args.batch, args.crop, gen)
         loss = dn.train_step(f, t, lr)
         if not math.isfinite(loss):

@@ -201,19 +201,20 @@ def test_render_denoised_mixed_tiles(DN, square_scene):
     assert np.array_equal(o[~pix], want.cpu().numpy().reshape(H, W, 3)[~pix])
 
 
-def test_trained_network_beats_the_plain_few_sample_render(DN, tmp_path):
-    """The shipped weights on held-out denoiser scenes: against the 256-sample render the
-    denoised frame gains PSNR and SSIM over the 4-sample frame it starts from, and clears
-    40 dB; the untrained network does not."""
+@pytest.mark.parametrize("samples", [4, 2, 1])
+def test_trained_network_beats_the_plain_few_sample_render(DN, tmp_path, samples):
+    """Each shipped parameter set on held-out denoiser scenes: against the 256-sample render the
+    denoised frame gains PSNR and SSIM over the few-sample frame it starts from (4, 2 or 1 light
+    samples per hit), and clears 40 dB; the untrained network does not."""
     from rtrace_amd import denoise
-    dn = DN.Denoiser.load()
-    rows = denoise.evaluate(dn, [200001, 200002], width=480, height=270, threshold=2.0, repeats=1,
+    dn = DN.Denoiser.load(denoise.WEIGHTS_BY_SAMPLES[samples])
+    rows = denoise.evaluate(dn, [200001, 200002], width=480, height=270, low=samples, threshold=2.0, repeats=1,
                             work_dir=str(tmp_path))
     for r in rows:
         print(r)
         assert r["psnr_denoised"] >= 40.0
         assert r["psnr_denoised"] >= r["psnr_low"] + 2.0
         assert r["ssim_denoised"] > r["ssim_low"]
-    raw = denoise.evaluate(DN.Denoiser(seed=3), [200001], width=480, height=270, threshold=2.0, repeats=1,
-                           work_dir=str(tmp_path))[0]
+    raw = denoise.evaluate(DN.Denoiser(seed=3), [200001], width=480, height=270, low=samples, threshold=2.0,
+                           repeats=1, work_dir=str(tmp_path))[0]
     assert raw["psnr_denoised"] < rows[0]["psnr_low"]

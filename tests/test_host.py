@@ -104,3 +104,20 @@ def test_batch_jobs_keep_scenes_and_outputs_alive():
     arr, keep = R._make_jobs([(scene, None, R.Renderer(8, 8), out)])
     assert any(k is scene for k in keep) and any(k is out for k in keep)
     assert arr[0].scene == 0x1000 and arr[0].d_rgb_out == 0x2000
+
+
+def test_shipped_denoiser_parameter_sets():
+    """One float32 parameter set per input sample count, each of the network's full size, and
+    distinct; the fallback threshold keeps its share of the variance estimate's maximum."""
+    import numpy as np
+    from oracle import denoise_ref as D
+    from rtrace_amd import denoise
+    sets = {n: np.load(p, allow_pickle=False) for n, p in denoise.WEIGHTS_BY_SAMPLES.items()}
+    assert sorted(sets) == [1, 2, 4]
+    for n, w in sets.items():
+        assert w.dtype == np.float32 and w.shape == (D.N_PARAMS,), n
+        assert np.isfinite(w).all()
+    assert not np.array_equal(sets[4], sets[2]) and not np.array_equal(sets[2], sets[1])
+    assert denoise.fallback_threshold(4) == 0.06
+    assert abs(denoise.fallback_threshold(2) - 0.12) < 1e-15
+    assert abs(denoise.fallback_threshold(1) - 0.24) < 1e-15

@@ -457,6 +457,7 @@ size_t frame_carve(FrameView& F, const SceneView& S, const FrameLayout& L, char*
     F.take = (int32_t*)take((3 * depth + 3) * 8 * 32 * sizeof(int32_t));
     F.shade = (ShadeRec*)take(cap * sizeof(ShadeRec));
     F.spt = (double*)take(cap * std::max(S.spt_per_hit, 1) * 3 * sizeof(double));
+    F.spt_stride = (long long)cap;
     F.vis = (uint8_t*)take(cap * L.K);
     F.skip = (uint32_t*)take(cap * sizeof(uint32_t));
     F.work = (unsigned long long*)take(2 * kWorkCount * sizeof(unsigned long long));

@@ -98,7 +98,9 @@ struct FrameView {
     int32_t* tie_count;   // [max_depth]
     int32_t* take;        // [max_depth][closest, shadow, resolve][8 slices][32] items handed out
     ShadeRec* shade;      // [queue capacity]
-    double* spt;          // [queue capacity * shadow_per_hit * 3] sample points
+    double* spt;          // stored sample points, sample-major: coordinate c of point s of hit hq at
+                          // spt[(3 s + c) * spt_stride + hq] (consecutive hits, consecutive lanes)
+    long long spt_stride; // the queue capacity
     uint8_t* vis;         // [queue capacity * shadow_per_hit]
     uint32_t* skip;       // [queue capacity]: bit l set when light l's queries are not traced
     unsigned long long* work;  // [2][kWorkCount]: closest, shadow
@@ -153,6 +155,20 @@ RT_HD void st3(double* p, d3 v) {
     p[1] = v.y;
     p[2] = v.z;
 }
+
+// The stored light points of one hit (FrameView::spt): point s at p[3 s * st], p[(3 s + 1) * st],
+// p[(3 s + 2) * st], so the lanes of a wavefront (consecutive hits) touch consecutive doubles.
+struct SptRef {
+    double* p;
+    long long st;
+    RT_HD void put(int s, d3 v) const {
+        p[3ll * s * st] = v.x;
+        p[(3ll * s + 1) * st] = v.y;
+        p[(3ll * s + 2) * st] = v.z;
+    }
+    RT_HD d3 get(int s) const { return mk(p[3ll * s * st], p[(3ll * s + 1) * st], p[(3ll * s + 2) * st]); }
+    RT_HD SptRef at(int s) const { return SptRef{p + 3ll * s * st, st}; }
+};
 
 // HitRecord::new face orientation (src/ray.rs:39-63)
 RT_HD d3 face(const d3& dir, const d3& outward) {
@@ -1262,7 +1278,7 @@ RT_HD void resolve_item(const SceneView& S, const FrameView& F, int slot, int le
 // consumed with static indices. A draw that lands on `high` (gen_range retries with a
 // smaller scale, probability ~2^-52) makes the fast path give up so the sequential reader
 // redoes the light.
-RT_HD bool disk_samples_fast(uint64_t seed, int count, double radius, d3 lp, d3 ub, d3 vb, double* out) {
+RT_HD bool disk_samples_fast(uint64_t seed, int count, double radius, d3 lp, d3 ub, d3 vb, SptRef out) {
     uint32_t key[8];
     stdrng_key(seed, key);
     const double low = -radius, high = radius, scl = high - low;
@@ -1278,7 +1294,7 @@ RT_HD bool disk_samples_fast(uint64_t seed, int count, double radius, d3 lp, d3 
             double y = (bits_to_f64((wy >> 12) | (1023ull << 52)) - 1.0) * scl + low;
             if (!(x < high) || !(y < high)) return false;
             if (n < count && x * x + y * y <= radius * radius) {
-                st3(out + 3 * n, add(add(lp, scale(x, ub)), scale(y, vb)));
+                out.put(n, add(add(lp, scale(x, ub)), scale(y, vb)));
                 ++n;
             }
         }
@@ -1286,7 +1302,7 @@ RT_HD bool disk_samples_fast(uint64_t seed, int count, double radius, d3 lp, d3 
     return true;
 }
 
-RT_COLD void disk_samples_slow(uint64_t seed, int count, double radius, d3 lp, d3 ub, d3 vb, double* out) {
+RT_COLD void disk_samples_slow(uint64_t seed, int count, double radius, d3 lp, d3 ub, d3 vb, SptRef out) {
     StdRngDev rng;
     rng.init(seed, 0);
     for (int s = 0; s < count; ++s) {
@@ -1296,7 +1312,7 @@ RT_COLD void disk_samples_slow(uint64_t seed, int count, double radius, d3 lp, d
             y = rng.gen_range(-radius, radius);
             if (x * x + y * y <= radius * radius) break;
         }
-        st3(out + 3 * s, add(add(lp, scale(x, ub)), scale(y, vb)));
+        out.put(s, add(add(lp, scale(x, ub)), scale(y, vb)));
     }
 }
 
@@ -1359,7 +1375,7 @@ RT_HD double diffuse_strength(d3 p, d3 n, d3 sp) { return fmax_r(dot(n, unit(sub
 // `half` of its centre, so n . (sample - p) lies in c +- e; the slack covers the
 // rounding of the samples and of the normalised dot product, and the samples themselves
 // are tested only when the interval straddles 0.
-RT_HD bool area_lit(const SceneView& S, const LightDev& L, const AreaFrame& a, d3 p, d3 n, const double* pts,
+RT_HD bool area_lit(const SceneView& S, const LightDev& L, const AreaFrame& a, d3 p, d3 n, SptRef pts,
                     int cnt) {
     const d3 lp = ld3(L.pos);
     const bool sq = L.shape == kLightSquare;
@@ -1373,7 +1389,7 @@ RT_HD bool area_lit(const SceneView& S, const LightDev& L, const AreaFrame& a, d
     if (c - e > slack && cnt > 0) return true;
     const int cols = sq ? light_cols(S, L) : 1;
     for (int q = 0; q < cnt; ++q) {
-        const d3 sp = sq ? square_sample(L, a, q, cnt, cols) : ld3(pts + 3 * q);
+        const d3 sp = sq ? square_sample(L, a, q, cnt, cols) : pts.get(q);
         if (diffuse_strength(p, n, sp) > 0.0) return true;
     }
     return false;
@@ -1494,7 +1510,7 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
     // and the lights whose shadow queries are not traced (light_lit; with features, such a
     // light's samples count as not visible)
     uint64_t seed = F.path_seed[rr.path];
-    double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
+    const SptRef spt{F.spt + hq, F.spt_stride};
     uint32_t skip = 0;
     int j = 0;
     for (int li = 0; li < S.n_lights; ++li) {
@@ -1502,7 +1518,7 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
         d3 lp = ld3(L.pos);
         const bool cull = li < 32;
         if (!L.is_area) {
-            st3(spt + 3 * j, lp);
+            spt.put(j, lp);
             ++j;
             if (cull && !(diffuse_strength(p, n, lp) > 0.0)) skip |= 1u << li;
             continue;
@@ -1511,10 +1527,10 @@ RT_HD bool shade_prepare_item(const SceneView& S, const FrameView& F, int hq, in
         AreaFrame a = area_frame(seed, p, lp);
         if (L.shape != kLightSquare) {  // squares are regenerated where needed
             double radius = L.diameter / 2.0;
-            if (!disk_samples_fast(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j))
-                disk_samples_slow(a.seed, cnt, radius, lp, a.ub, a.vb, spt + 3 * j);
+            if (!disk_samples_fast(a.seed, cnt, radius, lp, a.ub, a.vb, spt.at(j)))
+                disk_samples_slow(a.seed, cnt, radius, lp, a.ub, a.vb, spt.at(j));
         }
-        if (cull && !area_lit(S, L, a, p, n, spt + 3 * j, cnt)) skip |= 1u << li;
+        if (cull && !area_lit(S, L, a, p, n, spt.at(j), cnt)) skip |= 1u << li;
         if (L.shape != kLightSquare) j += cnt;
     }
     F.skip[hq] = skip;
@@ -1542,7 +1558,7 @@ RT_HD void shadow_ray(const SceneView& S, const FrameView& F, int task, RayD& r,
     d3 p = ld3(sr.p), n = ld3(sr.n);
     d3 sp;
     if (!S.has_square) {
-        sp = ld3(F.spt + (size_t)task * 3);  // slots == stored points
+        sp = SptRef{F.spt + hq, F.spt_stride}.get(task - hq * S.shadow_per_hit);  // slots == stored points
     } else {
         int j = task - hq * S.shadow_per_hit, k = 0;
         for (int li = 0;; ++li) {
@@ -1552,7 +1568,7 @@ RT_HD void shadow_ray(const SceneView& S, const FrameView& F, int task, RayD& r,
                 if (L.shape == kLightSquare)
                     sp = square_sample(L, area_frame(F.path_seed[sr.path], p, ld3(L.pos)), j, cnt, light_cols(S, L));
                 else
-                    sp = ld3(F.spt + ((size_t)hq * S.spt_per_hit + k + j) * 3);
+                    sp = SptRef{F.spt + hq, F.spt_stride}.get(k + j);
                 break;
             }
             j -= cnt;
@@ -1625,7 +1641,7 @@ RT_HD d3 area_slot_term(const SceneView& S, const FrameView& F, int hq, int j) {
                 const AreaFrame a = area_frame(F.path_seed[sr.path], p, ld3(L.pos));
                 sp = square_sample(L, a, smp, cnt, light_cols(S, L));
             } else {
-                sp = ld3(F.spt + ((size_t)hq * S.spt_per_hit + k + smp) * 3);
+                sp = SptRef{F.spt + hq, F.spt_stride}.get(k + smp);
             }
             return phong_sample(m, L, lc, mc, p, n, vd, sp);
         }
@@ -1646,7 +1662,7 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq, con
     d3 color = scale(m.ambient * S.amb_i, hadamard(ld3(S.amb), mc));
     d3 cam = ld3(S.cam_pos);
     const d3 vd = unit(sub(cam, p));
-    const double* spt = F.spt + (size_t)hq * S.spt_per_hit * 3;
+    const SptRef spt{F.spt + hq, F.spt_stride};
     const uint8_t* vis = F.vis + (size_t)hq * S.shadow_per_hit;
     uint64_t path_seed = 0;
     if (S.has_square) path_seed = F.path_seed[sr.path];
@@ -1689,7 +1705,7 @@ RT_HD void shade_finish_item(const SceneView& S, const FrameView& F, int hq, con
             for (int s = 0; s < cnt; ++s) {
                 if (!vis[j + s]) continue;
                 ++visible;
-                total = add(total, pre ? ld3(pre + 3 * (j + s)) : phong_sample(m, L, lc, mc, p, n, vd, ld3(spt + 3 * (k + s))));
+                total = add(total, pre ? ld3(pre + 3 * (j + s)) : phong_sample(m, L, lc, mc, p, n, vd, spt.get(k + s)));
             }
             k += cnt;
         }

@@ -422,7 +422,7 @@ def measure_frame(R, torch, workload, steps, warmup, stage_events=True, count=Tr
     return res, path, cfg
 
 
-def bench_batch(args):
+def bench_batch(args, steps=None, warmup=None, emit=True):
     """Batch workload: 64 seeded scenes x 4 auto-camera views at their own resolutions,
     rendered as one batch (persistent kernels, one captured graph) with autotuned tile,
     chunk and occupancy. A step is one replay of the whole batch. Also times the same jobs
@@ -452,16 +452,18 @@ def bench_batch(args):
     opts, tune_ms = R.autotune_batch(tune_jobs, opts0)
     _progress(f"autotuned on {len(tune_jobs)} jobs: {opts.as_dict()}")
     b = R.Batch(jobs, opts)
-    for _ in range(args.warmup):
+    steps = args.steps if steps is None else steps
+    warmup = args.warmup if warmup is None else warmup
+    for _ in range(warmup):
         b.run()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms_dev = []
-    for _ in range(args.steps):
+    for _ in range(steps):
         st = b.run()
         ms_dev.append(st.ms)
     torch.cuda.synchronize()
-    ms_step = (time.perf_counter() - t1) * 1e3 / args.steps
+    ms_step = (time.perf_counter() - t1) * 1e3 / steps
     _progress(f"batch: {ms_step:.1f} ms per step")
     # the same frames one at a time
     seq_ms = None
@@ -487,7 +489,7 @@ def bench_batch(args):
     rays = st.rays_total
     line = {
         "metric": "Mrays/s, batched (64 scenes x 4 views)", "value": round(rays / (ms_step * 1e-3) / 1e6, 2), "unit": "Mrays/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "n_gpus": 1, "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+f64",
         "data": "synthetic (seeded procedural scenes)",
         "config": {"workload": f"batch64: {len(work)} scenes x 4 auto-camera views, quincunx, depth 4, disk light "
@@ -503,7 +505,10 @@ def bench_batch(args):
     if seq_ms is not None:
         line["sequential"] = {"ms": round(seq_ms, 3), "mrays_s": round(rays / (seq_ms * 1e-3) / 1e6, 2),
                               "batch_speedup": round(seq_ms / ms_step, 3)}
-    print(json.dumps(line))
+    del b, jobs
+    if emit:
+        print(json.dumps(line))
+    return line
 
 
 SUB_CONFIGS = [("config 1", "sphere_800x600", False), ("config 2 as stated", "icosphere_square_720p", True),
@@ -527,6 +532,7 @@ def main():
     ap.add_argument("--dn-threshold", type=float, default=0.06, help="tile fallback threshold")
     ap.add_argument("--c4-scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--no-sequential", action="store_true", help="batch64: skip the one-by-one comparison")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batch64 entry of the default run's configs")
     ap.add_argument("--c4-group-gib", type=float, default=None,
                     help="batch64: GiB of frame buffers per job group (default: the library's)")
     args = ap.parse_args()
@@ -603,6 +609,17 @@ def main():
             res["config"] = label
             subs[wl] = res
             _progress(f"{wl}: {res['ms_per_step']} ms/frame, {res['mrays_per_s']} Mrays/s")
+        if not args.no_batch:
+            # config 4 in the same run: the 64-scene batch, a few steps (its own `--workload batch64`
+            # run times more)
+            bl = bench_batch(args, steps=3, warmup=1, emit=False)
+            subs["batch64"] = {"config": "config 4", "workload": bl["config"]["workload"], "mrays_per_s": bl["value"],
+                               "ms_per_step": bl["ms_per_step"], "steps": bl["steps"], "warmup": bl["warmup"],
+                               "jobs": bl["config"]["jobs"], "rays_per_step": bl["config"]["rays_per_step"],
+                               "frame_buffer_gib": bl["batch"]["frame_buffer_gib"],
+                               "job_groups": bl["batch"]["job_groups"], "opts": bl["batch"]["opts"],
+                               "sequential": bl.get("sequential")}
+            _progress(f"batch64: {bl['ms_per_step']} ms/step, {bl['value']} Mrays/s")
         line["configs"] = subs
     if not args.no_traffic:
         pmc, err = run_counters(args.workload)

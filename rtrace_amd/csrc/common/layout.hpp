@@ -117,6 +117,8 @@ struct ObjDev {
     float rf;            // sphere radius (f32)
     float bmin[3];       // cube: world box of its geometry, rounded outward (prefilter)
     float bmax[3];
+    float pc[3];         // plane point (f32) for the prefilter
+    float pn[3];         // plane normal (f32) for the prefilter
 };
 
 struct MatDev {

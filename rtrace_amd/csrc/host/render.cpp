@@ -194,6 +194,8 @@ void build_device_scene(rt_ctx* ctx, SceneDevice& D) {
             d.n[k] = o.normal[k];
             d.half[k] = o.half_size[k];
             d.cf[k] = (float)o.center[k];
+            d.pc[k] = (float)d.c[k];
+            d.pn[k] = (float)d.n[k];
         }
         d.r = o.radius;
         d.rf = round_up(o.radius);

@@ -290,4 +290,32 @@ RT_HD bool sphere_may_hit(const ObjDev& s, const RayD& r, const RayF& f, double 
     return true;
 }
 
+// Conservative FP32 prefilter of plane_exact: false only when Plane::hit rejects for certain,
+// i.e. |n.d| < 1e-8, or t = n.(c - o) / n.d lies below tmin (0 <= tmin: numerator and
+// denominator of opposite signs) or above tmax for every value the f32 inputs allow. The f32
+// copies of the point, normal, origin and direction carry 2^-24 of their magnitudes, and three
+// products and two sums add a few more rounding units: 1e-6 of the magnitudes (about 16 units)
+// bounds the error of the numerator and the denominator.
+RT_HD bool plane_may_hit(const ObjDev& p, const RayF& f, double tmin, double tmax) {
+    const float nx = p.pn[0], ny = p.pn[1], nz = p.pn[2];
+    const float den = nx * f.dx + ny * f.dy + nz * f.dz;
+    const float eden = 1e-6f * (fabsf(nx * f.dx) + fabsf(ny * f.dy) + fabsf(nz * f.dz)) + 1e-30f;
+    if (fabsf(den) + eden < 1e-8f) return false;  // 1e-8f is below the f64 1e-8
+    const float vx = p.pc[0] - f.ox, vy = p.pc[1] - f.oy, vz = p.pc[2] - f.oz;
+    const float num = nx * vx + ny * vy + nz * vz;
+    const float enm = 1e-6f * (fabsf(nx) * (fabsf(p.pc[0]) + fabsf(f.ox)) + fabsf(ny) * (fabsf(p.pc[1]) + fabsf(f.oy)) +
+                               fabsf(nz) * (fabsf(p.pc[2]) + fabsf(f.oz))) +
+                      1e-30f;
+    if (tmin >= 0.0) {
+        if (num + enm < 0.0f && den - eden > 0.0f) return false;
+        if (num - enm > 0.0f && den + eden < 0.0f) return false;
+    }
+    if (tmax < 1e30) {
+        const bool same = (num - enm > 0.0f && den - eden > 0.0f) || (num + enm < 0.0f && den + eden < 0.0f);
+        const float lo = fabsf(num) - enm, hi = (fabsf(den) + eden) * (float)tmax * (1.0f + 1e-6f);
+        if (same && lo > hi) return false;
+    }
+    return true;
+}
+
 }  // namespace rt

@@ -914,7 +914,7 @@ RT_HD bool world_closest_bvh(const SceneView& S, const RayD& r, double tmin, dou
         double t;
         if (ob.kind == kPlane) {
             if (count) w.v[kWorkPlane]++;
-            if (plane_exact(ob, r, tmin, INFINITY, t)) cs.offer(t, i, -1, true);
+            if (plane_may_hit(ob, f, tmin, INFINITY) && plane_exact(ob, r, tmin, INFINITY, t)) cs.offer(t, i, -1, true);
         } else {
             int32_t tri = -1;
             double best = cs.best;
@@ -950,7 +950,7 @@ RT_HD bool world_closest(const SceneView& S, const RayD& r, double tmin, double 
             }
         } else if (ob.kind == kPlane) {
             if (count) w.v[kWorkPlane]++;
-            if (plane_exact(ob, r, tmin, closest, t)) {
+            if (plane_may_hit(ob, f, tmin, closest) && plane_exact(ob, r, tmin, closest, t)) {
                 closest = t;
                 bo = i;
                 bt = -1;
@@ -1010,7 +1010,7 @@ RT_HD bool world_any_analytic(const SceneView& S, const RayD& r, double tmin, do
             double t;
             if (ob.kind == kPlane) {
                 if (count) w.v[kWorkPlane]++;
-                if (plane_exact(ob, r, tmin, tmax, t)) return true;
+                if (plane_may_hit(ob, f, tmin, tmax) && plane_exact(ob, r, tmin, tmax, t)) return true;
             } else if (ob.kind == kCube) {
                 d3 p, n;
                 bool may = cube_may_hit(wq, ob);
@@ -1030,7 +1030,7 @@ RT_HD bool world_any_analytic(const SceneView& S, const RayD& r, double tmin, do
             if (sphere_may_hit(ob, r, f, tmin, tmax) && sphere_exact(ob, r, tmin, tmax, t)) return true;
         } else if (ob.kind == kPlane) {
             if (count) w.v[kWorkPlane]++;
-            if (plane_exact(ob, r, tmin, tmax, t)) return true;
+            if (plane_may_hit(ob, f, tmin, tmax) && plane_exact(ob, r, tmin, tmax, t)) return true;
         } else if (ob.kind == kCube) {
             d3 p, n;
             bool may = cube_may_hit(wq, ob);

@@ -84,3 +84,19 @@ def test_shadow_task_order_is_a_permutation(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.strip() == "ok 48"
+
+
+@pytest.mark.skipif(shutil.which("g++") is None or not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"),
+                    reason="needs g++ and the HIP headers")
+def test_plane_prefilter_is_conservative(tmp_path):
+    """plane_may_hit (the FP32 prefilter before the f64 plane test, compiled for the host)
+    never rejects a ray the exact test accepts: random planes and rays up to 1e5 units out,
+    rays nearly parallel to the plane around the 1e-8 threshold, and rays whose t sits ulps
+    from tmin or tmax; and it settles at least 90 % of the shadow rays that leave a ground
+    plane lit from above."""
+    exe = str(tmp_path / "plane_check")
+    subprocess.run(["g++", "-std=c++17", "-O2", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+                    "-o", exe, os.path.join(ROOT, "tests", "cpp", "plane_check.cpp")], check=True)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok ")

@@ -203,12 +203,6 @@ void forward(rt_denoiser* d, const float* feat, int H, int W) {
     DN_TRY(dn::launch_prep(feat, d->B, H, W, d->Hp, d->Wp, d->x0.as<uint16_t>(), d->st));
     for (int l = 0; l < RT_DN_LAYERS; ++l) {
         const LayerDef& L = kLayers[l];
-        if (l == 2) DN_TRY(dn::launch_pool(d->act[1].as<uint16_t>(), d->B, level_h(d, 0), level_w(d, 0), 32,
-                                           d->pool1.as<uint16_t>(), d->st));
-        if (l == 4) DN_TRY(dn::launch_pool(d->act[3].as<uint16_t>(), d->B, level_h(d, 1), level_w(d, 1), 64,
-                                           d->pool3.as<uint16_t>(), d->st));
-        if (l == 6) DN_TRY(dn::launch_pool(d->act[5].as<uint16_t>(), d->B, level_h(d, 2), level_w(d, 2), 128,
-                                           d->pool5.as<uint16_t>(), d->st));
         dn::ConvArgs a{};
         a.in = conv_input(d, l);
         a.wt = d->wf[l].as<uint16_t>();
@@ -222,6 +216,11 @@ void forward(rt_denoiser* d, const float* feat, int H, int W) {
         a.B = d->B;
         a.H = level_h(d, L.level);
         a.W = level_w(d, L.level);
+        // the encoders' second convolutions also write their 2x2 average pools (the next
+        // level's input) from the outputs they hold
+        if (l == 1) a.pool_out = d->pool1.as<uint16_t>();
+        if (l == 3) a.pool_out = d->pool3.as<uint16_t>();
+        if (l == 5) a.pool_out = d->pool5.as<uint16_t>();
         DN_TRY(dn::launch_conv(a, d->st));
     }
 }

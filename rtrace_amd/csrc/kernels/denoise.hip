@@ -33,6 +33,12 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 constexpr int kTH = 8, kTW = 16;                   // output tile (pixels)
+
+// 2x2 average pooling of four bf16 outputs (top-left, top-right, bottom-left, bottom-right):
+// the two row sums added, times 1/4, rounded to bf16.
+__device__ __forceinline__ uint16_t pool4(uint16_t a, uint16_t b, uint16_t c, uint16_t d) {
+    return f2bf(((bf2f(a) + bf2f(b)) + (bf2f(c) + bf2f(d))) * 0.25f);
+}
 constexpr int kHH = kTH + 2, kHW = kTW + 2;        // input halo
 constexpr int kCK = 32;                            // input channels per K chunk
 constexpr int kPS = 40;                            // LDS pixel stride in bf16 (80 B)
@@ -178,6 +184,29 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
                 } else {
                     if (a.relu) v = fmaxf(v, 0.0f);
                     a.out_bf16[o] = f2bf(v);
+                }
+            }
+        }
+    }
+    if (a.pool_out) {
+        // this lane holds output rows y, y + 1 (y even) and columns x .. x + 3 (x even) of
+        // channel co: two pooled pixels
+        const int y = y0 + wave * 2;
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int co = co0 + nt * 16 + r16;
+            const float bias = a.bias && co < a.cout ? a.bias[co] : 0.0f;
+#pragma unroll
+            for (int p = 0; p < 2; ++p) {
+                const int x = x0 + 4 * g + 2 * p;
+                if (y + 1 < a.H && x + 1 < a.W && co < a.cout) {
+                    auto q = [&](int mt, int i) {
+                        float v = acc[mt][nt][i] + bias;
+                        if (a.relu) v = fmaxf(v, 0.0f);
+                        return f2bf(v);
+                    };
+                    const size_t o = (((size_t)b * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.out_pitch + co;
+                    a.pool_out[o] = pool4(q(0, 2 * p), q(0, 2 * p + 1), q(1, 2 * p), q(1, 2 * p + 1));
                 }
             }
         }
@@ -368,6 +397,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         }
                     }
                 }
+                if (a.pool_out) {
+                    // rows y, y + 1 (y even), columns x .. x + 3 (x even) of channel co: two pooled pixels
+                    const int y = y0 + wave * 2;
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) {
+                        const f32x4 top = nt == 0 ? acc00 : acc01, bot = nt == 0 ? acc10 : acc11;
+                        const int co = g.co0 + nt * 16 + r16;
+                        const float bias = a.bias && co < a.cout ? a.bias[co] : 0.0f;
+                        auto q = [&](float s) {
+                            float v = s + bias;
+                            if (a.relu) v = fmaxf(v, 0.0f);
+                            return f2bf(v);
+                        };
+#pragma unroll
+                        for (int p = 0; p < 2; ++p) {
+                            const int x = x0 + 4 * gq + 2 * p;
+                            if (y + 1 < a.H && x + 1 < a.W && co < a.cout) {
+                                const size_t o =
+                                    (((size_t)b * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.out_pitch + co;
+                                a.pool_out[o] = pool4(q(top[2 * p]), q(top[2 * p + 1]), q(bot[2 * p]), q(bot[2 * p + 1]));
+                            }
+                        }
+                    }
+                }
                 acc00 = acc01 = acc10 = acc11 = f32x4{0.f, 0.f, 0.f, 0.f};
             }
         }
@@ -504,33 +557,6 @@ __global__ void k_prep(const float* feat, int B, int H, int W, int Hp, int Wp, u
 }
 
 // 2x2 average pooling, 8 channels (one 16-byte vector) of one output pixel per thread.
-__global__ void k_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out) {
-    const int h2 = H >> 1, w2 = W >> 1, cv = C / 8;
-    const long long n = (long long)B * h2 * w2 * cv;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const int c = (int)(i % cv) * 8;
-        const long long p = i / cv;
-        const int x = (int)(p % w2);
-        const int y = (int)((p / w2) % h2);
-        const int b = (int)(p / ((long long)w2 * h2));
-        const uint16_t* q = in + (((size_t)b * H + 2 * y) * W + 2 * x) * C + c;
-        const uint4 a = *reinterpret_cast<const uint4*>(q), bq = *reinterpret_cast<const uint4*>(q + C);
-        const uint4 cq = *reinterpret_cast<const uint4*>(q + (size_t)W * C);
-        const uint4 d = *reinterpret_cast<const uint4*>(q + (size_t)W * C + C);
-        const uint16_t* ha = reinterpret_cast<const uint16_t*>(&a);
-        const uint16_t* hb = reinterpret_cast<const uint16_t*>(&bq);
-        const uint16_t* hc = reinterpret_cast<const uint16_t*>(&cq);
-        const uint16_t* hd = reinterpret_cast<const uint16_t*>(&d);
-        uint32_t r[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const float s0 = (bf2f(ha[2 * k]) + bf2f(hb[2 * k])) + (bf2f(hc[2 * k]) + bf2f(hd[2 * k]));
-            const float s1 = (bf2f(ha[2 * k + 1]) + bf2f(hb[2 * k + 1])) + (bf2f(hc[2 * k + 1]) + bf2f(hd[2 * k + 1]));
-            r[k] = (uint32_t)f2bf(s0 * 0.25f) | ((uint32_t)f2bf(s1 * 0.25f) << 16);
-        }
-        *reinterpret_cast<uint4*>(out + p * C + c) = make_uint4(r[0], r[1], r[2], r[3]);
-    }
-}
 __global__ void k_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf) {
     const long long n = (long long)B * H * W * C;
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -739,11 +765,6 @@ hipError_t launch_reduce(const float* part, int splits, long long n, float* out,
 hipError_t launch_prep(const float* feat, int B, int H, int W, int Hp, int Wp, uint16_t* x0, hipStream_t st) {
     hipLaunchKernelGGL(k_prep, dim3(grid1((long long)B * Hp * Wp, 256, 8192)), dim3(256), 0, st, feat, B, H, W, Hp, Wp,
                        x0);
-    return hipGetLastError();
-}
-hipError_t launch_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out, hipStream_t st) {
-    hipLaunchKernelGGL(k_pool, dim3(grid1((long long)B * (H / 2) * (W / 2) * (C / 8), 256, 8192)), dim3(256), 0, st, in,
-                       B, H, W, C, out);
     return hipGetLastError();
 }
 hipError_t launch_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf, hipStream_t st) {

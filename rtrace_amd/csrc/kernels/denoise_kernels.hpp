@@ -31,6 +31,8 @@ struct ConvArgs {
     float* out_f32;      // ... or f32 NHWC output (+bias), when set
     int relu;
     int B, H, W;  // output resolution
+    uint16_t* pool_out;  // optional: bf16 [B][H/2][W/2][out_pitch], the 2x2 average of the bf16 outputs
+                         // of each 2x2 block; H and W even, bf16 output only
 };
 
 struct WgradArgs {
@@ -46,7 +48,6 @@ hipError_t launch_wgrad(const WgradArgs& a, int splits, hipStream_t st);
 hipError_t launch_bias_grad(const uint16_t* dz, long long n_px, int c, float* part, int splits, hipStream_t st);
 hipError_t launch_reduce(const float* part, int splits, long long n, float* out, bool accumulate, hipStream_t st);
 hipError_t launch_prep(const float* feat, int B, int H, int W, int Hp, int Wp, uint16_t* x0, hipStream_t st);
-hipError_t launch_pool(const uint16_t* in, int B, int H, int W, int C, uint16_t* out, hipStream_t st);
 hipError_t launch_pool_bwd(const float* gh, int B, int H, int W, int C, float* gf, hipStream_t st);
 hipError_t launch_split(const float* dx, int B, int H, int W, int c0, int c1, float* g_low, float* g_skip,
                         hipStream_t st);

@@ -449,7 +449,16 @@ def bench_batch(args, steps=None, warmup=None, emit=True):
     opts0 = R.default_batch_opts()
     if args.c4_group_gib is not None:
         opts0.group_gib = args.c4_group_gib
-    opts, tune_ms = R.autotune_batch(tune_jobs, opts0)
+    if args.c4_concurrent is not None:
+        opts0.concurrent = args.c4_concurrent
+    # the subset is tuned with job groups scaled to its share of the jobs, so that it runs about as
+    # many groups, as many of them in flight, as the whole batch
+    tune_gib = opts0.group_gib * len(tune_jobs) / len(jobs)
+    tune0 = R.BatchOpts.from_buffer_copy(opts0)
+    tune0.group_gib = tune_gib
+    tuned, tune_ms = R.autotune_batch(tune_jobs, tune0)
+    opts = R.BatchOpts.from_buffer_copy(tuned)
+    opts.group_gib = opts0.group_gib
     _progress(f"autotuned on {len(tune_jobs)} jobs: {opts.as_dict()}")
     b = R.Batch(jobs, opts)
     steps = args.steps if steps is None else steps
@@ -497,8 +506,10 @@ def bench_batch(args, steps=None, warmup=None, emit=True):
                    "triangles_min": min(tris), "triangles_max": max(tris), "rays_per_step": rays,
                    "primary": st.rays_primary, "reflect": st.rays_reflect, "shadow": st.rays_shadow},
         "batch": {"opts": opts.as_dict(), "tune_ms_subset": round(tune_ms, 3), "tune_jobs": len(tune_jobs),
+                  "tune_group_gib": round(tune_gib, 4),
                   "graph_nodes": st.graph_nodes, "graph_launches_per_step": 1, "device_ms": round(min(ms_dev), 3),
                   "frame_buffer_gib": round(st.bytes / 2**30, 2), "job_groups": st.groups,
+                  "groups_in_flight": opts.concurrent,
                   "build_ms": round(st.ms_build, 1),
                   "scene_load_s": round(t_load, 1)},
     }
@@ -535,6 +546,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batch64 entry of the default run's configs")
     ap.add_argument("--c4-group-gib", type=float, default=None,
                     help="batch64: GiB of frame buffers per job group (default: the library's)")
+    ap.add_argument("--c4-concurrent", type=int, default=None,
+                    help="batch64: job groups in flight at once (default: the library's)")
     args = ap.parse_args()
     if args.counters_child:
         counters_child(args.workload)

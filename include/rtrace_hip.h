@@ -329,6 +329,9 @@ typedef struct {
     double max_gib;         /* refuse batches whose buffers need more device memory (GiB) */
     double group_gib;       /* jobs run in consecutive groups whose frame buffers fit this size (GiB),
                                all groups in the same launch, sharing one allocation; 0 = one group */
+    int32_t concurrent;     /* groups in flight at once (1-8), each on its own stream branch with
+                               its own frame buffers: a group's stage tails overlap another's work;
+                               the device holds this many groups' buffers */
 } rt_batch_opts;
 
 typedef struct {
@@ -336,8 +339,8 @@ typedef struct {
     double ms_build;        /* host time of create: layout, allocation, upload, capture */
     int32_t n_jobs;
     int32_t graph_nodes;    /* kernels in the captured graph (0 without a graph) */
-    int32_t groups;         /* job groups run one after another (group_gib) */
-    int64_t bytes;          /* device memory of the frame buffers (the largest group's) */
+    int32_t groups;         /* job groups (group_gib), `concurrent` of them in flight at once */
+    int64_t bytes;          /* device memory of the frame buffers (concurrent x the largest group's) */
     uint64_t pixels;
     uint64_t rays_primary, rays_reflect, rays_shadow;
 } rt_batch_stats;

@@ -79,9 +79,10 @@ class _BatchJob(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     """rt_batch_opts: raygen tile, chunk (items per take), persistent workgroups per CU, graph, memory cap,
-    job-group size (GiB of frame buffers per group; groups share one allocation)."""
+    job-group size (GiB of frame buffers per group), groups in flight at once (each with its own buffers)."""
     _fields_ = [("tile", ctypes.c_int32), ("chunk", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("max_gib", ctypes.c_double), ("group_gib", ctypes.c_double)]
+                ("use_graph", ctypes.c_int32), ("max_gib", ctypes.c_double), ("group_gib", ctypes.c_double),
+                ("concurrent", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

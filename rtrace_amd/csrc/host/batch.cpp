@@ -21,6 +21,14 @@ struct rt_batch {
     rt::DevBuf arena, views, frames, plan, tally;
     std::vector<int> group_start;        // jobs [group_start[g], group_start[g + 1]) form group g
     std::vector<int> group_depth;        // deepest job of each group
+    // Each group runs on one of bo.concurrent branches (group_branch); a branch runs its groups
+    // one after another on its stream (branch 0: the context's stream), in its own part of the
+    // arena, with its own plan and chunk counter; the others fork from and join the context's
+    // stream by events.
+    std::vector<int> group_branch;
+    std::vector<hipStream_t> side;       // streams of branches 1..concurrent-1
+    std::vector<hipEvent_t> fork, join;  // one each per side stream
+    size_t plan_stride = 0;              // int32 entries between the branches' plan buffers
     std::vector<rt::FrameView> hframes;  // [2 * n_jobs]
     std::vector<rt::SceneView> hviews;
     std::vector<rt::FrameLayout> layouts;
@@ -34,6 +42,9 @@ struct rt_batch {
     ~rt_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : fork) (void)hipEventDestroy(e);
+        for (hipEvent_t e : join) (void)hipEventDestroy(e);
     }
 };
 
@@ -52,23 +63,39 @@ void check_opts(const rt_batch_opts& bo) {
         throw Error(RT_E_INVALID_ARG, "chunk must be a positive multiple of 64");
     if (bo.blocks_per_cu < 1 || bo.blocks_per_cu > 64) throw Error(RT_E_INVALID_ARG, "blocks_per_cu out of range");
     if (!(bo.group_gib >= 0.0)) throw Error(RT_E_INVALID_ARG, "group_gib must be >= 0");
+    if (bo.concurrent < 1 || bo.concurrent > 8) throw Error(RT_E_INVALID_ARG, "concurrent must be 1 to 8");
 }
 
-// Every group's launch sequence, one after another (the groups share the frame buffers).
+// Every group's launch sequence on its branch, the groups of a branch one after another (they
+// share that branch's frame buffers).
 hipError_t launch_groups(rt_batch* b, hipStream_t st) {
-    for (size_t g = 0; g + 1 < b->group_start.size(); ++g) {
+    const size_t C = b->side.size() + 1;
+    hipError_t e = hipSuccess;
+    for (size_t c = 1; c < C && e == hipSuccess; ++c) {
+        e = hipEventRecord(b->fork[c - 1], st);
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->side[c - 1], b->fork[c - 1], 0);
+    }
+    for (size_t g = 0; g + 1 < b->group_start.size() && e == hipSuccess; ++g) {
+        const size_t c = (size_t)b->group_branch[g];
         const int j0 = b->group_start[g], j1 = b->group_start[g + 1];
         BatchArgs a = b->args;
         a.views += j0;
         a.frames += 2 * (size_t)j0;
         a.tally += 2 * (size_t)j0;
+        a.base += c * b->plan_stride;
+        a.counter = a.base + (j1 - j0) + 1;
         a.n_jobs = j1 - j0;
         BatchLaunch l = b->launch;
         l.max_depth = b->group_depth[g];
-        const hipError_t e = launch_batch(a, l, st);
-        if (e != hipSuccess) return e;
+        e = launch_batch(a, l, c == 0 ? st : b->side[c - 1]);
     }
-    return hipSuccess;
+    // the side branches join the context's stream (also after an error, so a capture ends whole)
+    for (size_t c = 1; c < C; ++c) {
+        hipError_t e2 = hipEventRecord(b->join[c - 1], b->side[c - 1]);
+        if (e2 == hipSuccess) e2 = hipStreamWaitEvent(st, b->join[c - 1], 0);
+        if (e == hipSuccess) e = e2;
+    }
+    return e;
 }
 
 std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, const rt_batch_opts& bo) {
@@ -113,7 +140,8 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
     const double budget = bo.group_gib > 0.0 ? bo.group_gib * 1073741824.0 : 1e300;
     std::vector<int> order(n);  // slot -> job: slots follow the job order
     std::vector<size_t> offs(n);
-    size_t arena = 0, in_group = 0;
+    std::vector<int> group_of(n);
+    size_t slot = 0, in_group = 0;  // slot: the largest group's buffers, 256-byte aligned
     b->group_start.assign(1, 0);
     b->group_depth.assign(1, 0);
     for (int j = 0; j < n; ++j) {
@@ -123,19 +151,45 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
             b->group_depth.push_back(0);
             in_group = 0;
         }
+        group_of[j] = (int)b->group_start.size() - 1;
         offs[j] = in_group;
         in_group += need[j];
-        arena = std::max(arena, in_group);
+        slot = std::max(slot, (in_group + 255) & ~(size_t)255);
         b->group_depth.back() = std::max(b->group_depth.back(), b->layouts[j].depth);
     }
     b->group_start.push_back(n);
+    const int n_groups = (int)b->group_start.size() - 1;
+    const int C = std::min(bo.concurrent, n_groups);  // branches with a group to run
+    // each group, in order, to the branch with the least frame-buffer bytes so far (the bytes
+    // follow the rays: a proxy for the work), so the branches end close together
+    std::vector<double> load(C, 0.0);
+    b->group_branch.assign(n_groups, 0);
+    for (int g = 0; g < n_groups; ++g) {
+        double bytes = 0;
+        for (int j = b->group_start[g]; j < b->group_start[g + 1]; ++j) bytes += (double)need[j];
+        const int c = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->group_branch[g] = c;
+        load[c] += bytes;
+    }
+    const size_t arena = slot * (size_t)C;
     if ((double)arena > bo.max_gib * 1073741824.0)
         throw Error(RT_E_OOM, "batch needs " + std::to_string(arena >> 20) + " MiB of frame buffers, over max_gib");
     b->arena.reserve(arena);
+    for (int c = 1; c < C; ++c) {
+        hipStream_t s = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        b->side.push_back(s);
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->fork.push_back(e);
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->join.push_back(e);
+    }
     for (int j = 0; j < n; ++j) {
         FrameView& F = b->hframes[2 * j];
         RayRec *q0 = nullptr, *q1 = nullptr;
-        frame_carve(F, b->hviews[j], b->layouts[j], b->arena.as<char>() + offs[j], &q0, &q1, false);
+        const size_t base = slot * (size_t)b->group_branch[group_of[j]];
+        frame_carve(F, b->hviews[j], b->layouts[j], b->arena.as<char>() + base + offs[j], &q0, &q1, false);
         F.rgb = static_cast<uint8_t*>(jobs[j].d_rgb_out);
         F.rays = q0;
         F.rays_next = q1;
@@ -155,7 +209,8 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
     hipStream_t st = ctx_stream(ctx);
     b->views.reserve(n * sizeof(SceneView));
     b->frames.reserve(2 * (size_t)n * sizeof(FrameView));
-    b->plan.reserve((n + 2) * sizeof(int32_t));
+    b->plan_stride = ((size_t)n + 2 + 31) & ~(size_t)31;  // each branch's counter on its own 128-byte line
+    b->plan.reserve(b->plan_stride * (size_t)C * sizeof(int32_t));
     b->tally.reserve(2 * (size_t)n * sizeof(unsigned long long));
     HIP_TRY(hipMemcpyAsync(b->views.p, dviews.data(), n * sizeof(SceneView), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(b->frames.p, dframes.data(), 2 * (size_t)n * sizeof(FrameView), hipMemcpyHostToDevice, st));
@@ -249,7 +304,8 @@ void rt_batch_opts_default(rt_batch_opts* bo) {
     bo->blocks_per_cu = 16;
     bo->use_graph = 1;
     bo->max_gib = 200.0;
-    bo->group_gib = 32.0;
+    bo->group_gib = 8.0;
+    bo->concurrent = 4;
 }
 
 int rt_batch_create(rt_ctx* ctx, const rt_batch_job* jobs, int32_t n_jobs, const rt_batch_opts* bo, rt_batch** out) {

@@ -74,14 +74,17 @@ def _with_camera(R, scene, cam):
     return _cam_scenes[key]
 
 
-@pytest.mark.parametrize("use_graph,tile,group_gib", [(1, 8, 0.0), (0, 8, 0.0), (1, 4, 0.0), (1, 16, 0.0),
-                                                     (1, 8, 0.004), (0, 8, 1e-9)])
-def test_batch_equals_single_frames(R, workload, use_graph, tile, group_gib):
-    """group_gib > 0 runs the jobs in consecutive groups sharing one set of frame buffers
-    (1e-9: every job its own group); the images and ray counts stay the same."""
+@pytest.mark.parametrize("use_graph,tile,group_gib,concurrent", [
+    (1, 8, 0.0, 1), (0, 8, 0.0, 1), (1, 4, 0.0, 1), (1, 16, 0.0, 1), (1, 8, 0.004, 1), (0, 8, 1e-9, 1),
+    (1, 8, 0.004, 3), (0, 8, 1e-9, 2), (1, 8, 1e-9, 4)])
+def test_batch_equals_single_frames(R, workload, use_graph, tile, group_gib, concurrent):
+    """group_gib > 0 runs the jobs in consecutive groups (1e-9: every job its own group);
+    `concurrent` of them at once on their own stream branches and frame buffers, the groups of
+    a branch sharing one set. The images and ray counts stay the same."""
     jobs = _jobs(R, workload)
     o = R.default_batch_opts()
     o.use_graph, o.tile, o.chunk, o.blocks_per_cu, o.group_gib = use_graph, tile, 256, 2, group_gib
+    o.concurrent = concurrent
     b = R.Batch(jobs, o)
     st = b.run()
     assert st.n_jobs == len(jobs) and (st.graph_nodes > 0) == bool(use_graph)
@@ -91,9 +94,9 @@ def test_batch_equals_single_frames(R, workload, use_graph, tile, group_gib):
         assert st.groups == 1
     else:
         assert st.groups > 1
-    if group_gib > 0.0:  # the groups share buffers smaller than all jobs' together
+    if group_gib > 0.0:  # the branches' buffers are smaller than all jobs' together
         o1 = R.default_batch_opts()
-        o1.use_graph, o1.tile, o1.chunk, o1.blocks_per_cu, o1.group_gib = 0, tile, 256, 2, 0.0
+        o1.use_graph, o1.tile, o1.chunk, o1.blocks_per_cu, o1.group_gib, o1.concurrent = 0, tile, 256, 2, 0.0, 1
         one = R.Batch(jobs, o1)
         assert st.bytes < one.run().bytes
         one.close()
@@ -107,10 +110,14 @@ def test_batch_equals_single_frames(R, workload, use_graph, tile, group_gib):
     b.close()
 
 
-def test_graph_replays_are_identical(R, workload):
+@pytest.mark.parametrize("concurrent", [1, 3])
+def test_graph_replays_are_identical(R, workload, concurrent):
+    """Replays of the captured graph, one job group or every job its own group on 3 branches."""
     import torch
     jobs = _jobs(R, workload)
-    b = R.Batch(jobs)
+    o = R.default_batch_opts()
+    o.group_gib, o.concurrent = (0.0, 1) if concurrent == 1 else (1e-9, concurrent)
+    b = R.Batch(jobs, o)
     b.run()
     first = [o.clone() for *_, o in jobs]
     for _ in range(3):
@@ -162,4 +169,8 @@ def test_batch_rejects_bad_jobs(R, workload):
     o = R.default_batch_opts()
     o.tile = 5
     with pytest.raises(R.RtraceError, match="tile"):
+        R.Batch([(scene, None, r, out)], o)
+    o = R.default_batch_opts()
+    o.concurrent = 0
+    with pytest.raises(R.RtraceError, match="concurrent"):
         R.Batch([(scene, None, r, out)], o)

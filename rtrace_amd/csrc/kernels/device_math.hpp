@@ -150,6 +150,31 @@ struct RayD {
 #define RT_HOT inline
 #endif
 
+// Loads from device memory through a pointer the compiler knows only as generic (one read
+// from memory, such as a mesh's node and triangle arrays): plain dereferences compile to
+// flat loads, which also count against the LDS wait counter and take 64-bit addresses;
+// these load through the global address space instead. On the host, plain loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define RT_AS_GLOBAL __attribute__((address_space(1)))
+typedef float rt_f4v __attribute__((ext_vector_type(4)));
+typedef uint32_t rt_u4v __attribute__((ext_vector_type(4)));
+RT_HOT float4 ldg(const float4* p) {
+    const rt_f4v v = *(const RT_AS_GLOBAL rt_f4v*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+RT_HOT uint4 ldg(const uint4* p) {
+    const rt_u4v v = *(const RT_AS_GLOBAL rt_u4v*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+RT_HOT double ldg(const double* p) { return *(const RT_AS_GLOBAL double*)p; }
+RT_HOT uint32_t ldg(const uint32_t* p) { return *(const RT_AS_GLOBAL uint32_t*)p; }
+#else
+template <class T>
+RT_HOT T ldg(const T* p) {
+    return *p;
+}
+#endif
+
 // ---------------------------------------------------------------- exact f64 tests
 // Sphere::hit (src/ray.rs:89-120); t_max inclusive.
 RT_HD bool sphere_exact(const ObjDev& s, const RayD& r, double tmin, double tmax, double& t) {
@@ -234,9 +259,9 @@ RT_HD bool cube_exact(const ObjDev& cb, const RayD& r, double tmin, double tmax,
 // intersect_triangle (src/ray.rs:527-583): strict t_min < t < t_max.
 RT_HD bool tri_exact(const double* v, const RayD& r, double tmin, double tmax, double& t, d3& n, double& u,
                                  double& vv) {
-    d3 v0 = mk(v[0], v[1], v[2]);
-    d3 e1 = sub(mk(v[3], v[4], v[5]), v0);
-    d3 e2 = sub(mk(v[6], v[7], v[8]), v0);
+    d3 v0 = mk(ldg(v + 0), ldg(v + 1), ldg(v + 2));
+    d3 e1 = sub(mk(ldg(v + 3), ldg(v + 4), ldg(v + 5)), v0);
+    d3 e2 = sub(mk(ldg(v + 6), ldg(v + 7), ldg(v + 8)), v0);
     d3 h = cross(r.d, e2);
     double a = dot(e1, h);
     if (a > -1e-8 && a < 1e-8) return false;

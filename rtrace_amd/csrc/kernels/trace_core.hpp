@@ -502,8 +502,8 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
         bool next = false;  // `node` already holds the next node to visit
         if (s.c0 <= s.best_hi) {
             const float4* np = reinterpret_cast<const float4*>(M.nodes + s.node);
-            float4 n0 = np[0];
-            float4 n1 = np[1];
+            float4 n0 = ldg(np);
+            float4 n1 = ldg(np + 1);
             if (count) w.v[kWorkNodes]++;
             const float bmin[3] = {n0.x, n0.y, n0.z}, bmax[3] = {n0.w, n1.x, n1.y};
             uint32_t na = f32_to_bits(n1.z), nb = f32_to_bits(n1.w);
@@ -513,7 +513,7 @@ RT_HOT uint32_t walk_to_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, Stack
                     leaf = s.node;
                     leaf_c0 = s.c0;
                 } else {
-                    double split = M.nodes[s.node].split;
+                    double split = ldg(&M.nodes[s.node].split);
                     double oa = ax == 0 ? rox : (ax == 1 ? roy : roz);
                     double da = ax == 0 ? rdx : (ax == 1 ? rdy : rdz);
                     float ia = ax == 0 ? s.q.ix : (ax == 1 ? s.q.iy : s.q.iz);
@@ -602,8 +602,9 @@ RT_HOT bool bvh_to_leaf(MeshWalk& s, const MeshDev& M, StackRef stk, Work& w, ui
     while (s.have_node) {
         const uint32_t nidx = s.node;
         const float4* np = reinterpret_cast<const float4*>(M.bvh + nidx);
-        const float4 lx = np[0], ly = np[1], lz = np[2], hx = np[3], hy = np[4], hz = np[5];
-        const uint4 lk = reinterpret_cast<const uint4*>(np)[6];
+        const float4 lx = ldg(np), ly = ldg(np + 1), lz = ldg(np + 2), hx = ldg(np + 3), hy = ldg(np + 4),
+                     hz = ldg(np + 5);
+        const uint4 lk = ldg(reinterpret_cast<const uint4*>(np) + 6);
         if (count) w.v[kWorkBvhNodes]++;
         float t[4];
         uint32_t hit = 0;
@@ -668,9 +669,8 @@ RT_HOT bool next_leaf(MeshWalk& s, const MeshDev& M, const RayD& r, StackRef stk
     if (s.bvh) return bvh_to_leaf<count>(s, M, stk, w, first, cnt, leaf_c0);
     const uint32_t leaf = walk_to_leaf<count>(s, M, r, stk, w, leaf_c0);
     if (leaf == kNoLeaf) return false;
-    const KdNode& ln = M.nodes[leaf];
-    first = ln.a;
-    cnt = ln.b >> 2;
+    first = ldg(&M.nodes[leaf].a);
+    cnt = ldg(&M.nodes[leaf].b) >> 2;
     return true;
 }
 
@@ -705,9 +705,9 @@ RT_HOT bool leaf_test(MeshWalk& s, const MeshDev& M, const RayD& r, uint32_t fir
     uint32_t c0 = 0, c1 = 0;
     int nc = 0;
     for (uint32_t k = 0; k < cnt; ++k) {
-        float4 r0 = recs[(size_t)(first + k) * 3 + 0];
-        float4 r1 = recs[(size_t)(first + k) * 3 + 1];
-        float4 r2 = recs[(size_t)(first + k) * 3 + 2];
+        float4 r0 = ldg(recs + (size_t)(first + k) * 3 + 0);
+        float4 r1 = ldg(recs + (size_t)(first + k) * 3 + 1);
+        float4 r2 = ldg(recs + (size_t)(first + k) * 3 + 2);
         if (count) w.v[kWorkTris]++;
         float e1x = r0.w, e1y = r1.x, e1z = r1.y;
         float e2x = r1.z, e2y = r1.w, e2z = r2.x;

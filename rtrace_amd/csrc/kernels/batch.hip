@@ -105,10 +105,15 @@ __global__ void k_batch_tally(BatchArgs a) {
 }
 
 // The scene and frame views reach the stage kernels as separate read-only, non-aliased
-// kernel arguments so that their wave-uniform fields load through the scalar cache.
-#define BATCH_PARAMS                                                                                        \
-    const SceneView *__restrict__ views, const FrameView *__restrict__ frames, const int32_t *__restrict__ base, \
-        int32_t *counter, int n_jobs, int chunk, int level
+// kernel arguments so that their wave-uniform fields load through the scalar cache. They
+// are in the constant address space (nothing writes them while the batch runs): the compiler
+// then takes the buffer pointers read from them as global, as it does for a single frame's
+// views passed by value, and the item functions' loads and stores through them are global
+// accesses rather than flat ones.
+#define RT_AS_CONST __attribute__((address_space(4)))
+#define BATCH_PARAMS                                                                        \
+    const RT_AS_CONST SceneView *__restrict__ views, const RT_AS_CONST FrameView *__restrict__ frames, \
+        const int32_t *__restrict__ base, int32_t *counter, int n_jobs, int chunk, int level
 
 // Takes the next chunk for this workgroup: false when none is left, else its job and the
 // job's first item of the chunk (both wave-uniform).
@@ -141,8 +146,8 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(4))
     int j;
     long long i0;
     while (next_chunk(base, counter, n_jobs, chunk, &s_chunk, j, i0)) {
-        const SceneView& S = views[j];
-        const FrameView& F = frames[2 * j + (level & 1)];
+        const SceneView& S = *(const SceneView*)(views + j);
+        const FrameView& F = *(const FrameView*)(frames + 2 * j + (level & 1));
         const long long n = stage_items(kStage, S, F, level);
         for (int k = 0; k < chunk; k += kBatchBlock) {
             const long long i = i0 + k + threadIdx.x;
@@ -171,8 +176,8 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_plain(BATCH_PARAMS) {
     int j;
     long long i0;
     while (next_chunk(base, counter, n_jobs, chunk, &s_chunk, j, i0)) {
-        const SceneView& S = views[j];
-        const FrameView& F = frames[2 * j + (level & 1)];
+        const SceneView& S = *(const SceneView*)(views + j);
+        const FrameView& F = *(const FrameView*)(frames + 2 * j + (level & 1));
         const long long n = stage_items(kStage, S, F, level);
         for (int k = 0; k < chunk; k += kBatchBlock) {
             const long long i = i0 + k + threadIdx.x;
@@ -200,7 +205,9 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_plain(BATCH_PARAMS) {
 
 hipError_t launch_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t st) {
     const dim3 blk(kBatchBlock), grid(l.grid);
-#define BATCH_ARGS(lvl) a.views, a.frames, a.base, a.counter, a.n_jobs, a.chunk, (lvl)
+#define BATCH_ARGS(lvl)                                                                                      \
+    (const RT_AS_CONST SceneView*)a.views, (const RT_AS_CONST FrameView*)a.frames, a.base, a.counter, a.n_jobs, \
+        a.chunk, (lvl)
     auto plan = [&](int stage, int level) { hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, st, a, stage, level); };
     hipLaunchKernelGGL(k_batch_init, dim3(a.n_jobs), dim3(64), 0, st, a);
     plan(kStRaygen, 0);

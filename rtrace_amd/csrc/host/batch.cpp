@@ -29,6 +29,7 @@ struct rt_batch {
     std::vector<hipStream_t> side;       // streams of branches 1..concurrent-1
     std::vector<hipEvent_t> fork, join;  // one each per side stream
     size_t plan_stride = 0;              // int32 entries between the branches' plan buffers
+    size_t plan_counters = 0;            // offset of the slice counters in a plan buffer
     std::vector<rt::FrameView> hframes;  // [2 * n_jobs]
     std::vector<rt::SceneView> hviews;
     std::vector<rt::FrameLayout> layouts;
@@ -83,7 +84,7 @@ hipError_t launch_groups(rt_batch* b, hipStream_t st) {
         a.frames += 2 * (size_t)j0;
         a.tally += 2 * (size_t)j0;
         a.base += c * b->plan_stride;
-        a.counter = a.base + (j1 - j0) + 1;
+        a.counter = a.base + b->plan_counters;
         a.n_jobs = j1 - j0;
         BatchLaunch l = b->launch;
         l.max_depth = b->group_depth[g];
@@ -209,7 +210,9 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
     hipStream_t st = ctx_stream(ctx);
     b->views.reserve(n * sizeof(SceneView));
     b->frames.reserve(2 * (size_t)n * sizeof(FrameView));
-    b->plan_stride = ((size_t)n + 2 + 31) & ~(size_t)31;  // each branch's counter on its own 128-byte line
+    // each branch: base[n + 1], then its slices' counters on 128-byte lines of their own
+    b->plan_counters = ((size_t)n + 1 + 31) & ~(size_t)31;
+    b->plan_stride = b->plan_counters + 32 * (size_t)kBatchSlices;
     b->plan.reserve(b->plan_stride * (size_t)C * sizeof(int32_t));
     b->tally.reserve(2 * (size_t)n * sizeof(unsigned long long));
     HIP_TRY(hipMemcpyAsync(b->views.p, dviews.data(), n * sizeof(SceneView), hipMemcpyHostToDevice, st));
@@ -217,7 +220,7 @@ std::unique_ptr<rt_batch> build(rt_ctx* ctx, const rt_batch_job* jobs, int n, co
     b->args.views = b->views.as<SceneView>();
     b->args.frames = b->frames.as<FrameView>();
     b->args.base = b->plan.as<int32_t>();
-    b->args.counter = b->plan.as<int32_t>() + n + 1;
+    b->args.counter = b->plan.as<int32_t>() + b->plan_counters;
     b->args.n_jobs = n;
     b->args.chunk = bo.chunk;
     b->args.tally = b->tally.as<unsigned long long>();

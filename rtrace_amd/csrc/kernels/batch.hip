@@ -4,7 +4,8 @@
 // Every stage of the single-frame pipeline (raygen, closest, resolve, shade, shadow,
 // finish, pixel, finalize) becomes one launch for the whole batch. A plan kernel reads
 // each job's device-side item count for the stage and lays the jobs' chunks end to end;
-// a persistent grid then takes chunks from one atomic counter. A chunk never spans two
+// a persistent grid then takes the chunks, each XCD from its own slice of them first
+// (ChunkQueue). A chunk never spans two
 // jobs, so the scene and frame views a workgroup reads are uniform (scalar loads), and
 // jobs of very different cost share the machine without a host round trip. The item
 // functions are the single-frame ones (trace_core.hpp), so a batched frame is the same
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(1024) void k_plan(BatchArgs a, int stage, int level
         run += (int32_t)((n + a.chunk - 1) / a.chunk);
     }
     if (threadIdx.x == 1023) a.base[a.n_jobs] = s_part[1023];
-    if (threadIdx.x == 0) *a.counter = 0;
+    if (threadIdx.x < kBatchSlices) a.counter[threadIdx.x * 32] = 0;
 }
 
 __global__ void k_batch_init(BatchArgs a) {
@@ -116,31 +117,45 @@ __global__ void k_batch_tally(BatchArgs a) {
         const int32_t *__restrict__ base, int32_t *counter, int n_jobs, int chunk, int level
 
 // The chunks of one stage launch, taken one at a time by a one-wavefront workgroup (lane 0's
-// atomic, no barrier); the chunks a workgroup takes only increase, so the search for a
-// chunk's job starts from the workgroup's previous job (usually the same one). Taking the
-// next chunk before working on the current one, to hide the atomic, measured 5 % slower on
-// the 64-scene batch: each wavefront then holds a chunk it has not started, and the stage's
-// last chunks wait behind busy wavefronts.
+// atomic, no barrier). The chunk range is cut into kBatchSlices contiguous slices, one per
+// XCD (workgroups go to the XCDs round-robin by id), each with its own counter: an XCD works
+// through its own part of the jobs, which its L2 then holds, and moves on to the other slices
+// once its own is drained. The search for a chunk's job starts from the workgroup's previous
+// job (usually the same one). Taking the next chunk before working on the current one, to hide
+// the atomic, measured 5 % slower on the 64-scene batch: each wavefront then holds a chunk it
+// has not started, and the stage's last chunks wait behind busy wavefronts.
 static_assert(kBatchBlock == 64, "a chunk queue belongs to one wavefront");
 struct ChunkQueue {
     const int32_t* __restrict__ base;  // [n_jobs + 1]: first chunk of each job, total last
-    int32_t* counter;
-    int n_jobs, chunk, total, job;
+    int32_t* counter;                   // [kBatchSlices * 32]
+    int n_jobs, chunk, total, job, slice, left;
     __device__ ChunkQueue(const int32_t* __restrict__ b, int32_t* c, int nj, int ch)
-        : base(b), counter(c), n_jobs(nj), chunk(ch), job(0) {
+        : base(b), counter(c), n_jobs(nj), chunk(ch), job(0), slice((int)(blockIdx.x % kBatchSlices)),
+          left(kBatchSlices) {
         total = base[n_jobs];
     }
+    // the next chunk of this workgroup's slice, else of the next slices; total when none is left
     __device__ int take() {
-        int c = 0;
-        if (__lane_id() == 0) c = atomicAdd(counter, 1);
-        return __builtin_amdgcn_readfirstlane(c);
+        while (left > 0) {
+            const int lo = (int)((long long)total * slice / kBatchSlices);
+            const int hi = (int)((long long)total * (slice + 1) / kBatchSlices);
+            if (lo < hi) {
+                int t = 0;
+                if (__lane_id() == 0) t = atomicAdd(&counter[slice * 32], 1);
+                t = __builtin_amdgcn_readfirstlane(t);
+                if (t < hi - lo) return lo + t;
+            }
+            slice = (slice + 1) % kBatchSlices;
+            --left;
+        }
+        return total;
     }
     // false when no chunk is left; else the chunk's job and its first item (wave-uniform)
     __device__ bool pop(int& j, long long& first) {
         const int c = take();
         if (c >= total) return false;
-        if (base[job + 1] <= c) {  // a later job: the last one whose first chunk is <= c
-            int lo = job + 1, hi = n_jobs;  // base[lo] <= c < base[hi] = total
+        if (c < base[job] || base[job + 1] <= c) {  // another job: the last one whose first chunk is <= c
+            int lo = 0, hi = n_jobs;  // base[lo] <= c < base[hi] = total
             while (hi - lo > 1) {
                 const int mid = (lo + hi) >> 1;
                 if (base[mid] <= c) lo = mid;

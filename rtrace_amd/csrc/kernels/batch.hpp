@@ -14,7 +14,7 @@ struct BatchArgs {
     const SceneView* views;   // [n_jobs]
     const FrameView* frames;  // [2 * n_jobs]
     int32_t* base;            // [n_jobs + 1] first chunk of each job in the current stage
-    int32_t* counter;         // next chunk to hand out
+    int32_t* counter;         // [kBatchSlices * 32]: chunks handed out of each slice, 128 bytes apart
     int32_t n_jobs;
     int32_t chunk;            // items per chunk (a multiple of the block size)
     unsigned long long* tally;  // [n_jobs][2]: reflection and shadow rays, written at the end
@@ -30,5 +30,6 @@ struct BatchLaunch {
 hipError_t launch_batch(const BatchArgs& a, const BatchLaunch& l, hipStream_t st);
 
 constexpr int kBatchBlock = 64;  // one wavefront per workgroup
+constexpr int kBatchSlices = 8;  // chunk-range slices of a stage launch, one per XCD
 
 }  // namespace rt

@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kOT * kOT) void k_outline_edges(FrameView F) {
         int lx = i % (kOT + 2), ly = i / (kOT + 2);
         int gx = x0 + lx, gy = y0 + ly;
         bool in = gx >= 0 && gy >= 0 && gx < F.W && gy < F.H;
-        double d = NAN;
+        double d = -1.0;
         double n0 = 0, n1 = 0, n2 = 0;
         if (in) {
             const double* q = F.px_dn + ((size_t)gy * F.W + gx) * 4;
@@ -264,7 +264,7 @@ __global__ __launch_bounds__(kOT * kOT) void k_outline_edges(FrameView F) {
         s_n[ly][lx][0] = n0;
         s_n[ly][lx][1] = n1;
         s_n[ly][lx][2] = n2;
-        s_f[ly][lx] = in && d == d;
+        s_f[ly][lx] = in && !(d < 0.0);  // a hit: depth >= 0, or NaN for a NaN camera ray
     }
     __syncthreads();
     int x = blockIdx.x * kOT + threadIdx.x, y = blockIdx.y * kOT + threadIdx.y;

@@ -87,7 +87,8 @@ struct FrameView {
     int32_t* prim_obj;    // [n_paths]
     int32_t* prim_tri;
     double* prim_t;
-    double* ol_dn;        // [n_paths][4] depth, nx, ny, nz (NaN depth = no hit)
+    double* ol_dn;        // [n_paths][4] depth, nx, ny, nz (depth -1 = no hit: a hit's depth is a
+                          // length, >= 0 or NaN for a NaN camera ray, which the reference outlines too)
     RayRec* rays;         // current level queue
     RayRec* rays_next;    // next level queue
     int32_t* counts;      // [max_depth + 1] queue lengths; counts[0] = n_q0
@@ -107,7 +108,7 @@ struct FrameView {
     double* pix;          // [W*H*3] final linear colour
     uint8_t* rgb;         // [W*H*3]
     // outline buffers
-    double* px_dn;        // [W*H*4] per-pixel depth/normal (NaN = background)
+    double* px_dn;        // [W*H*4] per-pixel depth/normal (depth -1 = background)
     double* edge;         // [W*H]
     double* edge2;        // [W*H]
     double ol_wd, ol_wn, ol_thr, ol_thick;
@@ -1253,7 +1254,7 @@ RT_HD bool closest_item(const SceneView& S, const FrameView& F, int slot, int le
         L[2] = c.z;
         L[3] = -1.0;
         if (level == 0 && F.outline) {
-            F.ol_dn[(size_t)rr.path * 4] = NAN;
+            F.ol_dn[(size_t)rr.path * 4] = -1.0;
         }
     }
     return hit;
@@ -1820,14 +1821,16 @@ RT_HD void pixel_item(const FrameView& F, int pix) {
     st3(F.pix + (size_t)pix * 3, c);
     if (F.outline) {
         double* o = F.px_dn + (size_t)pix * 4;
-        o[0] = NAN;
+        o[0] = -1.0;
         o[1] = o[2] = o[3] = 0.0;
         if (F.max_depth > 0) {
+            // the nearest sample with a hit, the first one on equal (or NaN) depths, as the
+            // reference picks it (src/renderer.rs:692-697)
             bool have = false;
             for (int s = 0; s < F.N; ++s) {
                 const double* q = F.ol_dn + ((size_t)pix * F.N + s) * 4;
                 double d = q[0];
-                if (d == d && (!have || d < o[0])) {
+                if (!(d < 0.0) && (!have || d < o[0])) {
                     have = true;
                     o[0] = d;
                     o[1] = q[1];

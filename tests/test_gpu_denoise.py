@@ -129,13 +129,13 @@ def square_scene(tmp_path_factory):
                           light_samples=4)
 
 
-def test_render_denoised_tile_fallback(DN, square_scene):
+@pytest.mark.parametrize("W,H", [(160, 90), (1, 1), (9, 5), (17, 3)])
+def test_render_denoised_tile_fallback(DN, square_scene, W, H):
     """threshold < 0 re-renders every tile: the frame is the many-sample render, exactly;
     threshold > 1 re-renders none: the frame is the network's output on the few-sample
-    features."""
+    features. Also on frames smaller than a tile and than the network's 8-pixel padding."""
     import torch
     import rtrace_amd as R
-    W, H = 160, 90
     scene = R.Scene.from_json_file(square_scene)
     r = R.Renderer(W, H)
     r.max_depth = 4
@@ -146,7 +146,7 @@ def test_render_denoised_tile_fallback(DN, square_scene):
     r.light_samples = 4
     out = torch.zeros_like(ref)
     st = dn.render(r, scene, out, fallback_samples=64, threshold=-1.0, tile=16)
-    assert st.tiles_fallback == st.tiles == 10 * 6
+    assert st.tiles_fallback == st.tiles == ((W + 15) // 16) * ((H + 15) // 16)
     assert torch.equal(out, ref)
     out2 = torch.zeros_like(ref)
     st2 = dn.render(r, scene, out2, fallback_samples=64, threshold=2.0, tile=16)

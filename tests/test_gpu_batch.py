@@ -110,6 +110,45 @@ def test_batch_equals_single_frames(R, workload, use_graph, tile, group_gib, con
     b.close()
 
 
+def test_batch_edge_jobs_equal_single_frames(R, workload, tmp_path):
+    """Jobs of 1x1, one row, one column and odd sizes in every sampling mode, on a scene with no
+    objects, one with no lights and a mesh scene, batched together (one group, and one group per
+    job on 3 branches): each image and ray count equals the frame rendered alone."""
+    import torch
+    mesh_path = workload[0][0]
+    with open(mesh_path) as f:
+        base = json.load(f)
+    paths = [mesh_path]
+    for name, sc in (("no_objects", dict(base, objects=[])), ("no_lights", dict(base, lights=[]))):
+        p = os.path.join(os.path.dirname(mesh_path), name + ".json")
+        with open(p, "w") as f:
+            json.dump(sc, f)
+        paths.append(p)
+    jobs = []
+    for i, path in enumerate(paths):
+        scene = R.Scene.from_json_file(path)
+        for k, (w, h) in enumerate([(1, 1), (1, 9), (9, 1), (17, 5)]):
+            aa, spp, ls = MODES[(i + k) % 3]
+            r = R.Renderer(w, h)
+            r.max_depth = 3
+            r.samples = spp
+            r.light_samples = ls
+            r.anti_aliasing_mode = O.AA_BY_NAME[aa]
+            jobs.append((scene, None, r, torch.zeros(h * w * 3, dtype=torch.uint8, device="cuda")))
+    for group_gib, concurrent in ((0.0, 1), (1e-9, 3)):
+        o = R.default_batch_opts()
+        o.group_gib, o.concurrent = group_gib, concurrent
+        b = R.Batch(jobs, o)
+        st = b.run()
+        rays = [0, 0, 0]
+        for scene, cam, r, out in jobs:
+            ref, s1 = _single(R, scene, cam, r)
+            assert np.array_equal(out.cpu().numpy(), ref.cpu().numpy()), (r.width, r.height)
+            rays = [rays[0] + s1.rays_primary, rays[1] + s1.rays_reflect, rays[2] + s1.rays_shadow]
+        assert [st.rays_primary, st.rays_reflect, st.rays_shadow] == rays
+        b.close()
+
+
 @pytest.mark.parametrize("concurrent", [1, 3])
 def test_graph_replays_are_identical(R, workload, concurrent):
     """Replays of the captured graph, one job group or every job its own group on 3 branches."""

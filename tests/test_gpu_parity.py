@@ -271,6 +271,29 @@ def test_edge_sizes_and_scenes_vs_oracle(R, edge_scenes, scene, aa, spp):
                                                                       ost["rays_shadow"])
 
 
+def test_oversized_frames_are_refused(R, config_dir):
+    """Frames beyond one launch's 32-bit item indices are refused with an error before anything
+    is allocated or launched (the 16-byte output stays untouched), and the context then renders
+    a frame equal to the oracle's."""
+    import torch
+    path = config_dir["icosphere_s4"]
+    scene = R.Scene.from_json_file(path)
+    out = torch.zeros(16, dtype=torch.uint8, device="cuda")
+    r = R.Renderer(23200, 23200)  # quincunx: 2 x 5.4e8 camera rays, over 2^30
+    with pytest.raises(R.RtraceError, match="image too large"):
+        r.render_device(scene, out)
+    r = R.Renderer(1920, 1080)
+    r.light_samples = 1024  # 4.2e6 queue entries x 1024 disk samples, over 2^31 shadow items
+    with pytest.raises(R.RtraceError, match="too many shadow samples"):
+        r.render_device(scene, out)
+    torch.cuda.synchronize()
+    assert int(torch.count_nonzero(out)) == 0
+    opts = O.make_opts(64, 48, max_depth=3)
+    img, st, gh = gpu_render(R, path, opts)
+    ref, ost, rh = O.render(path, opts, want_hits=True)
+    compare("after refused frames", img, ref, gh, rh)
+
+
 def test_zero_samples_error(R, tmp_path):
     path = os.path.join(REF_SCENES, "object-sphere.json")
     r = R.Renderer(10, 10)

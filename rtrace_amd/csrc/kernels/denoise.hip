@@ -10,7 +10,9 @@
 // and each of its four waves runs 9 taps x (2 pixel rows x 2 channel halves) MFMAs. The
 // same kernel computes backward-data (input = dZ, weights flipped and transposed, f32
 // output). Weight gradients stage the halo and dZ transposed (channel-major, one copy per
-// x shift) so that both MFMA operands are 16-byte aligned 8-pixel runs.
+// x shift) so that both MFMA operands are 16-byte aligned 8-pixel runs. Convolutions with
+// 32 input channels (forward and backward-data) run on a kernel without LDS: one wavefront
+// per 16-pixel column strip, the weights in registers, the pixels read straight from memory.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -427,6 +429,181 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
+// Layers with 32 input channels, one wavefront per column strip (16 pixels wide, kRows
+// rows tall) and no LDS or barrier: the wavefront keeps its 32 output channels' weights in
+// registers (9 taps x 2 halves of 16 channels) and walks the strip from top to bottom. The
+// pixel operand of a tap, 8 channels of one input pixel per lane, is read straight from
+// memory (16 pixels of a row are 1 KB in a row), once per input row and pixel shift; the
+// rows in use stay in kSlots register sets, and the loads of the row kSlots - 3 rows ahead
+// are in flight during the current row's 18 MFMAs. The products are the transposed ones of
+// the LDS kernels (weights as the A operand, pixels as B), so each lane ends with 4
+// consecutive channels of one pixel and stores them as one vector; every output is the
+// same 32-term dot product per tap, accumulated in tap order, as in those kernels. Sources
+// and outputs are under 2 GiB: raw buffer loads read an offset past the end as zero (the
+// convolution's zero padding) and buffer stores there are dropped (pixels past the edge).
+constexpr int kStripW = 16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t kOff = 0x80000000u;  // a buffer offset past every buffer's end
+
+__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+// the value of lane ^ 1 (DPP quad_perm [1, 0, 3, 2])
+__device__ __forceinline__ uint32_t lane_xor1(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+}
+
+template <int kRows, int kSlots>
+__global__ __launch_bounds__(256) void k_conv3x3_c32(ConvArgs a, int strips_x,
+                                                                                            int strips_y,
+                                                                                            unsigned src_bytes,
+                                                                                            unsigned out_bytes,
+                                                                                            unsigned pool_bytes) {
+    static_assert(kSlots >= 4 && kRows % 2 == 0, "three rows in use plus one loading; strips start on even rows");
+    const int lane = threadIdx.x & 63;
+    const int gq = lane >> 4, r16 = lane & 15;
+    const int co0 = blockIdx.y * kNT;
+    const bool first = a.in.c0 > 0;  // the 32 channels come from one source
+    const bool up = first && a.in.up0;
+    const int sh = up ? a.H >> 1 : a.H, sw = up ? a.W >> 1 : a.W;  // the source's resolution
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(first ? a.in.src0 : a.in.src1), (short)0, (int)src_bytes, 0x00020000);
+    const bool f32 = a.out_f32 != nullptr;
+    const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(
+        f32 ? (void*)a.out_f32 : (void*)a.out_bf16, (short)0, (int)out_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rp =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.pool_out, (short)0, (int)pool_bytes, 0x00020000);
+
+    // weights of tap t, half nt: output channel co0 + 16 nt + r16, input channels 8 gq .. 8 gq + 7
+    bf16x8 w[9][2];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+            w[t][nt] = *reinterpret_cast<const bf16x8*>(a.wt + ((size_t)t * a.coutp + co0 + nt * 16 + r16) * kCK + gq * 8);
+    // this lane's outputs: channels co0 + 16 nt + 4 gq + k of pixel x0 + r16
+    float bias[2][4];
+    bool full[2];  // all four channels written
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+        const int c = co0 + nt * 16 + 4 * gq;
+        full[nt] = c + 3 < a.cout;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) bias[nt][k] = a.bias && c + k < a.cout ? a.bias[c + k] : 0.0f;
+    }
+
+    const int n_strips = a.B * strips_x * strips_y;
+    for (int s = (int)blockIdx.x * 4 + (int)(threadIdx.x >> 6); s < n_strips; s += (int)gridDim.x * 4) {
+        const int b = s / (strips_x * strips_y);
+        const int t = s - b * strips_x * strips_y;
+        const int y0 = (t / strips_x) * kRows, x0 = (t % strips_x) * kStripW;
+        const int y_end = min(y0 + kRows, a.H);
+        const int x = x0 + r16;
+        // input row yy at the three pixel shifts of lane r16 (columns x - 1 .. x + 1)
+        auto load_row = [&](int yy, bf16x8(&f)[3]) {
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int xx = x + dx - 1;
+                const bool ok = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                const uint32_t e = up ? (uint32_t)((b * sh + (yy >> 1)) * sw + (xx >> 1))
+                                      : (uint32_t)((b * a.H + yy) * a.W + xx);
+                const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, ok ? (e * kCK + gq * 8) * 2u : kOff, 0, 0);
+                f[dx] = __builtin_bit_cast(bf16x8, v);
+            }
+        };
+        // slot k % kSlots holds input row y0 - 1 + k
+        bf16x8 F[kSlots][3];
+#pragma unroll
+        for (int k = 0; k < kSlots - 1; ++k) load_row(y0 - 1 + k, F[k]);
+        uint32_t top[2][2];  // the even row's bf16 outputs (two per dword), for the 2x2 pool
+        for (int yb = y0; yb < y_end; yb += kSlots) {
+#pragma unroll
+            for (int i = 0; i < kSlots; ++i) {
+                const int y = yb + i;
+                if (y >= y_end) break;
+                // input row y + kSlots - 2 into the slot row y - 2 held
+                const int ya = y + kSlots - 2;
+                if (ya <= y_end) load_row(ya, F[(i + kSlots - 1) % kSlots]);
+                f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+                for (int tap = 0; tap < 9; ++tap) {
+                    const int dy = tap / 3, dx = tap - dy * 3;
+                    const bf16x8 pv = F[(i + dy) % kSlots][dx];
+                    acc[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tap][0], pv, acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w[tap][1], pv, acc[1], 0, 0, 0);
+                }
+                // D layout: lane holds channels 4 gq .. 4 gq + 3 (of each half) of pixel r16
+                const uint32_t px = (uint32_t)((b * a.H + y) * a.W + x);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) {
+                    const int c = co0 + nt * 16 + 4 * gq;
+                    const uint32_t el = px * (uint32_t)a.out_pitch + (uint32_t)c;
+                    float v[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) v[k] = acc[nt][k] + bias[nt][k];
+                    if (f32) {
+                        if (full[nt]) {
+                            const u32x4 q = {__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]),
+                                             __float_as_uint(v[3])};
+                            __builtin_amdgcn_raw_buffer_store_b128(q, ro, x < a.W ? el * 4u : kOff, 0, 0);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+                                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[k]), ro,
+                                                                      x < a.W && c + k < a.cout ? (el + k) * 4u : kOff,
+                                                                      0, 0);
+                        }
+                        continue;
+                    }
+                    uint16_t q[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) q[k] = f2bf(a.relu ? fmaxf(v[k], 0.0f) : v[k]);
+                    const uint32_t lo = pack2(q[0], q[1]), hi = pack2(q[2], q[3]);
+                    if (full[nt]) {
+                        __builtin_amdgcn_raw_buffer_store_b64(u32x2{lo, hi}, ro, x < a.W ? el * 2u : kOff, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            __builtin_amdgcn_raw_buffer_store_b16(q[k], ro, x < a.W && c + k < a.cout ? (el + k) * 2u : kOff,
+                                                                  0, 0);
+                    }
+                    if (a.pool_out) {
+                        if ((y & 1) == 0) {
+                            top[nt][0] = lo;
+                            top[nt][1] = hi;
+                        } else {
+                            // the right-hand pixel's rows come from lane r16 ^ 1; even lanes store
+                            const uint32_t tr0 = lane_xor1(top[nt][0]), tr1 = lane_xor1(top[nt][1]);
+                            const uint32_t br0 = lane_xor1(lo), br1 = lane_xor1(hi);
+                            const uint32_t tl[2] = {top[nt][0], top[nt][1]}, bl[2] = {lo, hi}, tr[2] = {tr0, tr1},
+                                           br[2] = {br0, br1};
+                            uint16_t pq[4];
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                const int sft = (k & 1) * 16;
+                                pq[k] = pool4((uint16_t)(tl[k >> 1] >> sft), (uint16_t)(tr[k >> 1] >> sft),
+                                              (uint16_t)(bl[k >> 1] >> sft), (uint16_t)(br[k >> 1] >> sft));
+                            }
+                            const bool st = (r16 & 1) == 0 && x + 1 < a.W;
+                            const uint32_t pe =
+                                ((uint32_t)((b * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1))) * (uint32_t)a.out_pitch +
+                                (uint32_t)c;
+                            if (full[nt]) {
+                                __builtin_amdgcn_raw_buffer_store_b64(u32x2{pack2(pq[0], pq[1]), pack2(pq[2], pq[3])}, rp,
+                                                                      st ? pe * 2u : kOff, 0, 0);
+                            } else {
+#pragma unroll
+                                for (int k = 0; k < 4; ++k)
+                                    __builtin_amdgcn_raw_buffer_store_b16(pq[k], rp, st && c + k < a.cout ? (pe + k) * 2u : kOff,
+                                                                          0, 0);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Weight-gradient partial sums: part[split][tap][ci][co] = sum over this split's tiles of
 // X(p + tap)[ci] * dZ(p)[co]. grid.y enumerates (32-ci block, 32-co block).
 __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
@@ -736,6 +913,29 @@ int conv_resident() {
     }
     return cached[dev];
 }
+// Rows per strip and row register sets of k_conv3x3_c32 (217 VGPRs, 2 wavefronts per SIMD).
+// 1080p forward pass: 1.34 ms with 4 sets (rows 1 ahead), 1.33 with 6, 8 or 10 sets or 32
+// rows per strip; 1.46 ms on the LDS kernel. Held to 3 wavefronts per SIMD, 4 sets spilled
+// 20 registers and took 1.62 ms.
+constexpr int kC32Rows = 16, kC32Slots = 8;
+// Workgroups of k_conv3x3_c32 resident on the device at once, queried once per device.
+int c32_resident() {
+    static int cached[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_conv3x3_c32<kC32Rows, kC32Slots>, 256, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        cached[dev] = cus * per_cu;
+    }
+    return cached[dev];
+}
+
 hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
     const int tiles = a.B * ((a.H + kTH - 1) / kTH) * ((a.W + kTW - 1) / kTW);
     const int ny = a.coutp / kNT;
@@ -743,6 +943,21 @@ hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
     // all channel blocks, each taking tiles strided by gridDim.x.
     if (a.in.c0 + a.in.c1 != kCK) {
         hipLaunchKernelGGL(k_conv3x3_tile, dim3(tiles, ny), dim3(256), 0, st, a);
+        return hipGetLastError();
+    }
+    const bool first = a.in.c0 > 0;
+    const unsigned long long src_bytes =
+        (unsigned long long)a.B * (first && a.in.up0 ? (unsigned long long)(a.H >> 1) * (a.W >> 1) : (unsigned long long)a.H * a.W) *
+        kCK * 2;
+    const unsigned long long px = (unsigned long long)a.B * a.H * a.W;
+    const unsigned long long out_bytes = px * a.out_pitch * (a.out_f32 ? 4 : 2);
+    const unsigned long long pool_bytes = a.pool_out ? px / 4 * a.out_pitch * 2 : 0;
+    if (src_bytes < (1ull << 31) && out_bytes < (1ull << 31) && pool_bytes < (1ull << 31)) {
+        const int sx = (a.W + kStripW - 1) / kStripW, sy = (a.H + kC32Rows - 1) / kC32Rows;
+        const long long strips = (long long)a.B * sx * sy;
+        const int nx = (int)std::max(1LL, std::min((strips + 3) / 4, (long long)std::max(1, c32_resident() / ny)));
+        hipLaunchKernelGGL((k_conv3x3_c32<kC32Rows, kC32Slots>), dim3(nx, ny), dim3(256), 0, st, a, sx, sy,
+                           (unsigned)src_bytes, (unsigned)out_bytes, (unsigned)pool_bytes);
         return hipGetLastError();
     }
     const int nx = std::max(1, std::min(tiles, conv_resident() / ny));

@@ -604,6 +604,175 @@ __global__ __launch_bounds__(256) void k_conv3x3_c32(ConvArgs a, int strips_x,
     }
 }
 
+// Layers with several 32-channel K chunks, one wavefront per 16x4-pixel step and the
+// weights staged once: a workgroup of kRowsWaves wavefronts (one per CU, three per SIMD)
+// copies all weights of its 32 output channels into LDS (18 KB per K chunk, fragment-major
+// so a wavefront's read of one fragment is 1 KB in a row), then each wavefront takes steps
+// of 16 pixels x 4 rows on its own. Per K chunk it reads the 6 input rows x 3 pixel shifts
+// the step needs straight from memory (the pixel operand, 8 channels of one pixel per lane;
+// nearest upsampling of the first source is an index shift), and each weight fragment it
+// reads from LDS feeds the 4 rows' MFMAs. Products are the transposed ones of the LDS tile
+// kernel (weights as A, pixels as B) and accumulate in the same order, K chunk by K chunk
+// and tap by tap, so the outputs are the same bit for bit; each lane ends with 4
+// consecutive channels of one pixel. Sources and outputs are under 2 GiB (buffer loads past
+// the end read zero, stores there are dropped).
+constexpr size_t kRowsMaxLds = 8 * 9 * kNT * kCK * 2;  // 8 K chunks: 144 KiB
+
+template <int kStepRows, int kRowsWaves>
+__global__ __launch_bounds__(kRowsWaves * 64) void k_conv3x3_rows(ConvArgs a, int strips_x, int steps_y,
+                                                                  unsigned bytes0, unsigned bytes1, unsigned out_bytes,
+                                                                  unsigned pool_bytes) {
+    extern __shared__ __attribute__((aligned(16))) uint16_t s_wt[];  // [chunk][tap][half][lane] x 8 bf16
+    const int cin = a.in.c0 + a.in.c1, nc = cin / kCK, nc0 = a.in.c0 / kCK;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int gq = lane >> 4, r16 = lane & 15;
+    const int co0 = blockIdx.y * kNT;
+    {
+        const int units = nc * 9 * 2 * 64;
+        for (int u = threadIdx.x; u < units; u += blockDim.x) {
+            const int l = u & 63, nt = (u >> 6) & 1, tc = u >> 7, tap = tc % 9, c = tc / 9;
+            const uint4 v = *reinterpret_cast<const uint4*>(
+                a.wt + ((size_t)tap * a.coutp + co0 + nt * 16 + (l & 15)) * cin + c * kCK + (l >> 4) * 8);
+            *reinterpret_cast<uint4*>(s_wt + (size_t)u * 8) = v;
+        }
+    }
+    __syncthreads();  // the only barrier: from here on every wavefront works alone
+    const __amdgpu_buffer_rsrc_t rs0 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.in.src0, (short)0, (int)bytes0, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rs1 =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.in.src1, (short)0, (int)bytes1, 0x00020000);
+    const bool f32 = a.out_f32 != nullptr;
+    const __amdgpu_buffer_rsrc_t ro = __builtin_amdgcn_make_buffer_rsrc(
+        f32 ? (void*)a.out_f32 : (void*)a.out_bf16, (short)0, (int)out_bytes, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rp =
+        __builtin_amdgcn_make_buffer_rsrc((void*)a.pool_out, (short)0, (int)pool_bytes, 0x00020000);
+    const int h2 = a.H >> 1, w2 = a.W >> 1;
+
+    const int n_steps = a.B * strips_x * steps_y;
+    for (int s = (int)blockIdx.x * kRowsWaves + wave; s < n_steps; s += (int)gridDim.x * kRowsWaves) {
+        const int b = s / (strips_x * steps_y);
+        const int t = s - b * strips_x * steps_y;
+        const int y0 = (t / strips_x) * kStepRows, x = (t % strips_x) * kStripW + r16;
+        f32x4 acc[kStepRows][2];
+#pragma unroll
+        for (int r = 0; r < kStepRows; ++r) acc[r][0] = acc[r][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+        // input rows y0 - 1 .. y0 + kStepRows of K chunk c at the three pixel shifts
+        auto load_chunk = [&](int c, bf16x8(&f)[kStepRows + 2][3]) {
+            const bool first = c < nc0;
+            const bool up = first && a.in.up0;
+            const int cc = first ? c : c - nc0, cs = first ? a.in.c0 : a.in.c1;
+#pragma unroll
+            for (int r = 0; r < kStepRows + 2; ++r) {
+                const int yy = y0 + r - 1;
+#pragma unroll
+                for (int dx = 0; dx < 3; ++dx) {
+                    const int xx = x + dx - 1;
+                    const bool ok = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                    const uint32_t e = up ? (uint32_t)((b * h2 + (yy >> 1)) * w2 + (xx >> 1))
+                                          : (uint32_t)((b * a.H + yy) * a.W + xx);
+                    const uint32_t off = ok ? (e * (uint32_t)cs + (uint32_t)(cc * kCK + gq * 8)) * 2u : kOff;
+                    const u32x4 v = first ? __builtin_amdgcn_raw_buffer_load_b128(rs0, off, 0, 0)
+                                          : __builtin_amdgcn_raw_buffer_load_b128(rs1, off, 0, 0);
+                    f[r][dx] = __builtin_bit_cast(bf16x8, v);
+                }
+            }
+        };
+        auto mfma_chunk = [&](int c, const bf16x8(&f)[kStepRows + 2][3]) {
+#pragma unroll
+            for (int tap = 0; tap < 9; ++tap) {
+                const int dy = tap / 3, dx = tap - dy * 3;
+                const uint16_t* wp = s_wt + ((size_t)((c * 9 + tap) * 2) * 64 + lane) * 8;
+                const bf16x8 w0 = *reinterpret_cast<const bf16x8*>(wp);
+                const bf16x8 w1 = *reinterpret_cast<const bf16x8*>(wp + 64 * 8);
+#pragma unroll
+                for (int r = 0; r < kStepRows; ++r) {
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w0, f[r + dy][dx], acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w1, f[r + dy][dx], acc[r][1], 0, 0, 0);
+                }
+            }
+        };
+        for (int c = 0; c < nc; ++c) {
+            bf16x8 f[kStepRows + 2][3];
+            load_chunk(c, f);
+            mfma_chunk(c, f);
+        }
+        // D layout: lane holds channels 4 gq .. 4 gq + 3 (of each half) of pixel x
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int c = co0 + nt * 16 + 4 * gq;
+            const bool full_nt = c + 3 < a.cout;  // all four channels written
+            float bias[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) bias[k] = a.bias && c + k < a.cout ? a.bias[c + k] : 0.0f;
+            uint32_t top[2] = {0u, 0u};
+#pragma unroll
+            for (int r = 0; r < kStepRows; ++r) {
+                const int y = y0 + r;
+                const bool in = y < a.H && x < a.W;
+                const uint32_t el = (uint32_t)((b * a.H + y) * a.W + x) * (uint32_t)a.out_pitch + (uint32_t)c;
+                float v[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) v[k] = acc[r][nt][k] + bias[k];
+                if (f32) {
+                    if (full_nt) {
+                        const u32x4 q = {__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]),
+                                         __float_as_uint(v[3])};
+                        __builtin_amdgcn_raw_buffer_store_b128(q, ro, in ? el * 4u : kOff, 0, 0);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[k]), ro,
+                                                                  in && c + k < a.cout ? (el + k) * 4u : kOff, 0, 0);
+                    }
+                    continue;
+                }
+                uint16_t q[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) q[k] = f2bf(a.relu ? fmaxf(v[k], 0.0f) : v[k]);
+                const uint32_t lo = pack2(q[0], q[1]), hi = pack2(q[2], q[3]);
+                if (full_nt) {
+                    __builtin_amdgcn_raw_buffer_store_b64(u32x2{lo, hi}, ro, in ? el * 2u : kOff, 0, 0);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k)
+                        __builtin_amdgcn_raw_buffer_store_b16(q[k], ro, in && c + k < a.cout ? (el + k) * 2u : kOff, 0,
+                                                              0);
+                }
+                if (a.pool_out) {
+                    if ((r & 1) == 0) {
+                        top[0] = lo;
+                        top[1] = hi;
+                    } else {
+                        // the right-hand pixel's rows come from lane r16 ^ 1; even lanes store
+                        const uint32_t tl[2] = {top[0], top[1]}, bl[2] = {lo, hi};
+                        const uint32_t tr[2] = {lane_xor1(top[0]), lane_xor1(top[1])};
+                        const uint32_t br[2] = {lane_xor1(lo), lane_xor1(hi)};
+                        uint16_t pq[4];
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            const int sft = (k & 1) * 16;
+                            pq[k] = pool4((uint16_t)(tl[k >> 1] >> sft), (uint16_t)(tr[k >> 1] >> sft),
+                                          (uint16_t)(bl[k >> 1] >> sft), (uint16_t)(br[k >> 1] >> sft));
+                        }
+                        const bool st = (r16 & 1) == 0 && y < a.H && x + 1 < a.W;
+                        const uint32_t pe =
+                            (uint32_t)((b * h2 + (y >> 1)) * w2 + (x >> 1)) * (uint32_t)a.out_pitch + (uint32_t)c;
+                        if (full_nt) {
+                            __builtin_amdgcn_raw_buffer_store_b64(u32x2{pack2(pq[0], pq[1]), pack2(pq[2], pq[3])}, rp,
+                                                                  st ? pe * 2u : kOff, 0, 0);
+                        } else {
+#pragma unroll
+                            for (int k = 0; k < 4; ++k)
+                                __builtin_amdgcn_raw_buffer_store_b16(pq[k], rp, st && c + k < a.cout ? (pe + k) * 2u : kOff,
+                                                                      0, 0);
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Weight-gradient partial sums: part[split][tap][ci][co] = sum over this split's tiles of
 // X(p + tap)[ci] * dZ(p)[co]. grid.y enumerates (32-ci block, 32-co block).
 __global__ __launch_bounds__(256) void k_conv_wgrad(WgradArgs a) {
@@ -936,22 +1105,64 @@ int c32_resident() {
     return cached[dev];
 }
 
+// Row-step kernel: rows per step and wavefronts per workgroup (167 VGPRs, 3 per SIMD), and
+// the pixels a launch needs for it. It runs the full-resolution layers with several K chunks:
+// 1080p dec0a (96 -> 32) 196 us on the tile kernel, 153 us here. At half resolution and
+// below it was slower (1080p dec1a 152 -> 183 us, dec2a 101 -> 132 us, bota 20 -> 33 us),
+// also with the next K chunk's rows loaded during the current chunk's MFMAs (245 VGPRs, 8
+// wavefronts: dec1a 170 us, dec0a 160 us) and with 2-row steps (forward pass 1.51-1.60 ms
+// instead of 1.39).
+constexpr int kRowsStep = 4, kRowsWaveCount = 12;
+constexpr unsigned long long kRowsMinPixels = 1ull << 20;
+// Compute units of the current device, queried once per device.
+int cu_count() {
+    static int cached[64] = {};
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (dev < 0 || dev >= 64) dev = 0;
+    if (cached[dev] == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+        // the row-step kernel's weights take up to 8 K chunks of LDS
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(k_conv3x3_rows<kRowsStep, kRowsWaveCount>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRowsMaxLds);
+        cached[dev] = cus;
+    }
+    return cached[dev];
+}
+
 hipError_t launch_conv(const ConvArgs& a, hipStream_t st) {
     const int tiles = a.B * ((a.H + kTH - 1) / kTH) * ((a.W + kTW - 1) / kTW);
     const int ny = a.coutp / kNT;
-    // A layer with one K chunk keeps its weights in LDS: one resident wave of workgroups over
-    // all channel blocks, each taking tiles strided by gridDim.x.
-    if (a.in.c0 + a.in.c1 != kCK) {
+    // Several K chunks: the row-step kernel for launches of 2^20 pixels and more, else one
+    // tile per workgroup. One K chunk: the strip kernel, or, with sources or outputs of
+    // 2 GiB and more (beyond a buffer load's 32-bit offsets), the persistent LDS kernel.
+    const unsigned long long px = (unsigned long long)a.B * a.H * a.W;
+    const unsigned long long out_bytes = px * a.out_pitch * (a.out_f32 ? 4 : 2);
+    const unsigned long long pool_bytes = a.pool_out ? px / 4 * a.out_pitch * 2 : 0;
+    const int cin = a.in.c0 + a.in.c1;
+    if (cin != kCK) {
+        const unsigned long long bytes0 =
+            (unsigned long long)a.B * (a.in.up0 ? (unsigned long long)(a.H >> 1) * (a.W >> 1) : px / a.B) * a.in.c0 * 2;
+        const unsigned long long bytes1 = px * a.in.c1 * 2;
+        const size_t lds = (size_t)(cin / kCK) * 9 * kNT * kCK * 2;
+        if (px >= kRowsMinPixels && lds <= kRowsMaxLds && bytes0 < (1ull << 31) && bytes1 < (1ull << 31) &&
+            out_bytes < (1ull << 31) && pool_bytes < (1ull << 31)) {
+            const int sx = (a.W + kStripW - 1) / kStripW, sy = (a.H + kRowsStep - 1) / kRowsStep;
+            const long long steps = (long long)a.B * sx * sy;
+            const int nx = (int)std::max(1LL, std::min((steps + kRowsWaveCount - 1) / kRowsWaveCount,
+                                                       (long long)std::max(1, cu_count() / ny)));
+            hipLaunchKernelGGL((k_conv3x3_rows<kRowsStep, kRowsWaveCount>), dim3(nx, ny),
+                               dim3(kRowsWaveCount * 64), lds, st, a, sx, sy, (unsigned)bytes0, (unsigned)bytes1,
+                               (unsigned)out_bytes, (unsigned)pool_bytes);
+            return hipGetLastError();
+        }
         hipLaunchKernelGGL(k_conv3x3_tile, dim3(tiles, ny), dim3(256), 0, st, a);
         return hipGetLastError();
     }
     const bool first = a.in.c0 > 0;
     const unsigned long long src_bytes =
-        (unsigned long long)a.B * (first && a.in.up0 ? (unsigned long long)(a.H >> 1) * (a.W >> 1) : (unsigned long long)a.H * a.W) *
-        kCK * 2;
-    const unsigned long long px = (unsigned long long)a.B * a.H * a.W;
-    const unsigned long long out_bytes = px * a.out_pitch * (a.out_f32 ? 4 : 2);
-    const unsigned long long pool_bytes = a.pool_out ? px / 4 * a.out_pitch * 2 : 0;
+        (first && a.in.up0 ? (unsigned long long)a.B * (a.H >> 1) * (a.W >> 1) : px) * kCK * 2;
     if (src_bytes < (1ull << 31) && out_bytes < (1ull << 31) && pool_bytes < (1ull << 31)) {
         const int sx = (a.W + kStripW - 1) / kStripW, sy = (a.H + kC32Rows - 1) / kC32Rows;
         const long long strips = (long long)a.B * sx * sy;

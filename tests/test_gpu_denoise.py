@@ -74,6 +74,40 @@ def test_forward_matches_oracle(DN, B, H, W):
         assert r < ACT_REL, (l, r)
 
 
+def test_full_resolution_layers_match_oracle(DN):
+    """A 1024x1024 frame (2^20 pixels, where the full-resolution layers with several K
+    chunks take the row-step kernel and the 32-channel ones the strip kernel): each checked
+    layer is recomputed by the oracle's convolution from the device's own inputs to that
+    layer, so the bar is one layer's rounding: the bf16 outputs equal except where the f64
+    and f32 sums round to neighbouring bf16 values."""
+    import torch
+    dn = DN.Denoiser(seed=5)
+    p = dn.get_params()
+    p[:] += np.random.default_rng(2).standard_normal(p.size).astype(np.float32) * 1e-3  # nonzero biases
+    dn.set_params(p)
+    H = W = 1024
+    feat, target = _batch(4, 1, H, W)
+    dn.gradients(torch.from_numpy(feat).cuda(), torch.from_numpy(target).cuda())  # runs the forward pass
+    net = D.UNet(p, bf16_emulation=True)
+
+    def act(l):
+        lv, cout = D.LAYERS[l][0], D.LAYERS[l][5]
+        return dn.activation(l, (1, H >> lv, W >> lv, 3 if l == D.OUT else cout)).astype(np.float64)
+
+    acts = {l: act(l) for l in (0, 1, 11, 12, 13)}
+    # enc0b <- enc0a, enc1a <- pool(enc0b), dec0a <- [up(dec1b), enc0b], dec0b <- dec0a
+    cases = {1: acts[0], 2: D.bf16(D.pool(acts[1])).astype(np.float64),
+             12: np.concatenate([D.up(acts[11]), acts[1]], axis=3), 13: acts[12]}
+    for l, x in cases.items():
+        z = D.conv(x, net.w[l], net.b[l])
+        want = D.bf16(np.maximum(z, 0.0)).astype(np.float64)
+        got = act(l) if l != 2 else dn.activation(2, want.shape).astype(np.float64)
+        same = float(np.mean(got == want))
+        r = _rel(got, want)
+        print(f"layer {l}: rel {r:.2e} equal {same * 100:.4f}%")
+        assert r < 2e-3 and same > 0.99, (l, r, same)
+
+
 def test_gradients_match_oracle(DN):
     import torch
     dn = DN.Denoiser(seed=12)

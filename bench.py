@@ -340,16 +340,16 @@ def counter_summary(pmc, kernel, ms_per_launch):
 
 
 def mfma_summary(pmc):
-    """MFMA busy share of the denoiser's convolutions (both conv kernels: SQ_VALU_MFMA_BUSY_CYCLES
+    """MFMA busy share of the denoiser's convolutions (every conv kernel: SQ_VALU_MFMA_BUSY_CYCLES
     per SIMD over the dispatches' GPU cycles), and the bf16 MFMA work they did."""
     sq = pmc["sq"]
-    match = lambda n: re.search(r"::k_conv3x3(_tile)?[<(]", n) is not None  # noqa: E731
+    match = lambda n: re.search(r"::k_conv3x3(_tile|_c32|_rows)?[<(]", n) is not None  # noqa: E731
     busy = _per_dispatch(sq, "SQ_VALU_MFMA_BUSY_CYCLES", match)
     gui = _per_dispatch(sq, "GRBM_GUI_ACTIVE", match)
     mops = _per_dispatch(sq, "SQ_INSTS_VALU_MFMA_MOPS_BF16", match)
     if not busy or not gui:
         return None
-    return {"kernel": "dn::k_conv3x3 + dn::k_conv3x3_tile (15 layers)", "launches": len(busy),
+    return {"kernel": "dn::k_conv3x3_c32, _rows and _tile (the 15 layers)", "launches": len(busy),
             "mfma_busy": round(sum(busy) / max(sum(gui) / N_XCC * N_SIMD, 1.0), 4),
             "bf16_mops_per_forward": round(sum(mops), 1) if mops else None,
             "counter": "SQ_VALU_MFMA_BUSY_CYCLES / (1024 SIMDs x GRBM_GUI_ACTIVE / 8 XCCs), summed over the launches"}

@@ -51,7 +51,7 @@ def test_parameter_layout_matches_oracle(DN):
         assert (cin, cp) == (D.LAYERS[l][4], D.coutp(l)) and (wo, bo) == D.OFFS[l]
 
 
-@pytest.mark.parametrize("B,H,W", [(1, 16, 32), (2, 20, 44)])
+@pytest.mark.parametrize("B,H,W", [(1, 16, 32), (2, 20, 44), (1, 12, 40)])  # 40: a 16-pixel strip half outside
 def test_forward_matches_oracle(DN, B, H, W):
     import torch
     dn = DN.Denoiser(seed=11)

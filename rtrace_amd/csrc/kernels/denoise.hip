@@ -33,6 +33,13 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
     return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// two floats rounded to bf16 (nearest even) and packed, low one first: gfx950's
+// v_cvt_pk_bf16_f32, the same rounding as f2bf
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    const bf16x2 v = {(__bf16)lo, (__bf16)hi};
+    return __builtin_bit_cast(uint32_t, v);
+}
 
 constexpr int kTH = 8, kTW = 16;                   // output tile (pixels)
 
@@ -185,7 +192,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
                     a.out_f32[o] = v;
                 } else {
                     if (a.relu) v = fmaxf(v, 0.0f);
-                    a.out_bf16[o] = f2bf(v);
+                    a.out_bf16[o] = (uint16_t)f2bf2(v, 0.0f);
                 }
             }
         }
@@ -205,7 +212,7 @@ __global__ __launch_bounds__(256) void k_conv3x3_tile(ConvArgs a) {
                     auto q = [&](int mt, int i) {
                         float v = acc[mt][nt][i] + bias;
                         if (a.relu) v = fmaxf(v, 0.0f);
-                        return f2bf(v);
+                        return (uint16_t)f2bf2(v, 0.0f);
                     };
                     const size_t o = (((size_t)b * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1)) * a.out_pitch + co;
                     a.pool_out[o] = pool4(q(0, 2 * p), q(0, 2 * p + 1), q(1, 2 * p), q(1, 2 * p + 1));
@@ -446,7 +453,13 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr uint32_t kOff = 0x80000000u;  // a buffer offset past every buffer's end
 
-__device__ __forceinline__ uint32_t pack2(uint16_t lo, uint16_t hi) { return (uint32_t)lo | ((uint32_t)hi << 16); }
+// 2x2 average pools of two channel pairs (top-left, top-right, bottom-left, bottom-right
+// packed bf16), as pool4, packed
+__device__ __forceinline__ uint32_t pool4x2(uint32_t a, uint32_t b, uint32_t c, uint32_t d) {
+    auto lo = [](uint32_t v) { return __uint_as_float(v << 16); };
+    auto hi = [](uint32_t v) { return __uint_as_float(v & 0xffff0000u); };
+    return f2bf2(((lo(a) + lo(b)) + (lo(c) + lo(d))) * 0.25f, ((hi(a) + hi(b)) + (hi(c) + hi(d))) * 0.25f);
+}
 // the value of lane ^ 1 (DPP quad_perm [1, 0, 3, 2])
 __device__ __forceinline__ uint32_t lane_xor1(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
@@ -554,17 +567,19 @@ __global__ __launch_bounds__(256) void k_conv3x3_c32(ConvArgs a, int strips_x,
                         }
                         continue;
                     }
-                    uint16_t q[4];
+                    if (a.relu) {
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) q[k] = f2bf(a.relu ? fmaxf(v[k], 0.0f) : v[k]);
-                    const uint32_t lo = pack2(q[0], q[1]), hi = pack2(q[2], q[3]);
+                        for (int k = 0; k < 4; ++k) v[k] = fmaxf(v[k], 0.0f);
+                    }
+                    const uint32_t lo = f2bf2(v[0], v[1]), hi = f2bf2(v[2], v[3]);
                     if (full[nt]) {
                         __builtin_amdgcn_raw_buffer_store_b64(u32x2{lo, hi}, ro, x < a.W ? el * 2u : kOff, 0, 0);
                     } else {
+                        const uint32_t qw[2] = {lo, hi};
 #pragma unroll
                         for (int k = 0; k < 4; ++k)
-                            __builtin_amdgcn_raw_buffer_store_b16(q[k], ro, x < a.W && c + k < a.cout ? (el + k) * 2u : kOff,
-                                                                  0, 0);
+                            __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(qw[k >> 1] >> ((k & 1) * 16)), ro,
+                                                                  x < a.W && c + k < a.cout ? (el + k) * 2u : kOff, 0, 0);
                     }
                     if (a.pool_out) {
                         if ((y & 1) == 0) {
@@ -572,29 +587,20 @@ __global__ __launch_bounds__(256) void k_conv3x3_c32(ConvArgs a, int strips_x,
                             top[nt][1] = hi;
                         } else {
                             // the right-hand pixel's rows come from lane r16 ^ 1; even lanes store
-                            const uint32_t tr0 = lane_xor1(top[nt][0]), tr1 = lane_xor1(top[nt][1]);
-                            const uint32_t br0 = lane_xor1(lo), br1 = lane_xor1(hi);
-                            const uint32_t tl[2] = {top[nt][0], top[nt][1]}, bl[2] = {lo, hi}, tr[2] = {tr0, tr1},
-                                           br[2] = {br0, br1};
-                            uint16_t pq[4];
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                const int sft = (k & 1) * 16;
-                                pq[k] = pool4((uint16_t)(tl[k >> 1] >> sft), (uint16_t)(tr[k >> 1] >> sft),
-                                              (uint16_t)(bl[k >> 1] >> sft), (uint16_t)(br[k >> 1] >> sft));
-                            }
+                            const uint32_t p0 = pool4x2(top[nt][0], lane_xor1(top[nt][0]), lo, lane_xor1(lo));
+                            const uint32_t p1 = pool4x2(top[nt][1], lane_xor1(top[nt][1]), hi, lane_xor1(hi));
                             const bool st = (r16 & 1) == 0 && x + 1 < a.W;
                             const uint32_t pe =
                                 ((uint32_t)((b * (a.H >> 1) + (y >> 1)) * (a.W >> 1) + (x >> 1))) * (uint32_t)a.out_pitch +
                                 (uint32_t)c;
                             if (full[nt]) {
-                                __builtin_amdgcn_raw_buffer_store_b64(u32x2{pack2(pq[0], pq[1]), pack2(pq[2], pq[3])}, rp,
-                                                                      st ? pe * 2u : kOff, 0, 0);
+                                __builtin_amdgcn_raw_buffer_store_b64(u32x2{p0, p1}, rp, st ? pe * 2u : kOff, 0, 0);
                             } else {
+                                const uint32_t pw[2] = {p0, p1};
 #pragma unroll
                                 for (int k = 0; k < 4; ++k)
-                                    __builtin_amdgcn_raw_buffer_store_b16(pq[k], rp, st && c + k < a.cout ? (pe + k) * 2u : kOff,
-                                                                          0, 0);
+                                    __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(pw[k >> 1] >> ((k & 1) * 16)), rp,
+                                                                          st && c + k < a.cout ? (pe + k) * 2u : kOff, 0, 0);
                             }
                         }
                     }
@@ -726,17 +732,19 @@ __global__ __launch_bounds__(kRowsWaves * 64) void k_conv3x3_rows(ConvArgs a, in
                     }
                     continue;
                 }
-                uint16_t q[4];
+                if (a.relu) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) q[k] = f2bf(a.relu ? fmaxf(v[k], 0.0f) : v[k]);
-                const uint32_t lo = pack2(q[0], q[1]), hi = pack2(q[2], q[3]);
+                    for (int k = 0; k < 4; ++k) v[k] = fmaxf(v[k], 0.0f);
+                }
+                const uint32_t lo = f2bf2(v[0], v[1]), hi = f2bf2(v[2], v[3]);
                 if (full_nt) {
                     __builtin_amdgcn_raw_buffer_store_b64(u32x2{lo, hi}, ro, in ? el * 2u : kOff, 0, 0);
                 } else {
+                    const uint32_t qw[2] = {lo, hi};
 #pragma unroll
                     for (int k = 0; k < 4; ++k)
-                        __builtin_amdgcn_raw_buffer_store_b16(q[k], ro, in && c + k < a.cout ? (el + k) * 2u : kOff, 0,
-                                                              0);
+                        __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(qw[k >> 1] >> ((k & 1) * 16)), ro,
+                                                              in && c + k < a.cout ? (el + k) * 2u : kOff, 0, 0);
                 }
                 if (a.pool_out) {
                     if ((r & 1) == 0) {
@@ -744,27 +752,19 @@ __global__ __launch_bounds__(kRowsWaves * 64) void k_conv3x3_rows(ConvArgs a, in
                         top[1] = hi;
                     } else {
                         // the right-hand pixel's rows come from lane r16 ^ 1; even lanes store
-                        const uint32_t tl[2] = {top[0], top[1]}, bl[2] = {lo, hi};
-                        const uint32_t tr[2] = {lane_xor1(top[0]), lane_xor1(top[1])};
-                        const uint32_t br[2] = {lane_xor1(lo), lane_xor1(hi)};
-                        uint16_t pq[4];
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int sft = (k & 1) * 16;
-                            pq[k] = pool4((uint16_t)(tl[k >> 1] >> sft), (uint16_t)(tr[k >> 1] >> sft),
-                                          (uint16_t)(bl[k >> 1] >> sft), (uint16_t)(br[k >> 1] >> sft));
-                        }
+                        const uint32_t p0 = pool4x2(top[0], lane_xor1(top[0]), lo, lane_xor1(lo));
+                        const uint32_t p1 = pool4x2(top[1], lane_xor1(top[1]), hi, lane_xor1(hi));
                         const bool st = (r16 & 1) == 0 && y < a.H && x + 1 < a.W;
                         const uint32_t pe =
                             (uint32_t)((b * h2 + (y >> 1)) * w2 + (x >> 1)) * (uint32_t)a.out_pitch + (uint32_t)c;
                         if (full_nt) {
-                            __builtin_amdgcn_raw_buffer_store_b64(u32x2{pack2(pq[0], pq[1]), pack2(pq[2], pq[3])}, rp,
-                                                                  st ? pe * 2u : kOff, 0, 0);
+                            __builtin_amdgcn_raw_buffer_store_b64(u32x2{p0, p1}, rp, st ? pe * 2u : kOff, 0, 0);
                         } else {
+                            const uint32_t pw[2] = {p0, p1};
 #pragma unroll
                             for (int k = 0; k < 4; ++k)
-                                __builtin_amdgcn_raw_buffer_store_b16(pq[k], rp, st && c + k < a.cout ? (pe + k) * 2u : kOff,
-                                                                      0, 0);
+                                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(pw[k >> 1] >> ((k & 1) * 16)), rp,
+                                                                      st && c + k < a.cout ? (pe + k) * 2u : kOff, 0, 0);
                         }
                     }
                 }
